@@ -1,0 +1,256 @@
+"""ctypes front end of the FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` (built from femx_oracle.c by oracle/Makefile).
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module; the product package never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OxMesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_elems", C.c_int), ("npe", C.c_int), ("coords", _dp), ("conn", _ip)]
+
+
+class OxBC(C.Structure):
+    _fields_ = [("n_vel", C.c_int), ("vel_node", _ip), ("vel_dir", _ip), ("vel_row_off", _ip), ("vel_rows", _dp),
+                ("n_dir", C.c_int), ("dir_node", _ip), ("dir_val", _dp),
+                ("n_pres", C.c_int), ("pres_node", _ip), ("pres_val", _dp)]
+
+
+class OxVMSParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("inv_eps", C.c_double), ("c1", C.c_double),
+                ("c2", C.c_double), ("dt", C.c_double), ("dual_step", C.c_double), ("tot_dual_steps", C.c_int)]
+
+
+class OxSUPGParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("alpha", C.c_double), ("cI", C.c_double),
+                ("dt", C.c_double), ("use_euler", C.c_int), ("use_bhat", C.c_int), ("k1", C.c_int),
+                ("k2", C.c_int), ("k3", C.c_int), ("k4", C.c_int), ("fixed_dofs", C.c_int),
+                ("exit_condition", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.ox_vms_create.restype = C.c_void_p
+        L.ox_vms_create.argtypes = [C.POINTER(OxMesh), C.POINTER(OxBC), C.POINTER(OxVMSParams)]
+        L.ox_vms_destroy.argtypes = [C.c_void_p]
+        L.ox_vms_set_ic.argtypes = [C.c_void_p, _dp]
+        L.ox_vms_set_sdus.argtypes = [C.c_void_p, _dp]
+        L.ox_vms_step.argtypes = [C.c_void_p]
+        L.ox_vms_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ox_vms_element_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.ox_supg_create.restype = C.c_void_p
+        L.ox_supg_create.argtypes = [C.POINTER(OxMesh), C.POINTER(OxBC), C.POINTER(OxSUPGParams)]
+        L.ox_supg_destroy.argtypes = [C.c_void_p]
+        L.ox_supg_set_ic.argtypes = [C.c_void_p, _dp]
+        L.ox_supg_step.argtypes = [C.c_void_p]
+        L.ox_supg_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ox_supg_set_elem_range.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ox_supg_lumped_mass.argtypes = [C.POINTER(OxMesh), C.c_double, _dp]
+        L.ox_rule_second_order.argtypes = [C.c_int, _dp, _dp]
+        L.ox_interp_table.argtypes = [C.c_double, _dp, C.c_int, _dp]
+        L.ox_shape.argtypes = [C.c_int, _dp, _dp]
+        L.ox_global_deriv.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.ox_geom_matrix.argtypes = [C.c_int, _dp, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class _Keep:
+    """Holds numpy arrays alive while C structs point into them."""
+
+    def __init__(self, mesh, bc):
+        self.coords = np.ascontiguousarray(mesh.coords, np.float64)
+        self.conn = np.ascontiguousarray(mesh.conn, np.int32)
+        self.m = OxMesh(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(self.coords), _p(self.conn, _ip))
+        self.arrs = [np.ascontiguousarray(x) for x in (bc.vel_node, bc.vel_dir, bc.vel_row_off, bc.vel_rows,
+                                                        bc.dir_node, bc.dir_val, bc.pres_node, bc.pres_val)]
+        a = self.arrs
+        self.b = OxBC(bc.n_vel, _p(a[0], _ip), _p(a[1], _ip), _p(a[2], _ip), _p(a[3]),
+                      len(bc.dir_node), _p(a[4], _ip), _p(a[5]), len(bc.pres_node), _p(a[6], _ip), _p(a[7]))
+
+
+class VMSOracle:
+    """FP64 restatement of femVMSExplicitFluidSolver::solve, step by step."""
+
+    GET = dict(sol_n=(0, "N4"), sol_tau=(1, "N4"), sdus=(2, "E12"), sgps=(3, "E4"), nd_mass=(4, "N"),
+               volumes=(5, "E"), char_length=(6, "E"), DNs=(7, "E12"), elmTaus=(8, "E2"), el_u_res=(9, "E3"),
+               el_p_res=(10, "E"), el_u_sdus=(11, "E3"), el_p_sdus=(12, "E"), scalars=(13, "S6"))
+
+    def __init__(self, mesh, bc, rho, mu, inv_eps, c1, c2, dt, dual_step=None, tot_dual_steps=0):
+        L = lib()
+        self.N, self.E = mesh.n_nodes, mesh.n_elems
+        k = _Keep(mesh, bc)
+        p = OxVMSParams(rho, mu, inv_eps, c1, c2, dt, dt if dual_step is None else dual_step, tot_dual_steps)
+        self._h = L.ox_vms_create(C.byref(k.m), C.byref(k.b), C.byref(p))
+        if not self._h:
+            raise ValueError("ox_vms_create failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ox_vms_destroy(self._h)
+            self._h = None
+
+    def set_ic(self, sol):
+        sol = np.ascontiguousarray(sol, np.float64).reshape(self.N, 4)
+        if lib().ox_vms_set_ic(self._h, _p(sol)) != 0:
+            raise ValueError("BC table error")
+
+    def set_sdus(self, sdus):
+        sdus = np.ascontiguousarray(sdus, np.float64).reshape(12 * self.E)
+        lib().ox_vms_set_sdus(self._h, _p(sdus))
+
+    def step(self, n=1):
+        for _ in range(n):
+            if lib().ox_vms_step(self._h) != 0:
+                raise ValueError("BC table error")
+
+    def get(self, name):
+        which, shape = self.GET[name]
+        dims = {"N4": (self.N, 4), "E12": (12, self.E), "E4": (4, self.E), "N": (self.N,), "E": (self.E,),
+                "E2": (2, self.E), "E3": (self.E, 3), "S6": (6,)}[shape]
+        out = np.zeros(dims)
+        lib().ox_vms_get(self._h, which, _p(out))
+        return out
+
+    def element_residual(self, e, cur, prev):
+        cur = np.ascontiguousarray(cur, np.float64)
+        prev = np.ascontiguousarray(prev, np.float64)
+        out = np.zeros(16)
+        lib().ox_vms_element_residual(self._h, e, _p(cur), _p(prev), _p(out))
+        return out.reshape(4, 4)
+
+
+_FIXED = {"NONE": 0, "X": 1, "Y": 2, "Z": 3, "XY": 4, "YZ": 5, "XZ": 6}
+
+
+class SUPGOracle:
+    """FP64 restatement of femSUPGExplicitFluidSolver::solve, step by step."""
+
+    GET = dict(sol_n=(0, "N3"), sol_p=(1, "N3"), l_M=(2, "N3"), el_lam=(3, "E"), el_regime=(4, "E"),
+               pres_n=(5, "E"), k_sol_p=(6, "E24"), k_sol_pp=(7, "E24"), log=(8, "S19"), scalars=(9, "S2"))
+
+    def __init__(self, mesh, bc, rho, mu, alpha, cI, dt, exopts=(0, 1, 1, 1, 1, 1), fixed_dofs="NONE",
+                 exit_condition=1.0e5):
+        L = lib()
+        self.N, self.E = mesh.n_nodes, mesh.n_elems
+        k = _Keep(mesh, bc)
+        e = [int(bool(x)) for x in exopts]
+        p = OxSUPGParams(rho, mu, alpha, cI, dt, e[0], e[1], e[2], e[3], e[4], e[5], _FIXED[fixed_dofs],
+                         exit_condition)
+        self._h = L.ox_supg_create(C.byref(k.m), C.byref(k.b), C.byref(p))
+        if not self._h:
+            raise ValueError("ox_supg_create failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ox_supg_destroy(self._h)
+            self._h = None
+
+    def set_ic(self, sol):
+        sol = np.ascontiguousarray(np.asarray(sol, np.float64)[:, :3]).reshape(self.N, 3)
+        if lib().ox_supg_set_ic(self._h, _p(sol)) != 0:
+            raise ValueError("BC table error")
+
+    def set_elem_range(self, e0, e1):
+        lib().ox_supg_set_elem_range(self._h, e0, e1)
+
+    def step(self, n=1):
+        """Returns 1 if the exit condition fired on the last step."""
+        r = 0
+        for _ in range(n):
+            r = lib().ox_supg_step(self._h)
+            if r < 0:
+                raise ValueError("BC / option error")
+            if r == 1:
+                return 1
+        return r
+
+    def get(self, name):
+        which, shape = self.GET[name]
+        dims = {"N3": (self.N, 3), "E": (self.E,), "E24": (24, self.E), "S19": (19,), "S2": (2,)}[shape]
+        out = np.zeros(dims)
+        lib().ox_supg_get(self._h, which, _p(out))
+        return out
+
+
+def lumped_mass(mesh, rho):
+    k = _Keep(mesh, __import__("femorph_amd.mesh", fromlist=["BoundaryConditions"]).BoundaryConditions())
+    out = np.zeros(3 * mesh.n_nodes)
+    lib().ox_supg_lumped_mass(C.byref(k.m), rho, _p(out))
+    return out
+
+
+def rule_second_order(npe):
+    c = np.zeros(32)
+    w = np.zeros(8)
+    n = lib().ox_rule_second_order(npe, _p(c), _p(w))
+    return c.reshape(8, 4)[:n], w[:n]
+
+
+def interp_table(t, rows):
+    rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 4)
+    out = np.zeros(3)
+    r = lib().ox_interp_table(t, _p(rows), len(rows), _p(out))
+    if r != 0:
+        raise ValueError("time precedes table")
+    return out
+
+
+def shape(npe, xi):
+    xi = np.ascontiguousarray(xi, np.float64)
+    N = np.zeros(npe)
+    lib().ox_shape(npe, _p(xi), _p(N))
+    return N
+
+
+def global_deriv(npe, xel, xi):
+    xel = np.ascontiguousarray(xel, np.float64).reshape(npe, 3)
+    xi = np.ascontiguousarray(xi, np.float64)
+    det = np.zeros(1)
+    DN = np.zeros((npe, 3))
+    lib().ox_global_deriv(npe, _p(xel), _p(xi), _p(det), _p(DN))
+    return float(det[0]), DN
+
+
+def geom_matrix(npe, xel, xi):
+    xel = np.ascontiguousarray(xel, np.float64).reshape(npe, 3)
+    xi = np.ascontiguousarray(xi, np.float64)
+    G = np.zeros(9)
+    lib().ox_geom_matrix(npe, _p(xel), _p(xi), _p(G))
+    return G.reshape(3, 3)
+
+
+def ref_rules_lib():
+    """The reference's own femIntegrationRule (oracle/_ref), or None."""
+    path = os.path.join(_HERE, "_ref", "libref_rules.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ref_rule.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+    return L

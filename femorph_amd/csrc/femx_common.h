@@ -1,0 +1,190 @@
+// femx_common.h -- shared host/device infrastructure of the femx library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/femx.h"
+
+namespace femx {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+femx_status hip_fail(hipError_t err, const char *what);
+
+#define FEMX_HIP(call)                                 \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return femx::hip_fail(_e, #call); \
+  } while (0)
+
+// --------------------------------------------------------- device buffers
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  femx_status alloc(size_t count) {
+    free();
+    n = count;
+    if (count == 0) return FEMX_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+      return FEMX_ERR_NOMEM;
+    }
+    return FEMX_OK;
+  }
+  femx_status upload(const T *h, size_t count, hipStream_t st) {
+    femx_status s = alloc(count);
+    if (s != FEMX_OK || count == 0) return s;
+    (void)st;
+    FEMX_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return FEMX_OK;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// -------------------------------------------------------------- the mesh
+// Device layout (one MI355X holds meshes of tens of millions of elements):
+//   xyz   double4[N]      node coordinates (x, y, z, 0), one 32-byte load
+//   conn  int[E*npe]      element connectivity, element-major, so one tet4
+//                         element is one 16-byte int4 load (hexa8: two)
+//   g_off int[N+1]        node -> incident (element, local node) slots
+//   g_slot int[npe*E]     slot = e*npe + a, ascending e (the reference's
+//                         element-order scatter, src/femVMSExplicitFluidSolver.cpp:394-398)
+struct Mesh {
+  int N = 0, E = 0, npe = 0, max_deg = 0;
+  std::vector<double> h_coords;
+  std::vector<int> h_conn;
+  DevBuf<double4> xyz;
+  DevBuf<int> conn;
+  DevBuf<int> g_off, g_slot;
+  int device = 0;
+};
+
+// --------------------------------------------------- boundary conditions
+// One record per BC-touched node, merging the reference's three lists with
+// "last entry wins" inside a list and the order velocity -> pressure ->
+// walls across lists (src/femVMSExplicitFluidSolver.cpp:1180-1197).
+struct BCRecord {
+  int vel;       // index into the velocity tables or -1
+  int vdir;      // direction code 0..6
+  int has_pres;  // 1 if a NODEPRES entry touches the node
+  int has_dir;   // 1 if a NODEDIRBC entry touches the node
+  double pres;
+  double dval[3];
+};
+
+struct BCDev {
+  int n_rec = 0;
+  DevBuf<int> node_rec;      // [N] record index or -1
+  DevBuf<BCRecord> rec;      // [n_rec]
+  DevBuf<int> row_off;       // [n_vel+1]
+  DevBuf<double4> rows;      // vx vy vz t
+  DevBuf<int> err;           // [1] set when a table does not cover t
+  std::vector<int> h_rec_node;
+  femx_status build(const femx_bc_desc *bc, int N, bool with_pressure, hipStream_t st);
+};
+
+// femUtils::interpTableData (src/femUtils.cpp:830-871, time_idx = 3).
+__device__ inline bool interp_table(double t, const double4 *rows, int nrows, double out[3]) {
+  const double4 last = rows[nrows - 1];
+  if (t >= last.w) {
+    out[0] = last.x;
+    out[1] = last.y;
+    out[2] = last.z;
+    return true;
+  }
+  for (int c = 1; c < nrows; c++) {
+    const double4 r1 = rows[c - 1], r2 = rows[c];
+    if (t >= r1.w && t < r2.w) {
+      const double dt = r2.w - r1.w, s = t - r1.w;
+      out[0] = r1.x + ((r2.x - r1.x) / dt) * s;
+      out[1] = r1.y + ((r2.y - r1.y) / dt) * s;
+      out[2] = r1.z + ((r2.z - r1.z) / dt) * s;
+      return true;
+    }
+  }
+  return false;
+}
+
+// setNodeVelocity -> apply_zeroTraction -> setDirichletBC on one node
+// (src/femModel.cpp:4581-4643, src/femVMSExplicitFluidSolver.cpp:648-653).
+__device__ inline void apply_bc(const BCRecord &r, const int *row_off, const double4 *rows, double t, double &u,
+                                double &v, double &w, double *p, int *err) {
+  if (r.vel >= 0) {
+    double val[3];
+    if (!interp_table(t, rows + row_off[r.vel], row_off[r.vel + 1] - row_off[r.vel], val)) {
+      atomicOr(err, 1);
+    } else {
+      const int d = r.vdir;
+      if (d == 0 || d == 3 || d == 5 || d == 6) u = val[0];
+      if (d == 1 || d == 3 || d == 4 || d == 6) v = val[1];
+      if (d == 2 || d == 4 || d == 5 || d == 6) w = val[2];
+    }
+  }
+  if (p && r.has_pres) *p = r.pres;
+  if (r.has_dir) {
+    u = r.dval[0];
+    v = r.dval[1];
+    w = r.dval[2];
+  }
+}
+
+// ---------------------------------------------------- per-kernel timing
+struct KernelTimer {
+  bool enabled = false;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  struct Acc {
+    std::string name;
+    int launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+  };
+  std::vector<Acc> acc;
+  int kind(const char *name, double bytes);
+  void begin(int k, hipStream_t st, hipEvent_t *a);
+  void end(int k, hipStream_t st, hipEvent_t a);
+  void resolve();
+  int export_to(femx_kernel_time *out, int max_entries);
+  ~KernelTimer();
+};
+
+struct TimedLaunch {
+  KernelTimer &t;
+  int k;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  TimedLaunch(KernelTimer &t_, int k_, hipStream_t st_) : t(t_), k(k_), st(st_) {
+    if (t.enabled) t.begin(k, st, &a);
+  }
+  ~TimedLaunch() {
+    if (t.enabled) t.end(k, st, a);
+  }
+};
+
+int grid_for(long n, int block);
+
+}  // namespace femx
+
+struct femx_mesh {
+  femx::Mesh m;
+};

@@ -1,0 +1,174 @@
+/*
+ * femx.h -- C ABI of the MI355X-native explicit stabilized fluid time step.
+ *
+ * Drop-in for the hot path of feMorph's two explicit solvers:
+ *   femVMSExplicitFluidSolver::solve(femModel*)  src/femVMSExplicitFluidSolver.h:9-14,
+ *                                                src/femVMSExplicitFluidSolver.cpp:842-1334
+ *   femSUPGExplicitFluidSolver::solve(femModel*) src/femSUPGExplicitFluidSolver.h:9-14,
+ *                                                src/femSUPGExplicitFluidSolver.cpp:794-1735
+ * The reference has no FFI of its own: its boundary is the C++ class API
+ * above, called from solveExplicitNS (src/main.cpp:788-805).  These entry
+ * points are what those classes bind to (see the drop-in classes in
+ * femorph_amd/csrc/host/): POD structs, explicit sizes, caller-owned host
+ * buffers, device memory owned by handles, int status codes, no exceptions.
+ *
+ * Threading: a handle is used by one host thread at a time; every handle
+ * owns one HIP stream (default priority) and all of its work is ordered on
+ * it.  Calls that return host data synchronise that stream.
+ */
+#ifndef FEMX_H
+#define FEMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int femx_status;
+enum {
+  FEMX_OK = 0,
+  FEMX_ERR_ARG = 1,          /* invalid argument or size                     */
+  FEMX_ERR_HIP = 2,          /* HIP runtime error (message: femx_last_error)  */
+  FEMX_ERR_BC = 3,           /* BC time table does not cover t (reference throws femException) */
+  FEMX_ERR_NOMEM = 4,        /* device allocation failed                      */
+  FEMX_ERR_UNSUPPORTED = 5,  /* option the reference cannot run (e.g. B-hat on tet4) */
+  FEMX_EXIT_CONDITION = 6,   /* SUPG max |u| exceeded EXITCOND (reference: exit(-1)) */
+  FEMX_ERR_NO_DEVICE = 7     /* no HIP device visible                          */
+};
+
+enum { FEMX_FP64 = 0, FEMX_MIXED = 1 }; /* MIXED: FP32 element math, FP64 state/accumulation */
+
+/* Mesh as the reference's femModel holds it: nodeList[i]->coords and
+ * elementList[e]->elementConnections (0-based), src/femModel.h:24-25. */
+typedef struct {
+  int n_nodes;
+  int n_elems;
+  int npe;               /* 4 = tet4, 8 = hexa8 */
+  const double *coords;  /* [n_nodes][3] */
+  const int *conn;       /* [n_elems][npe] */
+} femx_mesh_desc;
+
+/* Boundary conditions as femModel holds them after
+ * postProcessNodeVelocities (src/femModel.cpp:4549-4576):
+ *   velNodesID / velNodesDir / velNodesTimeVals  -> vel_*
+ *   diricheletBCNode / diricheletBCValues        -> dir_*
+ *   presNodesID / presNodesValue                 -> pres_*  (VMS only)
+ * Applied in the reference's order: velocity tables, pressure, walls. */
+typedef struct {
+  int n_vel;
+  const int *vel_node;     /* [n_vel] unique node ids                     */
+  const int *vel_dir;      /* [n_vel] 0..6 = X,Y,Z,XY,YZ,XZ,XYZ           */
+  const int *vel_row_off;  /* [n_vel+1] offsets into vel_rows             */
+  const double *vel_rows;  /* [rows][4] = vx vy vz t, per node, ascending t */
+  int n_dir;
+  const int *dir_node;     /* [n_dir]    */
+  const double *dir_val;   /* [n_dir][3] */
+  int n_pres;
+  const int *pres_node;    /* [n_pres]   */
+  const double *pres_val;  /* [n_pres]   */
+} femx_bc_desc;
+
+typedef struct femx_mesh femx_mesh;
+typedef struct femx_vms femx_vms;
+typedef struct femx_supg femx_supg;
+
+/* Text of the last error on this thread ("" if none). */
+const char *femx_last_error(void);
+/* Number of visible HIP devices (0 on a host without GPU). */
+int femx_device_count(void);
+
+/* Upload a mesh to the current device: SoA/AoS device layout, node->element
+ * gather lists, locality-ordered element blocks.  The descriptor's arrays
+ * are copied; the caller keeps ownership. Replaces the per-solve pointer
+ * chasing over model->nodeList / elementList (src/femModel.h:24-25). */
+femx_status femx_mesh_upload(const femx_mesh_desc *desc, femx_mesh **out);
+femx_status femx_mesh_free(femx_mesh *mesh);
+
+/* ------------------------------------------------------------------ VMS */
+/* vmsProps[0..4] = rho, mu, invEps, c1, c2 (src/femModel.h:32, read at
+ * src/femVMSExplicitFluidSolver.cpp:187-188,447-450); timeStep, dual_step,
+ * tot_dual_steps (src/femModel.h:66-75). */
+typedef struct {
+  double rho, mu, inv_eps, c1, c2;
+  double dt;
+  double dual_step;
+  int tot_dual_steps;
+  int precision;         /* FEMX_FP64 or FEMX_MIXED */
+} femx_vms_params;
+
+/* Replaces the set-up part of femVMSExplicitFluidSolver::solve
+ * (src/femVMSExplicitFluidSolver.cpp:862-952): state allocation and
+ * initial_assemble (:21-91, lumped nodal mass, volumes, lengths). */
+femx_status femx_vms_create(femx_mesh *mesh, const femx_bc_desc *bc, const femx_vms_params *p, femx_vms **out);
+femx_status femx_vms_destroy(femx_vms *s);
+/* Initial-condition hook (the reference always starts from zeros,
+ * :864-898): sol [N][4] = u v w p before the t=0 BCs; the t=0 BCs are then
+ * applied as at :883-889.  sdus [12][E] in the reference layout
+ * sdus[(g*3+i)*E+e] may be NULL (zero subscales, as in the reference). */
+femx_status femx_vms_set_state(femx_vms *s, const double *sol, const double *sdus);
+/* nsteps iterations of the time loop (:1025-1235), enqueued on the handle's
+ * stream without host synchronisation. */
+femx_status femx_vms_step(femx_vms *s, int nsteps);
+/* Current state: sol [N][4] (the reference's sol_n after update_solution),
+ * sdus [12][E] reference layout, time.  Any pointer may be NULL. */
+femx_status femx_vms_get_state(femx_vms *s, double *sol, double *sdus, double *time);
+/* Lumped nodal mass nd_mass [N] (:88-90). */
+femx_status femx_vms_get_mass(femx_vms *s, double *nd_mass);
+/* max over nodes of |u| of the pre-BC state of the last dual iteration and
+ * the CFL number printed by the reference (:1202-1215). */
+femx_status femx_vms_get_cfl(femx_vms *s, double *max_u, double *cfl);
+femx_status femx_vms_sync(femx_vms *s);
+/* The handle's hipStream_t (as void*), for event timing by the caller. */
+void *femx_vms_stream(femx_vms *s);
+
+/* ----------------------------------------------------------------- SUPG */
+/* vmsProps[0..3] = rho, mu, alpha, cI (src/femSUPGExplicitFluidSolver.cpp:835-838),
+ * exOpts[0..5] = euler, B-hat, K1..K4 (:797-802), glob_fixed_dofs
+ * (0 NONE, 1 X, 2 Y, 3 Z, 4 XY, 5 YZ, 6 XZ; :1527-1561), EXITCOND (:1635). */
+typedef struct {
+  double rho, mu, alpha, cI;
+  double dt;
+  int use_euler, use_bhat, k1, k2, k3, k4;
+  int fixed_dofs;
+  double exit_condition;
+  int precision;         /* FEMX_FP64 or FEMX_MIXED */
+} femx_supg_params;
+
+/* Set-up of femSUPGExplicitFluidSolver::solve (:794-1010) incl.
+ * form_lumped_mass (:507-598). */
+femx_status femx_supg_create(femx_mesh *mesh, const femx_bc_desc *bc, const femx_supg_params *p, femx_supg **out);
+femx_status femx_supg_destroy(femx_supg *s);
+/* IC hook: sol [N][3] before the t=0 BCs (:889-893); computes the initial
+ * post-processed pressure (:896). */
+femx_status femx_supg_set_state(femx_supg *s, const double *sol);
+/* nsteps time steps (:1034-1734).  Returns FEMX_EXIT_CONDITION when max |u|
+ * exceeded exit_condition; the state is then the one of that step
+ * (*steps_done counts it) and later steps are not run. */
+femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done);
+/* sol [N][3] = sol_n, pres [E] = pres_n, lam [E] = el_lam, regime [E] =
+ * el_regime; any may be NULL. */
+femx_status femx_supg_get_state(femx_supg *s, double *sol, double *pres, double *lam, double *regime, double *time);
+femx_status femx_supg_get_mass(femx_supg *s, double *l_M /*[N*3]*/);
+/* The reference's 19 log columns of the last step (:1601-1630). */
+femx_status femx_supg_get_log(femx_supg *s, double *cols19);
+femx_status femx_supg_sync(femx_supg *s);
+void *femx_supg_stream(femx_supg *s);
+
+/* --------------------------------------------------------- measurement */
+/* Per-kernel HIP-event timing on the handle's stream (off by default).
+ * While enabled, every kernel launch is bracketed by events; femx_*_kernel_times
+ * returns, per kernel name, the launch count and the summed milliseconds. */
+typedef struct {
+  char name[48];
+  int launches;
+  double total_ms;
+  double bytes_per_launch;  /* algorithmic HBM bytes of one launch */
+} femx_kernel_time;
+femx_status femx_vms_kernel_timing(femx_vms *s, int enable);
+int femx_vms_kernel_times(femx_vms *s, femx_kernel_time *out, int max_entries);
+femx_status femx_supg_kernel_timing(femx_supg *s, int enable);
+int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int max_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEMX_H */

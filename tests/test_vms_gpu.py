@@ -1,0 +1,142 @@
+"""VMS tet4 on the GPU vs the FP64 oracle (restatement of
+src/femVMSExplicitFluidSolver.cpp), through the C ABI.
+
+Tolerances: plain FP64 <= 1e-12 relative L2 per step (state u v w p and the
+subgrid velocity); mixed FP32-element / FP64-state: stated per test."""
+import numpy as np
+import pytest
+
+from femorph_amd import meshgen
+from oracle.oracle import VMSOracle
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def _case(kind, n, jitter=0.0, seed=0):
+    if kind == "cavity":
+        m = meshgen.tet_cube(n, jitter, seed)
+        bc = meshgen.lid_cavity_bc(m)
+        mu = 0.01
+    else:
+        m = meshgen.tet_cube(n, jitter, seed)
+        bc = meshgen.channel_bc(m)
+        mu = 1.0 / 200.0
+    dt = 0.1 * meshgen.min_tet_h(m)
+    params = dict(rho=1.0, mu=mu, inv_eps=1.0, c1=4.0, c2=2.0, dt=dt)
+    ic = meshgen.random_velocity(m, 1e-3, seed + 7)
+    return m, bc, params, ic
+
+
+def _pair(femx, m, bc, params, ic, precision=0, tot_dual_steps=0, sdus=None):
+    o = VMSOracle(m, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"],
+                  tot_dual_steps=tot_dual_steps)
+    o.set_ic(ic)
+    if sdus is not None:
+        o.set_sdus(sdus)
+    dm = femx.DeviceMesh(m)
+    g = femx.VMSSolver(dm, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"],
+                       params["dt"], tot_dual_steps=tot_dual_steps, precision=precision)
+    g.set_state(ic, sdus)
+    return o, g, dm
+
+
+def test_mass_and_initial_state(gpu):
+    m, bc, params, ic = _case("cavity", 6)
+    o, g, _ = _pair(gpu, m, bc, params, ic)
+    assert rel_l2(g.get_mass(), o.get("nd_mass")) <= 1e-15
+    sol, sdus, t = g.get_state()
+    assert np.array_equal(sol, o.get("sol_n"))
+    assert t == 0.0 and not sdus.any()
+
+
+@pytest.mark.parametrize("kind,n,jitter", [("cavity", 6, 0.0), ("channel", 6, 0.15), ("cavity", 5, 0.2)])
+def test_fp64_per_step_parity(gpu, kind, n, jitter):
+    m, bc, params, ic = _case(kind, n, jitter, seed=3)
+    o, g, _ = _pair(gpu, m, bc, params, ic)
+    for step in range(25):
+        o.step()
+        g.step()
+        sol, sdus, t = g.get_state()
+        assert rel_l2(sol, o.get("sol_n")) <= TOL64, step
+        assert rel_l2(sdus, o.get("sdus")) <= 1e-11, step
+        assert t == o.get("scalars")[0]
+
+
+def test_fp64_fused_run_matches(gpu):
+    """Continuous stepping (projection fused into the next residual launch)."""
+    m, bc, params, ic = _case("channel", 6, 0.15, seed=5)
+    o, g, _ = _pair(gpu, m, bc, params, ic)
+    o.step(40)
+    g.step(40)
+    sol, sdus, _ = g.get_state()
+    assert rel_l2(sol, o.get("sol_n")) <= TOL64
+    assert rel_l2(sdus, o.get("sdus")) <= 1e-11
+    mx, cfl = g.get_cfl()
+    assert abs(cfl - o.get("scalars")[4]) <= 1e-12 * max(1.0, cfl)
+
+
+def test_fp64_dual_iterations(gpu):
+    """tot_dual_steps = 2: acceleration term and pre-BC base of later dual iterations."""
+    m, bc, params, ic = _case("cavity", 5, 0.1, seed=2)
+    params["dt"] *= 0.5
+    o, g, _ = _pair(gpu, m, bc, params, ic, tot_dual_steps=2)
+    for step in range(10):
+        o.step()
+        g.step()
+        sol, sdus, _ = g.get_state()
+        assert rel_l2(sol, o.get("sol_n")) <= TOL64, step
+        assert rel_l2(sdus, o.get("sdus")) <= 1e-11, step
+
+
+def test_fp64_nonzero_subscale_ic(gpu):
+    m, bc, params, ic = _case("channel", 5, 0.1, seed=4)
+    rng = np.random.default_rng(11)
+    sd = rng.uniform(-1e-3, 1e-3, size=(12, m.n_elems))
+    o, g, _ = _pair(gpu, m, bc, params, ic, sdus=sd)
+    o.step(5)
+    g.step(5)
+    sol, sdus, _ = g.get_state()
+    assert rel_l2(sol, o.get("sol_n")) <= TOL64
+    assert rel_l2(sdus, o.get("sdus")) <= 1e-11
+
+
+def test_fp64_deterministic(gpu):
+    m, bc, params, ic = _case("channel", 7, 0.15, seed=8)
+    dm = gpu.DeviceMesh(m)
+    outs = []
+    for _ in range(2):
+        g = gpu.VMSSolver(dm, bc, **params)
+        g.set_state(ic)
+        g.step(15)
+        outs.append(g.get_state())
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_c1_cavity_100_steps(gpu):
+    """C1: 20^3 x 6 = 48,000 tets, Re 100 lid cavity, 100 time steps (99 run,
+    like the reference's loopTime = 1 .. totalSteps-1), per-step <= 1e-12."""
+    m, bc, params, ic = _case("cavity", 20)
+    assert m.n_elems == 48000 and m.n_nodes == 9261
+    o, g, _ = _pair(gpu, m, bc, params, ic)
+    worst = 0.0
+    for step in range(99):
+        o.step()
+        g.step()
+        if step % 11 == 0 or step == 98:
+            sol, _, _ = g.get_state()
+            worst = max(worst, rel_l2(sol, o.get("sol_n")))
+    assert worst <= TOL64, worst
+
+
+def test_mixed_precision_error(gpu):
+    """FP32 element / FP64 state vs the FP64 oracle after 200 steps."""
+    m, bc, params, ic = _case("channel", 8, 0.15, seed=6)
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=1)
+    o.step(200)
+    g.step(200)
+    sol, _, _ = g.get_state()
+    ref = o.get("sol_n")
+    err_u = rel_l2(sol[:, :3], ref[:, :3])
+    assert err_u <= 1e-5, err_u

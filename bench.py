@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit VMS tet4 time step on one MI355X.
+
+Workload (the element type and size the metric is quoted on): VMS tet4 on a
+96^3 x 6 Kuhn-split unit cube (5,308,416 tets, 912,673 nodes), interior node
+jitter +-0.15h, parabolic channel inflow u_max = 1, outflow p = 0, no-slip
+walls, Re = 200, dt = 0.1 h_min, one dual iteration per step, synthetic
+seeded input.  Default precision: FP32 element math / FP64 nodal state; the
+plain FP64 path is timed too and reported beside it.
+
+Prints ONE JSON line.  value = element-steps/s of the whole step (all
+launches), inputs resident in HBM.  roofline = the dominant kernel's
+algorithmic bytes per launch / its mean HIP-event duration on the solver's
+stream.  cpu_baseline = the FP64 oracle (plain-C restatement of the
+reference's loops, 1 thread) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from femorph_amd import femx, meshgen  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "element-steps/sec (VMS tet4) + % HBM roofline; p50/p99 solve latency"
+
+
+def algorithmic_bytes_per_element_step(r: float, mixed: bool) -> float:
+    """Compulsory HBM bytes per element-step with ideal node reuse, r = N/E:
+    conn int32 16 B + subgrid velocity r+w (12 values) + r x (coordinates,
+    state u v w p read + write, lumped mass).  FP64: 208 + 96 r; mixed (FP32
+    subgrid and coordinates, FP64 state and mass): 112 + 84 r."""
+    if mixed:
+        return 16 + 96 + r * (12 + 64 + 8)
+    return 16 + 192 + r * (24 + 64 + 8)
+
+
+def make_workload(n=96, jitter=0.15, seed=1):
+    m = meshgen.tet_cube(n, jitter, seed)
+    bc = meshgen.channel_bc(m, u_max=1.0)
+    dt = 0.1 * meshgen.min_tet_h(m)
+    params = dict(rho=1.0, mu=1.0 / 200.0, inv_eps=1.0, c1=4.0, c2=2.0, dt=dt)
+    ic = meshgen.random_velocity(m, 1e-3, seed)
+    return m, bc, params, ic
+
+
+def time_solver(dm, bc, params, ic, precision, steps, warmup):
+    g = femx.VMSSolver(dm, bc, precision=precision, **params)
+    g.set_state(ic)
+    g.step(warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    # per-kernel HIP events on the solver's stream, separate pass
+    g.kernel_timing(True)
+    g.step(steps)
+    g.sync()
+    kt = g.kernel_times()
+    g.kernel_timing(False)
+    sol, _, _ = g.get_state()
+    finite = bool(np.isfinite(sol).all())
+    return (t1 - t0), kt, finite
+
+
+def cpu_baseline(m, bc, params, ic, budget_s=20.0):
+    """FP64 oracle, 1 thread, on whole steps of the same mesh."""
+    from oracle.oracle import VMSOracle  # test infrastructure, checker only
+    o = VMSOracle(m, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"])
+    o.set_ic(ic)
+    steps, t_tot = 0, 0.0
+    while t_tot < budget_s and steps < 10:
+        t0 = time.perf_counter()
+        o.step()
+        t_tot += time.perf_counter() - t0
+        steps += 1
+    del o
+    return m.n_elems * steps / t_tot, steps, t_tot
+
+
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device copy of nbytes (HIP's blit kernel): the reference
+    point that scales rocprof's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=96, help="cube edge count (96 -> 5.31M tets)")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (profiler counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(--gpus must be 1)"}))
+        sys.exit(2)
+    if femx.device_count() < 1:
+        print(json.dumps({"error": "no HIP device visible"}))
+        sys.exit(1)
+
+    m, bc, params, ic = make_workload(args.n)
+    E, N = m.n_elems, m.n_nodes
+    r = N / E
+    dm = femx.DeviceMesh(m)
+    results = {}
+    for prec_name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64)):
+        secs, kt, finite = time_solver(dm, bc, params, ic, prec, args.steps, args.warmup)
+        results[prec_name] = dict(secs=secs, kt=kt, finite=finite)
+    main_res = results[args.precision]
+    mixed = args.precision == "mixed"
+    ms_per_step = main_res["secs"] / args.steps * 1e3
+    value = E * args.steps / main_res["secs"]
+
+    # dominant kernel and its roofline
+    kt = main_res["kt"]
+    dom_name = max(kt, key=lambda k: kt[k]["total_ms"])
+    dom = kt[dom_name]
+    dom_ms = dom["total_ms"] / max(dom["launches"], 1)
+    dom_bytes = dom["bytes_per_launch"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = algorithmic_bytes_per_element_step(r, mixed) * E
+    step_ms_events = sum(v["total_ms"] for v in kt.values()) / args.steps
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+            "kernel_ms": round(dom_ms, 4), "kernel_bytes": dom_bytes,
+            "step_achieved": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+            "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_bytes_per_element": round(algorithmic_bytes_per_element_step(r, mixed), 2),
+            "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4),
+                                bytes=v["bytes_per_launch"]) for k, v in kt.items()}}
+    other = "fp64" if mixed else "mixed"
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "element-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-element/f64-state" if mixed else "f64",
+        "data": "synthetic (seeded jittered Kuhn-split cube, channel BCs)",
+        "config": {"workload": f"VMS tet4 {args.n}^3x6 channel Re200 jitter0.15 ({E} tets, {N} nodes)",
+                   "elements": E, "nodes": N, "dual_iterations_per_step": 1,
+                   "precision": args.precision},
+        "roofline": roof,
+        other: {"value": round(E * args.steps / results[other]["secs"], 1),
+                "ms_per_step": round(results[other]["secs"] / args.steps * 1e3, 4)},
+        "finite_state": all(v["finite"] for v in results.values()),
+        "event_step_ms": round(step_ms_events, 4),
+    }
+    if not args.no_cpu:
+        cv, csteps, csecs = cpu_baseline(m, bc, params, ic, args.cpu_budget)
+        out["cpu_baseline"] = {"value": round(cv, 1), "unit": "element-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{csteps} full FP64 steps of the same {E}-tet mesh, oracle "
+                                         f"(plain-C restatement, 1 thread), {csecs:.1f} s"}
+        out["speedup_vs_cpu_1t"] = round(value / cv, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,34 @@ struct DevBuf {
 //   g_off int[N+1]        node -> incident (element, local node) slots
 //   g_slot int[npe*E]     slot = e*npe + a, ascending e (the reference's
 //                         element-order scatter, src/femVMSExplicitFluidSolver.cpp:394-398)
+//
+// Element blocks (the unit of one workgroup of the element kernels):
+// elements are renumbered along a Morton curve of their centroids and cut
+// greedily into blocks of at most B elements touching at most 256 nodes.
+//   perm      int[E]        block order -> original element id
+//   elem_off  int[nb+1]     element range of each block (block order)
+//   node_off  int[nb+1]     range of each block's local nodes in loc_node
+//   loc_node  int[L]        global id of each (block, local node), ascending
+//   loc_pslot int[L]        -1 if every element of the node is in the block
+//                           ("complete": updated inside the block), else the
+//                           node's partial-sum slot (finished by the fix-up)
+//   lcsr_off  int[L+1]      per local node: its range of contributions,
+//                           block contributions sorted by (node, element)
+//   lcsr      u16[npe*E]    per (element, corner): its position in that
+//                           node-sorted order, relative to the block start
+//   lconn     u8[E*npe]     local node index of each corner (block order)
+//   sh_node   int[S]        nodes shared between blocks
+//   sh_off    int[S+1]      their partial slots, in block order
+struct Blocks {
+  int B = 0, nb = 0, max_local = 0, n_sh = 0, total_local = 0;
+  std::vector<int> perm, elem_off, node_off, loc_node, loc_pslot, lcsr_off, sh_node, sh_off;
+  std::vector<uint16_t> lcsr;
+  std::vector<uint8_t> lconn;
+  DevBuf<int> d_perm, d_elem_off, d_node_off, d_loc_node, d_loc_pslot, d_lcsr_off, d_sh_node, d_sh_off;
+  DevBuf<uint16_t> d_lcsr;
+  DevBuf<uint8_t> d_lconn;
+};
+
 struct Mesh {
   int N = 0, E = 0, npe = 0, max_deg = 0;
   std::vector<double> h_coords;
@@ -72,8 +100,13 @@ struct Mesh {
   DevBuf<double4> xyz;
   DevBuf<int> conn;
   DevBuf<int> g_off, g_slot;
+  Blocks blk;
   int device = 0;
 };
+
+// Builds m.blk on the host (needs h_coords, h_conn, the global CSR) and uploads it.
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B);
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B);
 
 // --------------------------------------------------- boundary conditions
 // One record per BC-touched node, merging the reference's three lists with
@@ -142,6 +175,14 @@ __device__ inline void apply_bc(const BCRecord &r, const int *row_off, const dou
     v = r.dval[1];
     w = r.dval[2];
   }
+}
+
+// gfx950 deals workgroups round-robin over its 8 XCDs (each with its own L2);
+// give every XCD a contiguous range of the Morton-ordered element blocks so
+// neighbouring blocks share node data in one L2.  Bijective on [0, nblk).
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+  const int q = nblk >> 3, rmd = nblk & 7, x = bid & 7, k = bid >> 3;
+  return (x < rmd ? x * (q + 1) : rmd * (q + 1) + (x - rmd) * q) + k;
 }
 
 // ---------------------------------------------------- per-kernel timing

@@ -84,6 +84,161 @@ femx_status BCDev::build(const femx_bc_desc *bc, int N, bool with_pressure, hipS
   return FEMX_OK;
 }
 
+// ------------------------------------------------------- element blocks
+static inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B) {
+  Blocks &b = m.blk;
+  const long E = m.E, npe = m.npe;
+  b.B = B;
+  // Morton order of element centroids
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (long n = 0; n < m.N; n++)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = std::min(lo[k], m.h_coords[3 * n + k]);
+      hi[k] = std::max(hi[k], m.h_coords[3 * n + k]);
+    }
+  std::vector<std::pair<uint64_t, int>> key(E);
+  for (long e = 0; e < E; e++) {
+    double c[3] = {0, 0, 0};
+    for (int a = 0; a < npe; a++)
+      for (int k = 0; k < 3; k++) c[k] += m.h_coords[3L * m.h_conn[e * npe + a] + k];
+    uint64_t code = 0;
+    for (int k = 0; k < 3; k++) {
+      double span = hi[k] - lo[k];
+      double f = span > 0 ? (c[k] / npe - lo[k]) / span : 0.0;
+      uint64_t q = (uint64_t)std::min(std::max(f, 0.0) * 2097151.0, 2097151.0);
+      code |= spread3(q) << k;
+    }
+    key[e] = {code, (int)e};
+  }
+  std::sort(key.begin(), key.end());
+  b.perm.resize(E);
+  for (long i = 0; i < E; i++) b.perm[i] = key[i].second;
+  key.clear();
+  key.shrink_to_fit();
+  // greedy blocks: <= B elements, <= 256 local nodes
+  std::vector<int> stamp(m.N, -1), local_id(m.N, 0);
+  b.elem_off.assign(1, 0);
+  b.node_off.assign(1, 0);
+  b.loc_node.clear();
+  b.lconn.assign(E * npe, 0);
+  std::vector<int> cur_nodes;
+  int blk = 0;
+  long e_begin = 0;
+  auto close_block = [&](long e_end) {
+    std::sort(cur_nodes.begin(), cur_nodes.end());
+    for (size_t j = 0; j < cur_nodes.size(); j++) local_id[cur_nodes[j]] = (int)j;
+    for (long i = e_begin; i < e_end; i++) {
+      const int eo = b.perm[i];
+      for (int a = 0; a < npe; a++) b.lconn[i * npe + a] = (uint8_t)local_id[m.h_conn[(long)eo * npe + a]];
+    }
+    b.loc_node.insert(b.loc_node.end(), cur_nodes.begin(), cur_nodes.end());
+    b.max_local = std::max(b.max_local, (int)cur_nodes.size());
+    b.elem_off.push_back((int)e_end);
+    b.node_off.push_back((int)b.loc_node.size());
+    cur_nodes.clear();
+    blk++;
+    e_begin = e_end;
+  };
+  for (long i = 0; i < E; i++) {
+    const int eo = b.perm[i];
+    int fresh = 0;
+    for (int a = 0; a < npe; a++) fresh += stamp[m.h_conn[(long)eo * npe + a]] != blk;
+    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256)) close_block(i);
+    for (int a = 0; a < npe; a++) {
+      const int n = m.h_conn[(long)eo * npe + a];
+      if (stamp[n] != blk) {
+        stamp[n] = blk;
+        cur_nodes.push_back(n);
+      }
+    }
+  }
+  close_block(E);
+  b.nb = blk;
+  const long L = (long)b.loc_node.size();
+  b.total_local = (int)L;
+  // local gather lists and completeness
+  b.lcsr_off.assign(L + 1, 0);
+  b.lcsr.assign(E * npe, 0);
+  b.loc_pslot.assign(L, -1);
+  std::vector<int> cnt;
+  for (int k = 0; k < b.nb; k++) {
+    const int n0 = b.node_off[k], nn = b.node_off[k + 1] - n0, e0 = b.elem_off[k], e1 = b.elem_off[k + 1];
+    cnt.assign(nn + 1, 0);
+    for (long i = e0; i < e1; i++)
+      for (int a = 0; a < npe; a++) cnt[b.lconn[i * npe + a] + 1]++;
+    for (int j = 0; j < nn; j++) cnt[j + 1] += cnt[j];
+    for (int j = 0; j <= nn; j++) b.lcsr_off[n0 + j] = (int)(e0 * npe) + cnt[j];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (long i = e0; i < e1; i++)
+      for (int a = 0; a < npe; a++) {
+        const int j = b.lconn[i * npe + a];
+        // position of this corner's contribution in the block's node-sorted order
+        b.lcsr[i * npe + a] = (uint16_t)(fill[j]++);
+      }
+    for (int j = 0; j < nn; j++) {
+      const int g = b.loc_node[n0 + j];
+      const int local_deg = cnt[j + 1] - cnt[j];
+      const int global_deg = g_off[g + 1] - g_off[g];
+      b.loc_pslot[n0 + j] = local_deg == global_deg ? -1 : -2;  // -2: shared, slot assigned below
+    }
+  }
+  // shared nodes: partial slots grouped per node, in block order
+  std::vector<int> nparts(m.N, 0);
+  for (long l = 0; l < L; l++)
+    if (b.loc_pslot[l] == -2) nparts[b.loc_node[l]]++;
+  std::vector<int> sh_index(m.N, -1);
+  b.sh_node.clear();
+  b.sh_off.assign(1, 0);
+  // nodes of no element go through the fix-up with zero partial sums, so
+  // they are updated like every other node (the reference divides 0 by a
+  // zero mass there)
+  for (int n = 0; n < m.N; n++)
+    if (nparts[n] > 0 || g_off[n + 1] == g_off[n]) {
+      sh_index[n] = (int)b.sh_node.size();
+      b.sh_node.push_back(n);
+      b.sh_off.push_back(b.sh_off.back() + nparts[n]);
+    }
+  b.n_sh = (int)b.sh_node.size();
+  std::vector<int> taken(b.n_sh, 0);
+  for (long l = 0; l < L; l++)
+    if (b.loc_pslot[l] == -2) {
+      const int si = sh_index[b.loc_node[l]];
+      b.loc_pslot[l] = b.sh_off[si] + taken[si]++;
+    }
+}
+
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B) {
+  build_blocks_host(m, g_off, B);
+  Blocks &b = m.blk;
+  const long E = m.E;
+  const long L = b.total_local;
+  femx_status s;
+  hipStream_t st = nullptr;
+  if ((s = b.d_perm.upload(b.perm.data(), E, st)) != FEMX_OK) return s;
+  if ((s = b.d_elem_off.upload(b.elem_off.data(), b.elem_off.size(), st)) != FEMX_OK) return s;
+  if ((s = b.d_node_off.upload(b.node_off.data(), b.node_off.size(), st)) != FEMX_OK) return s;
+  if ((s = b.d_loc_node.upload(b.loc_node.data(), L, st)) != FEMX_OK) return s;
+  if ((s = b.d_loc_pslot.upload(b.loc_pslot.data(), L, st)) != FEMX_OK) return s;
+  if ((s = b.d_lcsr_off.upload(b.lcsr_off.data(), L + 1, st)) != FEMX_OK) return s;
+  if ((s = b.d_lcsr.upload(b.lcsr.data(), b.lcsr.size(), st)) != FEMX_OK) return s;
+  if ((s = b.d_lconn.upload(b.lconn.data(), b.lconn.size(), st)) != FEMX_OK) return s;
+  if (b.n_sh > 0) {
+    if ((s = b.d_sh_node.upload(b.sh_node.data(), b.n_sh, st)) != FEMX_OK) return s;
+    if ((s = b.d_sh_off.upload(b.sh_off.data(), b.n_sh + 1, st)) != FEMX_OK) return s;
+  }
+  return FEMX_OK;
+}
+
 // ------------------------------------------------------------ timing
 int KernelTimer::kind(const char *name, double bytes) {
   for (size_t i = 0; i < acc.size(); i++)
@@ -205,7 +360,8 @@ extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out
   hipStream_t st = nullptr;
   femx_status s;
   if ((s = m.xyz.upload(xyz.data(), N, st)) != FEMX_OK || (s = m.conn.upload(m.h_conn.data(), npe * E, st)) != FEMX_OK ||
-      (s = m.g_off.upload(off.data(), N + 1, st)) != FEMX_OK || (s = m.g_slot.upload(slot.data(), npe * E, st)) != FEMX_OK) {
+      (s = m.g_off.upload(off.data(), N + 1, st)) != FEMX_OK || (s = m.g_slot.upload(slot.data(), npe * E, st)) != FEMX_OK ||
+      (s = build_blocks(m, off, npe == 4 ? 256 : 128)) != FEMX_OK) {
     delete mh;
     return s;
   }
@@ -215,6 +371,41 @@ extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out
     return hip_fail(e, "mesh upload");
   }
   *out = mh;
+  return FEMX_OK;
+}
+
+// Host-only partition statistics (no device needed): stats[0..7] =
+// blocks, elements, total local nodes, max local nodes, shared nodes,
+// partial slots, complete local nodes, max elements in a block.
+extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_elems, long long *stats) {
+  if (!d || !stats || !d->coords || !d->conn || d->n_nodes <= 0 || d->n_elems <= 0 || (d->npe != 4 && d->npe != 8) ||
+      block_elems <= 0 || block_elems * d->npe > 65536)
+    return set_error("bad argument"), FEMX_ERR_ARG;
+  const long N = d->n_nodes, E = d->n_elems, npe = d->npe;
+  for (long i = 0; i < npe * E; i++)
+    if (d->conn[i] < 0 || d->conn[i] >= N) return set_error("connectivity out of range"), FEMX_ERR_ARG;
+  Mesh m;
+  m.N = (int)N;
+  m.E = (int)E;
+  m.npe = (int)npe;
+  m.h_coords.assign(d->coords, d->coords + 3 * N);
+  m.h_conn.assign(d->conn, d->conn + npe * E);
+  std::vector<int> off(N + 1, 0);
+  for (long i = 0; i < npe * E; i++) off[m.h_conn[i] + 1]++;
+  for (long n = 0; n < N; n++) off[n + 1] += off[n];
+  build_blocks_host(m, off, block_elems);
+  const Blocks &b = m.blk;
+  long long complete = 0, maxe = 0;
+  for (int x : b.loc_pslot) complete += x < 0;
+  for (int k = 0; k < b.nb; k++) maxe = std::max<long long>(maxe, b.elem_off[k + 1] - b.elem_off[k]);
+  stats[0] = b.nb;
+  stats[1] = E;
+  stats[2] = b.total_local;
+  stats[3] = b.max_local;
+  stats[4] = b.n_sh;
+  stats[5] = b.n_sh ? b.sh_off[b.n_sh] : 0;
+  stats[6] = complete;
+  stats[7] = maxe;
   return FEMX_OK;
 }
 

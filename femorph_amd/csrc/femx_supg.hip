@@ -1,16 +1,1081 @@
-// femx_supg.hip -- placeholder until the SUPG kernels land.
+// femx_supg.hip -- explicit SUPG / grad-div step with a local bulk modulus
+// (femSUPGExplicitFluidSolver) on hexa8 and tet4 for gfx950.
+//
+// The reference forms nine dense 24x24 element matrices per Gauss point and
+// multiplies K by the element velocity (src/femSUPGExplicitFluidSolver.cpp:
+// 1095-1507).  Here the same operator is applied matrix-free, one element per
+// thread, term by term per Gauss point:
+//   K1 u : rho N_a (u_g . grad) u_k dV                         (:1318-1322)
+//   K2 u : B^T D(lambda_g, mu) B u dV, B with the reference's B-hat (:1333-1337, :222-296)
+//   K3 u : rho tau (u_g . grad N_a) (u_g . grad) u_k dV        (:1341-1345)
+//   K4 u : rho nu_C dN_a/dx_k div(u) dV                        (:1348)
+// then k_e = -dt K u_e and Euler / AB2 accumulation into the nodes.
+// Element blocks (<= 128 elements, <= 256 nodes) are assembled in LDS exactly
+// like the VMS kernel: block-complete nodes are updated in the block, block
+// boundary nodes through FP64 partial sums and a fix-up launch.
+//
+// Differences of form (not of value) from the reference, all within
+// rounding: the AB2 history is kept as the assembled nodal force of the
+// previous step (1.5 F_n - 0.5 F_{n-1} equals the sum over elements of
+// 1.5 k - 0.5 k_prev), which replaces 384 B/element of history traffic by
+// 24 B/node; the post-processed pressure of step n (compute_pressure
+// :446-501, needed only as an output) is evaluated inside step n+1's element
+// pass from the same Gauss-point divergence when no global DOFs are fixed.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
 #include "femx_common.h"
+
+namespace femx {
+namespace {
+
+// threads per workgroup = max elements per block of the mesh partition
+// (femx_mesh_upload cuts tet4 meshes into <= 256-element blocks, hexa8 into <= 128)
+template <int NPE>
+constexpr int threads_for() { return NPE == 4 ? 256 : 128; }
+constexpr int RING = 64; // per-step diagnostic rows kept on the device
+
+// Gauss points: shape functions and local derivatives of the reference's
+// irSecondOrder rule (femHexa8 / femTetra4), filled from the host.
+__constant__ double cN8[8][8], cdN8[8][8][3];
+__constant__ double cN4[4][4], cdN4[4][4][3];
+__constant__ double cW8, cW4;
+
+template <int NPE>
+struct Shape;
+template <>
+struct Shape<8> {
+  static constexpr int NGP = 8;
+  __device__ static double N(int g, int a) { return cN8[g][a]; }
+  __device__ static double dN(int g, int a, int k) { return cdN8[g][a][k]; }
+  __device__ static double W() { return cW8; }
+};
+template <>
+struct Shape<4> {
+  static constexpr int NGP = 4;
+  __device__ static double N(int g, int a) { return cN4[g][a]; }
+  __device__ static double dN(int g, int a, int k) { return cdN4[g][a][k]; }
+  __device__ static double W() { return cW4; }
+};
+
+// diagnostic row layout: 0..5 min/max u v w, 6,7 min/max tau, 8,9 min/max
+// lambda, 10 v-CFL max, 11 a-CFL max, 12 l-CFL max, 13,14 min/max E,
+// 15,16 min/max nu, 17 max |u|^2 (the log columns of :1601-1630)
+enum { D_UMIN = 0, D_TAU = 6, D_LAM = 8, D_VCFL = 10, D_ACFL = 11, D_LCFL = 12, D_E = 13, D_NU = 15, D_UMOD = 17,
+       D_COLS = 18 };
+__host__ __device__ constexpr bool d_is_min(int c) {
+  return c == 0 || c == 2 || c == 4 || c == D_TAU || c == D_LAM || c == D_E || c == D_NU;
+}
+__host__ __device__ constexpr double d_identity(int c) {
+  return (c == D_VCFL || c == D_ACFL || c == D_LCFL || c == D_UMOD) ? 0.0
+         : d_is_min(c) ? 1.79769313486231570815e+308 : -1.79769313486231570815e+308;
+}
+
+// Block-wide min/max of the columns [c0, c0+K) of a diagnostic row; thread 0
+// stores the block's values to out[blockIdx.x * D_COLS + c]. NaN never wins
+// (like the reference's `if (x > max)` tests).
+template <int NT, int K>
+__device__ __forceinline__ void block_diag_store(double v[K], int c0, double *out) {
+  __shared__ double wv[NT / 64][K];
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    const bool mn = d_is_min(c0 + i);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v[i], off);
+      v[i] = mn ? fmin(v[i], o) : fmax(v[i], o);
+    }
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < K; i++) wv[threadIdx.x >> 6][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      double m = wv[0][i];
+      for (int w = 1; w < NT / 64; w++) m = d_is_min(c0 + i) ? fmin(m, wv[w][i]) : fmax(m, wv[w][i]);
+      out[(long)blockIdx.x * D_COLS + c0 + i] = m;
+    }
+}
+
+struct SupgConst {
+  double rho, mu, alpha, cI, dt, t, exit2;
+  int euler_step;   // 1: Euler update (Euler mode or the first step)
+  int fixed;        // glob_fixed_dofs code
+  int k1, k2, k3, k4;
+};
+
+// femUtils::invert3x3Matrix (src/femUtils.cpp:716-770).
+__device__ __forceinline__ void inv3(const double m[3][3], double iv[3][3], double &det) {
+  det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+        m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  const double id = 1.0 / det;
+  iv[0][0] = (m[1][1] * m[2][2] - m[2][1] * m[1][2]) * id;
+  iv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+  iv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+  iv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+  iv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+  iv[1][2] = (m[1][0] * m[0][2] - m[0][0] * m[1][2]) * id;
+  iv[2][0] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]) * id;
+  iv[2][1] = (m[2][0] * m[0][1] - m[0][0] * m[2][1]) * id;
+  iv[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) * id;
+}
+
+// evalJacobianMatrixLocal + invert + evalGlobalShapeFunctionDerivative +
+// evalGeometricMatrix (src/femElement.cpp:104-354) at Gauss point g, with
+// coordinates relative to node 0 (the Jacobian is translation invariant).
+template <int NPE>
+__device__ __forceinline__ void kinematics(int g, const double dx[NPE][3], double DN[NPE][3], double &detJ,
+                                           double iJ[3][3]) {
+  double J[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      double t = 0.0;
+#pragma unroll
+      for (int c = 1; c < NPE; c++) t += Shape<NPE>::dN(g, c, a) * dx[c][b];
+      J[a][b] = t;
+    }
+  inv3(J, iJ, detJ);
+#pragma unroll
+  for (int a = 0; a < NPE; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+      DN[a][b] = iJ[b][0] * Shape<NPE>::dN(g, a, 0) + iJ[b][1] * Shape<NPE>::dN(g, a, 1) +
+                 iJ[b][2] * Shape<NPE>::dN(g, a, 2);
+}
+
+template <typename T>
+struct SupgArgs {
+  int nblk, E;
+  const int *elem_off, *node_off, *loc_node, *loc_pslot, *lcsr_off;
+  const uint16_t *lpos;  // [E][npe] node-sorted position of each corner
+  const uint8_t *lconn;  // [E][npe]
+  const double4 *xyzm;   // x y z, lumped mass
+  const double4 *U;      // sol_p (zeroed by the global fixed DOFs)
+  const double4 *Upress; // post-BC state before the fixed-DOF zeroing (pressure of the previous step)
+  double4 *Un, *Unp;     // outputs: next sol_p, next pre-zeroing state
+  double4 *Fprev;        // assembled force of the previous step (AB2)
+  double4 *partials;
+  double *lam, *regime, *pres;  // [E] block order
+  const int *node_rec;
+  const BCRecord *rec;
+  const int *row_off;
+  const double4 *rows;
+  int *err;
+  double *eblk;              // [nblk][D_COLS] per-block element diagnostics (DIAG)
+  int *halt;                 // INT_MAX or the step that exceeded EXITCOND
+  const double *prev_row;    // previous step's diagnostic row (exit check), or null
+  int step_id;
+  SupgConst k;
+};
+
+template <int NPE>
+__device__ __forceinline__ void lconn_load(const uint8_t *lconn, int e, int id[NPE]) {
+  if (NPE == 4) {
+    const uchar4 c = reinterpret_cast<const uchar4 *>(lconn)[e];
+    id[0] = c.x; id[1] = c.y; id[2] = c.z; id[3] = c.w;
+  } else {
+    const uchar4 c0 = reinterpret_cast<const uchar4 *>(lconn)[2 * e], c1 = reinterpret_cast<const uchar4 *>(lconn)[2 * e + 1];
+    id[0] = c0.x; id[1] = c0.y; id[2] = c0.z; id[3] = c0.w;
+    id[4 % NPE] = c1.x; id[5 % NPE] = c1.y; id[6 % NPE] = c1.z; id[7 % NPE] = c1.w;
+  }
+}
+template <int NPE>
+__device__ __forceinline__ void lpos_load(const uint16_t *lpos, int e, int ps[NPE]) {
+  const ushort4 *p = reinterpret_cast<const ushort4 *>(lpos);
+  const ushort4 a = p[(NPE / 4) * e];
+  ps[0] = a.x; ps[1] = a.y; ps[2] = a.z; ps[3] = a.w;
+  if (NPE == 8) {
+    const ushort4 b = p[2 * e + 1];
+    ps[4 % NPE] = b.x; ps[5 % NPE] = b.y; ps[6 % NPE] = b.z; ps[7 % NPE] = b.w;
+  }
+}
+
+__device__ __forceinline__ bool halted(const int *halt) { return __builtin_nontemporal_load(halt) != INT_MAX; }
+
+// SUPG node update of one node (:1510-1514 update, :1519-1521 BCs,
+// :1527-1561 fixed DOFs): returns nothing, writes sol_p of the next step.
+__device__ __forceinline__ void supg_node(const double F[3], double4 fprev, double m, double4 up, const SupgConst &k,
+                                          int ri, const BCRecord *rec, const int *row_off, const double4 *rows,
+                                          int *err, double4 &next, double4 &press_state, double4 &fstore) {
+  double acc[3];
+  if (k.euler_step) {
+    acc[0] = F[0]; acc[1] = F[1]; acc[2] = F[2];
+  } else {
+    acc[0] = (3.0 / 2.0) * F[0] - (1.0 / 2.0) * fprev.x;
+    acc[1] = (3.0 / 2.0) * F[1] - (1.0 / 2.0) * fprev.y;
+    acc[2] = (3.0 / 2.0) * F[2] - (1.0 / 2.0) * fprev.z;
+  }
+  const double im = 1.0 / m;
+  double u = up.x + im * acc[0], v = up.y + im * acc[1], w = up.z + im * acc[2];
+  if (ri >= 0) apply_bc(rec[ri], row_off, rows, k.t, u, v, w, nullptr, err);
+  press_state = make_double4(u, v, w, 0.0);
+  const int f = k.fixed;
+  if (f == 1 || f == 4 || f == 6) u = 0.0;
+  if (f == 2 || f == 4 || f == 5) v = 0.0;
+  if (f == 3 || f == 5 || f == 6) w = 0.0;
+  next = make_double4(u, v, w, 0.0);
+  fstore = make_double4(F[0], F[1], F[2], 0.0);
+}
+
+template <typename T, int NPE, bool BHAT, bool DYN, bool PRESS, bool PSEP, bool DIAG>
+__global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) {
+  constexpr int NGP = Shape<NPE>::NGP;
+  constexpr int SB = threads_for<NPE>();
+  __shared__ double sx[256], sy[256], sz[256], sm[256];
+  __shared__ double su[256], sv[256], sw[256];
+  __shared__ double pu[PSEP ? 256 : 1], pv[PSEP ? 256 : 1], pw[PSEP ? 256 : 1];
+  __shared__ T sres[DYN ? 3 * NPE * SB : 1];
+  const int tid = threadIdx.x;
+  if (halted(a.halt)) return;
+  if (DYN && a.prev_row) {
+    // exit condition of the previous step (:1635-1643), decided identically by every block
+    const double m2 = a.prev_row[D_UMOD];
+    if (m2 > a.k.exit2) {
+      if (tid == 0) atomicMin(a.halt, a.step_id - 1);
+      return;
+    }
+  }
+  const int blk = xcd_block(blockIdx.x, a.nblk);
+  const int n0 = a.node_off[blk], nn = a.node_off[blk + 1] - n0;
+  const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
+  const bool has_e = tid < ne;
+  const int e = e0 + tid;
+  int id[NPE], ps[NPE];
+  double dg[11];
+#pragma unroll
+  for (int i = 0; i < 11; i++) dg[i] = d_identity(D_TAU + i);
+  double lam_prev = 0.0;
+  if (has_e) {
+    lconn_load<NPE>(a.lconn, e, id);
+    if (DYN) lpos_load<NPE>(a.lpos, e, ps);
+    if (PRESS) lam_prev = a.lam[e];
+  }
+  for (int j = tid; j < nn; j += SB) {
+    const int g = a.loc_node[n0 + j];
+    const double4 x = a.xyzm[g];
+    sx[j] = x.x; sy[j] = x.y; sz[j] = x.z; sm[j] = x.w;
+    const double4 u = a.U[g];
+    su[j] = u.x; sv[j] = u.y; sw[j] = u.z;
+    if (PSEP) {
+      const double4 q = a.Upress[g];
+      pu[j] = q.x; pv[j] = q.y; pw[j] = q.z;
+    }
+  }
+  __syncthreads();
+  if (has_e) {
+    const SupgConst &k = a.k;
+    double dx[NPE][3];
+    double U[NPE][3];
+#pragma unroll
+    for (int n = 0; n < NPE; n++) {
+      dx[n][0] = sx[id[n]] - sx[id[0]];
+      dx[n][1] = sy[id[n]] - sy[id[0]];
+      dx[n][2] = sz[id[n]] - sz[id[0]];
+      U[n][0] = su[id[n]];
+      U[n][1] = sv[id[n]];
+      U[n][2] = sw[id[n]];
+    }
+    // eval_avg_shapeDeriv (:660-707): volume average of DN (hexa only)
+    double avg[NPE][3];
+    if (BHAT) {
+#pragma unroll
+      for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = 0.0;
+      double den = 0.0;
+      for (int g = 0; g < NGP; g++) {
+        double DN[NPE][3], detJ, iJ[3][3];
+        kinematics<NPE>(g, dx, DN, detJ, iJ);
+        const double dv = detJ * Shape<NPE>::W();
+#pragma unroll
+        for (int n = 0; n < NPE; n++) {
+          avg[n][0] += DN[n][0] * dv;
+          avg[n][1] += DN[n][1] * dv;
+          avg[n][2] += DN[n][2] * dv;
+        }
+        den += dv;
+      }
+      const double iden = 1.0 / den;
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        avg[n][0] *= iden;
+        avg[n][1] *= iden;
+        avg[n][2] *= iden;
+      }
+    }
+    T out[NPE][3];
+#pragma unroll
+    for (int n = 0; n < NPE; n++) out[n][0] = out[n][1] = out[n][2] = (T)0;
+    // per-thread diagnostic accumulators, columns D_TAU .. D_NU+1
+    double vol = 0.0, lam_int = 0.0, reg = 0.0, pnum = 0.0;
+    const double nu_k = k.mu / k.rho;
+    for (int g = 0; g < NGP; g++) {
+      double DN[NPE][3], detJ, iJ[3][3];
+      kinematics<NPE>(g, dx, DN, detJ, iJ);
+      const double dv = detJ * Shape<NPE>::W();
+      vol += dv;
+      double ug[3] = {0.0, 0.0, 0.0}, div = 0.0;
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        const double Nn = Shape<NPE>::N(g, n);
+        ug[0] += Nn * U[n][0];
+        ug[1] += Nn * U[n][1];
+        ug[2] += Nn * U[n][2];
+        div += DN[n][0] * (U[n][0] - U[0][0]) + DN[n][1] * (U[n][1] - U[0][1]) + DN[n][2] * (U[n][2] - U[0][2]);
+      }
+      if (PRESS) {
+        if (PSEP) {
+          double dp = 0.0;
+#pragma unroll
+          for (int n = 0; n < NPE; n++)
+            dp += DN[n][0] * (pu[id[n]] - pu[id[0]]) + DN[n][1] * (pv[id[n]] - pv[id[0]]) +
+                  DN[n][2] * (pw[id[n]] - pw[id[0]]);
+          pnum += dp * dv;
+        } else {
+          pnum += div * dv;
+        }
+      }
+      if (!DYN) continue;
+      // metric G = invJ^T invJ in the reference's index order (src/femElement.cpp:222-231)
+      double G[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) G[i][j] = iJ[0][i] * iJ[0][j] + iJ[1][i] * iJ[1][j] + iJ[2][i] * iJ[2][j];
+      // eval_stabilization (:336-377), tau without the (2/dt)^2 term
+      double t2 = 0.0, t3 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        t2 += ug[i] * (G[i][0] * ug[0] + G[i][1] * ug[1] + G[i][2] * ug[2]);
+        t3 += G[i][0] * G[i][0] + G[i][1] * G[i][1] + G[i][2] * G[i][2];
+      }
+      t3 = t3 * nu_k * nu_k * k.cI;
+      const double tau = 1.0 / sqrt(t2 + t3);
+      const double trG = G[0][0] + G[1][1] + G[2][2];
+      const double nuC = 1.0 / (trG * tau);
+      // eval_bulk_modulus (:382-441): lambda = alpha max(mu/rho, sqrt(u^T G^-1 u))
+      double iG[3][3], dG;
+      inv3(G, iG, dG);
+      double l2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) l2 += (iG[i][0] * ug[0] + iG[i][1] * ug[1] + iG[i][2] * ug[2]) * ug[i];
+      l2 = sqrt(l2);
+      const double lam = k.alpha * ((nu_k < l2) ? l2 : nu_k);
+      reg += (nu_k > l2) ? 1.0 : 2.0;
+      lam_int += lam * dv;
+      if (DIAG) {
+        // get_el_lengths_from_G (:756-790) and the CFL / tau / lambda extrema (:1187-1240)
+        const double un = sqrt(ug[0] * ug[0] + ug[1] * ug[1] + ug[2] * ug[2]);
+        const double o0 = ug[0] / un, o1 = ug[1] / un, o2 = ug[2] / un;
+        const double el = 2.0 * sqrt((iG[0][0] * o0 + iG[0][1] * o1 + iG[0][2] * o2) * o0 +
+                                     (iG[1][0] * o0 + iG[1][1] * o1 + iG[1][2] * o2) * o1 +
+                                     (iG[2][0] * o0 + iG[2][1] * o1 + iG[2][2] * o2) * o2);
+        dg[D_ACFL - D_TAU] = fmax(dg[D_ACFL - D_TAU], fabs(ug[0] * k.dt) / el);
+        dg[D_VCFL - D_TAU] = fmax(dg[D_VCFL - D_TAU], 2.0 * nu_k * (k.dt / (el * el)));
+        dg[D_LCFL - D_TAU] = fmax(dg[D_LCFL - D_TAU], 2.0 * lam * (k.dt / (el * el)));
+        dg[0] = fmin(dg[0], tau);
+        dg[1] = fmax(dg[1], tau);
+        dg[2] = fmin(dg[2], lam);
+        dg[3] = fmax(dg[3], lam);
+      }
+      // matrix-free K u contributions at this Gauss point
+      double c[3] = {0.0, 0.0, 0.0};  // (u_g . grad) u
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        const double ud = ug[0] * DN[n][0] + ug[1] * DN[n][1] + ug[2] * DN[n][2];
+        c[0] += ud * (U[n][0] - U[0][0]);
+        c[1] += ud * (U[n][1] - U[0][1]);
+        c[2] += ud * (U[n][2] - U[0][2]);
+      }
+      double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        const double u0 = U[n][0], u1 = U[n][1], u2 = U[n][2];
+        const double d0 = DN[n][0], d1 = DN[n][1], d2 = DN[n][2];
+        if (BHAT) {
+          const double b0 = (1.0 / 3.0) * (avg[n][0] - d0), b1 = (1.0 / 3.0) * (avg[n][1] - d1),
+                       b2 = (1.0 / 3.0) * (avg[n][2] - d2);
+          const double sn = b0 * u0 + b1 * u1 + b2 * u2;
+          eps[0] += (d0 - 2.0 * b0) * u0 + sn;
+          eps[1] += (d1 - 2.0 * b1) * u1 + sn;
+          eps[2] += (d2 - 2.0 * b2) * u2 + sn;
+        } else {
+          eps[0] += d0 * u0;
+          eps[1] += d1 * u1;
+          eps[2] += d2 * u2;
+        }
+        eps[3] += d1 * u0 + d0 * u1;
+        eps[4] += d2 * u1 + d1 * u2;
+        eps[5] += d2 * u0 + d0 * u2;
+      }
+      const double trE = eps[0] + eps[1] + eps[2];
+      const double s0 = 2.0 * k.mu * eps[0] + lam * trE, s1 = 2.0 * k.mu * eps[1] + lam * trE,
+                   s2 = 2.0 * k.mu * eps[2] + lam * trE;
+      const double s3 = k.mu * eps[3], s4 = k.mu * eps[4], s5 = k.mu * eps[5];
+      const double ssum = s0 + s1 + s2;
+      const double f1 = k.k1 ? k.rho * dv : 0.0, f3 = k.k3 ? k.rho * tau * dv : 0.0;
+      const double f4 = k.k4 ? k.rho * nuC * div * dv : 0.0, f2 = k.k2 ? dv : 0.0;
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        const double d0 = DN[n][0], d1 = DN[n][1], d2 = DN[n][2];
+        const double Nn = Shape<NPE>::N(g, n);
+        const double ud = ug[0] * d0 + ug[1] * d1 + ug[2] * d2;
+        double k2x, k2y, k2z;
+        if (BHAT) {
+          const double b0 = (1.0 / 3.0) * (avg[n][0] - d0), b1 = (1.0 / 3.0) * (avg[n][1] - d1),
+                       b2 = (1.0 / 3.0) * (avg[n][2] - d2);
+          k2x = b0 * ssum + (d0 - 2.0 * b0) * s0;
+          k2y = b1 * ssum + (d1 - 2.0 * b1) * s1;
+          k2z = b2 * ssum + (d2 - 2.0 * b2) * s2;
+        } else {
+          k2x = d0 * s0;
+          k2y = d1 * s1;
+          k2z = d2 * s2;
+        }
+        k2x += d1 * s3 + d2 * s5;
+        k2y += d0 * s3 + d2 * s4;
+        k2z += d1 * s4 + d0 * s5;
+        const double cv = f1 * Nn + f3 * ud;
+        out[n][0] += (T)(cv * c[0] + f2 * k2x + f4 * d0);
+        out[n][1] += (T)(cv * c[1] + f2 * k2y + f4 * d1);
+        out[n][2] += (T)(cv * c[2] + f2 * k2z + f4 * d2);
+      }
+    }
+    if (PRESS) a.pres[e] = -lam_prev * (pnum / vol);  // compute_pressure of the previous step (:498)
+    if (DYN) {
+      const double lame = lam_int / vol;  // :1446
+      a.lam[e] = lame;
+      a.regime[e] = reg / NGP;
+      if (DIAG) {
+        const double cE = (k.mu * (3 * lame + 2 * k.mu)) / (lame + k.mu), cnu = lame / (2 * (lame + k.mu));
+        dg[D_E - D_TAU] = fmin(dg[D_E - D_TAU], cE);
+        dg[D_E + 1 - D_TAU] = fmax(dg[D_E + 1 - D_TAU], cE);
+        dg[D_NU - D_TAU] = fmin(dg[D_NU - D_TAU], cnu);
+        dg[D_NU + 1 - D_TAU] = fmax(dg[D_NU + 1 - D_TAU], cnu);
+      }
+      const T mdt = (T)(-k.dt);
+#pragma unroll
+      for (int n = 0; n < NPE; n++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) sres[i * (NPE * SB) + ps[n]] = mdt * out[n][i];
+    }
+  }
+  if (!DYN) return;
+  if (DIAG) block_diag_store<SB, 11>(dg, D_TAU, a.eblk);
+  __syncthreads();
+  const int base_s = e0 * NPE;
+  for (int j = tid; j < nn; j += SB) {
+    double F[3] = {0.0, 0.0, 0.0};
+    const int s0 = a.lcsr_off[n0 + j] - base_s, s1 = a.lcsr_off[n0 + j + 1] - base_s;
+#pragma unroll 4
+    for (int s = s0; s < s1; s++) {
+      F[0] += (double)sres[0 * (NPE * SB) + s];
+      F[1] += (double)sres[1 * (NPE * SB) + s];
+      F[2] += (double)sres[2 * (NPE * SB) + s];
+    }
+    const int psl = a.loc_pslot[n0 + j];
+    const int g = a.loc_node[n0 + j];
+    if (psl < 0) {
+      double4 next, pst, fst;
+      const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
+      supg_node(F, fp, sm[j], make_double4(su[j], sv[j], sw[j], 0.0), a.k, a.node_rec[g], a.rec, a.row_off, a.rows,
+                a.err, next, pst, fst);
+      a.Un[g] = next;
+      if (a.Unp) a.Unp[g] = pst;
+      a.Fprev[g] = fst;
+    } else {
+      a.partials[psl] = make_double4(F[0], F[1], F[2], 0.0);
+    }
+  }
+}
+
+// Velocity extrema and max |u|^2 of the new state (get_solution_extrema
+// :604-655), one row per block of 256 nodes.
+__global__ __launch_bounds__(256) void supg_node_diag(int N, const double4 *U, double *nblk, const int *halt) {
+  if (halted(halt)) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6], m2 = 0.0;
+  if (n < N) {
+    const double4 u = U[n];
+    v[0] = v[1] = u.x;
+    v[2] = v[3] = u.y;
+    v[4] = v[5] = u.z;
+    m2 = u.x * u.x + u.y * u.y + u.z * u.z;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = d_identity(i);
+  }
+  block_diag_store<256, 6>(v, D_UMIN, nblk);
+  double w[1] = {m2};
+  __syncthreads();
+  block_diag_store<256, 1>(w, D_UMOD, nblk);
+}
+
+// Reduce the per-block rows into this step's ring row (one workgroup).
+__global__ __launch_bounds__(256) void supg_diag_reduce(const double *eblk, int ne_blk, const double *nblk, int nn_blk,
+                                                        int with_elem, double *row, const int *halt) {
+  if (halted(halt)) return;
+  __shared__ double red[256];
+  for (int c = 0; c < D_COLS; c++) {
+    const bool node_col = c < 6 || c == D_UMOD;
+    const double *src = node_col ? nblk : eblk;
+    const int nb = node_col ? nn_blk : (with_elem ? ne_blk : 0);
+    const bool mn = d_is_min(c);
+    double m = d_identity(c);
+    for (int b = threadIdx.x; b < nb; b += 256) {
+      const double x = src[(long)b * D_COLS + c];
+      m = mn ? fmin(m, x) : fmax(m, x);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] = mn ? fmin(red[threadIdx.x], red[threadIdx.x + w])
+                                                      : fmax(red[threadIdx.x], red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) row[c] = red[0];
+    __syncthreads();
+  }
+}
+
+struct SupgFix {
+  int n_sh;
+  const int *sh_node, *sh_off;
+  const double4 *partials, *xyzm, *U;
+  double4 *Un, *Unp, *Fprev;
+  const int *node_rec;
+  const BCRecord *rec;
+  const int *row_off;
+  const double4 *rows;
+  int *err;
+  const int *halt;
+  SupgConst k;
+};
+
+__global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
+  if (halted(a.halt)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n_sh) return;
+  const int g = a.sh_node[s];
+  double F[3] = {0.0, 0.0, 0.0};
+  for (int q = a.sh_off[s]; q < a.sh_off[s + 1]; q++) {
+    const double4 p = a.partials[q];
+    F[0] += p.x;
+    F[1] += p.y;
+    F[2] += p.z;
+  }
+  double4 next, pst, fst;
+  const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
+  supg_node(F, fp, a.xyzm[g].w, a.U[g], a.k, a.node_rec[g], a.rec, a.row_off, a.rows, a.err, next, pst, fst);
+  a.Un[g] = next;
+  if (a.Unp) a.Unp[g] = pst;
+  a.Fprev[g] = fst;
+}
+
+// form_lumped_mass (:507-598): element row sums of the Bubnov mass
+// rho N_a N_b detJ w, accumulated over Gauss points then over b, gathered
+// per node in ascending original element order.
+template <int NPE>
+__global__ __launch_bounds__(256) void supg_mass(int N, const int *g_off, const int *g_slot, const int *conn,
+                                                 const double4 *xyz, double rho, double4 *xyzm) {
+  constexpr int NGP = Shape<NPE>::NGP;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double acc = 0.0;
+  for (int s = g_off[n]; s < g_off[n + 1]; s++) {
+    const int e = g_slot[s] / NPE, a = g_slot[s] % NPE;
+    double dx[NPE][3];
+    const double4 x0 = xyz[conn[(long)e * NPE]];
+#pragma unroll
+    for (int c = 0; c < NPE; c++) {
+      const double4 x = xyz[conn[(long)e * NPE + c]];
+      dx[c][0] = x.x - x0.x;
+      dx[c][1] = x.y - x0.y;
+      dx[c][2] = x.z - x0.z;
+    }
+    double detJ[NGP];
+    for (int g = 0; g < NGP; g++) {
+      double DN[NPE][3], iJ[3][3];
+      kinematics<NPE>(g, dx, DN, detJ[g], iJ);
+    }
+    double row = 0.0;
+    for (int b = 0; b < NPE; b++) {
+      double bm = 0.0;
+      for (int g = 0; g < NGP; g++) bm += rho * Shape<NPE>::N(g, a) * Shape<NPE>::N(g, b) * detJ[g] * Shape<NPE>::W();
+      row += bm;
+    }
+    acc += row;
+  }
+  const double4 x = xyz[n];
+  xyzm[n] = make_double4(x.x, x.y, x.z, acc);
+}
+
+__global__ void supg_bc_only(int nrec, const int *rec_node, const BCRecord *rec, const int *row_off, const double4 *rows,
+                             double t, double4 *U, int *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const int n = rec_node[i];
+  double4 q = U[n];
+  double u = q.x, v = q.y, w = q.z;
+  apply_bc(rec[i], row_off, rows, t, u, v, w, nullptr, err);
+  U[n] = make_double4(u, v, w, 0.0);
+}
+
+__global__ void block_to_orig(int E, const int *perm, const double *src, double *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < E) dst[perm[i]] = src[i];
+}
+
+}  // namespace
+
+}  // namespace femx
+
 using namespace femx;
-struct femx_supg { int dummy; };
-#define UNSUP return set_error("SUPG not built yet"), FEMX_ERR_UNSUPPORTED
-extern "C" femx_status femx_supg_create(femx_mesh *, const femx_bc_desc *, const femx_supg_params *, femx_supg **) { UNSUP; }
-extern "C" femx_status femx_supg_destroy(femx_supg *) { return FEMX_OK; }
-extern "C" femx_status femx_supg_set_state(femx_supg *, const double *) { UNSUP; }
-extern "C" femx_status femx_supg_step(femx_supg *, int, int *) { UNSUP; }
-extern "C" femx_status femx_supg_get_state(femx_supg *, double *, double *, double *, double *, double *) { UNSUP; }
-extern "C" femx_status femx_supg_get_mass(femx_supg *, double *) { UNSUP; }
-extern "C" femx_status femx_supg_get_log(femx_supg *, double *) { UNSUP; }
-extern "C" femx_status femx_supg_sync(femx_supg *) { UNSUP; }
-extern "C" void *femx_supg_stream(femx_supg *) { return nullptr; }
-extern "C" femx_status femx_supg_kernel_timing(femx_supg *, int) { UNSUP; }
-extern "C" int femx_supg_kernel_times(femx_supg *, femx_kernel_time *, int) { return -1; }
+
+struct femx_supg {
+  Mesh *mesh = nullptr;
+  femx_supg_params p{};
+  hipStream_t st = nullptr;
+  BCDev bc;
+  DevBuf<int> rec_node;
+  DevBuf<double4> xyzm, U[2], Up[2], Fprev, partials;
+  DevBuf<double> lam, regime, pres, scratch;
+  DevBuf<double> ring, eblk, nblk;  // [RING][D_COLS] reduced rows; per-block rows
+  DevBuf<int> halt;
+  int cur = 0;
+  long loop_time = 0;
+  double time = 0.0;
+  bool press_pending = false;
+  bool diag_enabled = false;
+  bool halted = false;  // the exit condition fired; the solve is over
+  KernelTimer timer;
+};
+
+namespace {
+
+bool g_tables_ready[64] = {false};
+
+femx_status upload_tables(int dev) {
+  if (dev >= 0 && dev < 64 && g_tables_ready[dev]) return FEMX_OK;
+  // femIntegrationRule irSecondOrder (src/femIntegrationRule.cpp:204-252, :320-329)
+  const double q = 1.0 / sqrt(3.0);
+  const double gp8[8][3] = {{-q, -q, q}, {q, -q, q}, {q, q, q}, {-q, q, q},
+                            {-q, -q, -q}, {q, -q, -q}, {q, q, -q}, {-q, q, -q}};
+  const double c1[8] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0};
+  const double c2[8] = {-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0};
+  const double c3[8] = {-1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0};
+  double N8[8][8], dN8[8][8][3];
+  for (int g = 0; g < 8; g++)
+    for (int a = 0; a < 8; a++) {
+      const double x = gp8[g][0], y = gp8[g][1], z = gp8[g][2];
+      N8[g][a] = (1.0 / 8.0) * (1.0 + x * c1[a]) * (1.0 + y * c2[a]) * (1.0 + z * c3[a]);  // femHexa8.cpp:21-23
+      dN8[g][a][0] = (1.0 / 8.0) * c1[a] * (1.0 + y * c2[a]) * (1.0 + z * c3[a]);        // femHexa8.cpp:36-38
+      dN8[g][a][1] = (1.0 / 8.0) * c2[a] * (1.0 + x * c1[a]) * (1.0 + z * c3[a]);
+      dN8[g][a][2] = (1.0 / 8.0) * c3[a] * (1.0 + x * c1[a]) * (1.0 + y * c2[a]);
+    }
+  const double A = 0.58541020, B = 0.13819660;
+  const double gp4[4][3] = {{A, B, B}, {B, A, B}, {B, B, A}, {B, B, B}};
+  double N4[4][4], dN4[4][4][3];
+  const double d4[4][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}, {-1.0, -1.0, -1.0}};
+  for (int g = 0; g < 4; g++) {
+    N4[g][0] = gp4[g][0];  // femTetra4.cpp:295-301
+    N4[g][1] = gp4[g][1];
+    N4[g][2] = gp4[g][2];
+    N4[g][3] = 1.0 - gp4[g][0] - gp4[g][1] - gp4[g][2];
+    for (int a = 0; a < 4; a++)
+      for (int k = 0; k < 3; k++) dN4[g][a][k] = d4[a][k];
+  }
+  const double w8 = 8.0 * (1.0 / 8.0), w4 = (1.0 / 6.0) * (1.0 / 4.0);
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cN8), N8, sizeof(N8)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cdN8), dN8, sizeof(dN8)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cN4), N4, sizeof(N4)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cdN4), dN4, sizeof(dN4)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW8), &w8, sizeof(w8)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW4), &w4, sizeof(w4)));
+  if (dev >= 0 && dev < 64) g_tables_ready[dev] = true;
+  return FEMX_OK;
+}
+
+SupgConst supg_const(const femx_supg *s, double t, bool euler_step) {
+  SupgConst k;
+  k.rho = s->p.rho;
+  k.mu = s->p.mu;
+  k.alpha = s->p.alpha;
+  k.cI = s->p.cI;
+  k.dt = s->p.dt;
+  k.t = t;
+  k.exit2 = s->p.exit_condition * s->p.exit_condition;
+  k.euler_step = euler_step ? 1 : 0;
+  k.fixed = s->p.fixed_dofs;
+  k.k1 = s->p.k1;
+  k.k2 = s->p.k2;
+  k.k3 = s->p.k3;
+  k.k4 = s->p.k4;
+  return k;
+}
+
+template <typename T>
+SupgArgs<T> supg_args(femx_supg *s, bool euler_step, double t) {
+  Mesh &m = *s->mesh;
+  Blocks &b = m.blk;
+  SupgArgs<T> a;
+  a.nblk = b.nb;
+  a.E = m.E;
+  a.elem_off = b.d_elem_off.p;
+  a.node_off = b.d_node_off.p;
+  a.loc_node = b.d_loc_node.p;
+  a.loc_pslot = b.d_loc_pslot.p;
+  a.lcsr_off = b.d_lcsr_off.p;
+  a.lpos = b.d_lcsr.p;
+  a.lconn = b.d_lconn.p;
+  a.xyzm = s->xyzm.p;
+  a.U = s->U[s->cur].p;
+  a.Upress = s->p.fixed_dofs ? s->Up[s->cur].p : s->U[s->cur].p;
+  a.Un = s->U[s->cur ^ 1].p;
+  a.Unp = s->p.fixed_dofs ? s->Up[s->cur ^ 1].p : nullptr;
+  a.Fprev = s->Fprev.p;
+  a.partials = s->partials.p;
+  a.lam = s->lam.p;
+  a.regime = s->regime.p;
+  a.pres = s->pres.p;
+  a.node_rec = s->bc.node_rec.p;
+  a.rec = s->bc.rec.p;
+  a.row_off = s->bc.row_off.p;
+  a.rows = s->bc.rows.p;
+  a.err = s->bc.err.p;
+  a.eblk = s->eblk.p;
+  a.halt = s->halt.p;
+  a.prev_row = nullptr;
+  a.step_id = (int)s->loop_time;
+  a.k = supg_const(s, t, euler_step);
+  return a;
+}
+
+template <typename T, int NPE, bool BHAT>
+void launch_supg_t(femx_supg *s, SupgArgs<T> &a, bool dyn, bool press, bool diag) {
+  const int grd = s->mesh->blk.nb;
+  const bool psep = press && s->p.fixed_dofs != 0;
+  constexpr int SB = threads_for<NPE>();
+#define L(D, P, PS, G) supg_block<T, NPE, BHAT, D, P, PS, G><<<grd, SB, 0, s->st>>>(a)
+  if (!dyn) {
+    if (psep) L(false, true, true, false);
+    else L(false, true, false, false);
+    return;
+  }
+  if (press) {
+    if (psep) {
+      if (diag) L(true, true, true, true); else L(true, true, true, false);
+    } else {
+      if (diag) L(true, true, false, true); else L(true, true, false, false);
+    }
+  } else {
+    if (diag) L(true, false, false, true); else L(true, false, false, false);
+  }
+#undef L
+}
+
+template <typename T>
+void launch_supg_npe(femx_supg *s, SupgArgs<T> &a, bool dyn, bool press, bool diag) {
+  if (s->mesh->npe == 8) {
+    if (s->p.use_bhat) launch_supg_t<T, 8, true>(s, a, dyn, press, diag);
+    else launch_supg_t<T, 8, false>(s, a, dyn, press, diag);
+  } else {
+    launch_supg_t<T, 4, false>(s, a, dyn, press, diag);
+  }
+}
+
+// Algorithmic bytes of one element launch: u8 connectivity + u16 positions
+// (3 B per corner), coordinates + mass and the state read once per node, the
+// next state and nodal history written once per node, el_lam / regime /
+// pressure (read + 3 writes) per element.
+double supg_bytes(const femx_supg *s) {
+  const double E = s->mesh->E, N = s->mesh->N;
+  return E * (3.0 * s->mesh->npe + 32.0) + N * (32.0 + 32.0 + 32.0 + 32.0);
+}
+
+femx_status launch_supg(femx_supg *s, bool dyn, bool press, bool euler_step, double t, long step_id,
+                        const double *prev_row) {
+  const char *name = dyn ? (press ? "supg_block_press" : "supg_block") : "supg_press";
+  TimedLaunch tl(s->timer, s->timer.kind(name, supg_bytes(s)), s->st);
+  if (s->p.precision == FEMX_MIXED) {
+    SupgArgs<float> a = supg_args<float>(s, euler_step, t);
+    a.prev_row = prev_row;
+    a.step_id = (int)step_id;
+    launch_supg_npe<float>(s, a, dyn, press, s->diag_enabled);
+  } else {
+    SupgArgs<double> a = supg_args<double>(s, euler_step, t);
+    a.prev_row = prev_row;
+    a.step_id = (int)step_id;
+    launch_supg_npe<double>(s, a, dyn, press, s->diag_enabled);
+  }
+  FEMX_HIP(hipGetLastError());
+  return FEMX_OK;
+}
+
+femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
+  Blocks &b = s->mesh->blk;
+  if (b.n_sh == 0) return FEMX_OK;
+  TimedLaunch tl(s->timer, s->timer.kind("supg_fixup", b.sh_off[b.n_sh] * 64.0 + b.n_sh * 200.0), s->st);
+  SupgFix a;
+  a.n_sh = b.n_sh;
+  a.sh_node = b.d_sh_node.p;
+  a.sh_off = b.d_sh_off.p;
+  a.partials = s->partials.p;
+  a.xyzm = s->xyzm.p;
+  a.U = s->U[s->cur].p;
+  a.Un = s->U[s->cur ^ 1].p;
+  a.Unp = s->p.fixed_dofs ? s->Up[s->cur ^ 1].p : nullptr;
+  a.Fprev = s->Fprev.p;
+  a.node_rec = s->bc.node_rec.p;
+  a.rec = s->bc.rec.p;
+  a.row_off = s->bc.row_off.p;
+  a.rows = s->bc.rows.p;
+  a.err = s->bc.err.p;
+  a.halt = s->halt.p;
+  a.k = supg_const(s, t, euler_step);
+  supg_fixup<<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  FEMX_HIP(hipGetLastError());
+  return FEMX_OK;
+}
+
+femx_status flush_pressure(femx_supg *s) {
+  if (!s->press_pending) return FEMX_OK;
+  femx_status r = launch_supg(s, false, true, true, s->time, s->loop_time, nullptr);
+  if (r != FEMX_OK) return r;
+  s->press_pending = false;
+  return FEMX_OK;
+}
+
+femx_status check_bc_err(femx_supg *s) {
+  int e = 0;
+  FEMX_HIP(hipMemcpyAsync(&e, s->bc.err.p, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  if (e) return set_error("Could not find time in femUtils::interpTableData."), FEMX_ERR_BC;
+  return FEMX_OK;
+}
+
+}  // namespace
+
+extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, const femx_supg_params *p,
+                                        femx_supg **out) {
+  if (!mh || !p || !out) return set_error("null argument"), FEMX_ERR_ARG;
+  Mesh &m = mh->m;
+  if (m.npe == 4 && p->use_bhat)
+    return set_error("B-hat needs hexa8 (eval_avg_shapeDeriv reads 8 nodes)"), FEMX_ERR_UNSUPPORTED;
+  if (p->fixed_dofs < 0 || p->fixed_dofs > 6) return set_error("ERROR: Invalid global fixed DOFs string."), FEMX_ERR_ARG;
+  if (p->precision != FEMX_FP64 && p->precision != FEMX_MIXED) return set_error("precision"), FEMX_ERR_ARG;
+  if (!(p->dt > 0.0) || p->rho == 0.0) return set_error("parameters"), FEMX_ERR_ARG;
+  const int threads = m.npe == 4 ? 256 : 128;
+  if (m.blk.B > threads || m.blk.max_local > 256)
+    return set_error("element blocks do not fit the SUPG workgroup"), FEMX_ERR_ARG;
+  femx_status r;
+  if ((r = upload_tables(m.device)) != FEMX_OK) return r;
+  femx_supg *s = new femx_supg();
+  s->mesh = &m;
+  s->p = *p;
+  auto fail = [&](femx_status st) {
+    femx_supg_destroy(s);
+    return st;
+  };
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(FEMX_ERR_HIP);
+  if ((r = s->bc.build(bc, m.N, false, s->st)) != FEMX_OK) return fail(r);
+  if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
+  if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->Fprev.alloc(m.N)) != FEMX_OK) return fail(r);
+  for (int k = 0; k < 2; k++) {
+    if ((r = s->U[k].alloc(m.N)) != FEMX_OK) return fail(r);
+    if (p->fixed_dofs && (r = s->Up[k].alloc(m.N)) != FEMX_OK) return fail(r);
+  }
+  const int nparts = m.blk.n_sh ? m.blk.sh_off[m.blk.n_sh] : 0;
+  if ((r = s->partials.alloc(std::max(nparts, 1))) != FEMX_OK) return fail(r);
+  if ((r = s->lam.alloc(m.E)) != FEMX_OK || (r = s->regime.alloc(m.E)) != FEMX_OK || (r = s->pres.alloc(m.E)) != FEMX_OK)
+    return fail(r);
+  if ((r = s->ring.alloc((size_t)RING * D_COLS)) != FEMX_OK || (r = s->halt.alloc(1)) != FEMX_OK) return fail(r);
+  if ((r = s->eblk.alloc((size_t)m.blk.nb * D_COLS)) != FEMX_OK) return fail(r);
+  if ((r = s->nblk.alloc((size_t)grid_for(m.N, 256) * D_COLS)) != FEMX_OK) return fail(r);
+  if (hipMemsetAsync(s->ring.p, 0, (size_t)RING * D_COLS * sizeof(double), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
+  if (m.npe == 8)
+    supg_mass<8><<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, m.conn.p, m.xyz.p, p->rho, s->xyzm.p);
+  else
+    supg_mass<4><<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, m.conn.p, m.xyz.p, p->rho, s->xyzm.p);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
+  *out = s;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_destroy(femx_supg *s) {
+  if (!s) return FEMX_OK;
+  hipStream_t st = s->st;
+  if (st) (void)hipStreamSynchronize(st);
+  delete s;
+  if (st) (void)hipStreamDestroy(st);
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_set_state(femx_supg *s, const double *sol) {
+  if (!s || !sol) return set_error("null argument"), FEMX_ERR_ARG;
+  Mesh &m = *s->mesh;
+  std::vector<double4> h(m.N);
+  for (int n = 0; n < m.N; n++) h[n] = make_double4(sol[3L * n], sol[3L * n + 1], sol[3L * n + 2], 0.0);
+  s->cur = 0;
+  s->loop_time = 0;
+  s->time = 0.0;
+  s->halted = false;
+  FEMX_HIP(hipMemcpyAsync(s->U[0].p, h.data(), m.N * sizeof(double4), hipMemcpyHostToDevice, s->st));
+  FEMX_HIP(hipMemsetAsync(s->bc.err.p, 0, sizeof(int), s->st));
+  const int imax = INT_MAX;
+  FEMX_HIP(hipMemcpyAsync(s->halt.p, &imax, sizeof(int), hipMemcpyHostToDevice, s->st));
+  if (s->bc.n_rec > 0)
+    supg_bc_only<<<grid_for(s->bc.n_rec, 256), 256, 0, s->st>>>(s->bc.n_rec, s->rec_node.p, s->bc.rec.p,
+                                                                s->bc.row_off.p, s->bc.rows.p, 0.0, s->U[0].p,
+                                                                s->bc.err.p);
+  FEMX_HIP(hipGetLastError());
+  if (s->p.fixed_dofs)
+    FEMX_HIP(hipMemcpyAsync(s->Up[0].p, s->U[0].p, m.N * sizeof(double4), hipMemcpyDeviceToDevice, s->st));
+  FEMX_HIP(hipMemsetAsync(s->lam.p, 0, m.E * sizeof(double), s->st));  // el_lam = 0 at the initial pressure (:896)
+  FEMX_HIP(hipMemsetAsync(s->regime.p, 0, m.E * sizeof(double), s->st));
+  FEMX_HIP(hipMemsetAsync(s->Fprev.p, 0, m.N * sizeof(double4), s->st));
+  s->press_pending = true;
+  femx_status r = check_bc_err(s);
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  return r;
+}
+
+extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done) {
+  if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (steps_done) *steps_done = 0;
+  if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
+  const long first = s->loop_time;
+  const double t0 = s->time;
+  const int cur0 = s->cur;
+  for (int k = 0; k < nsteps; k++) {
+    s->loop_time++;
+    s->time += s->p.dt;  // currentTime += timeStep (:1041)
+    const bool euler_step = s->p.use_euler || s->loop_time == 1;
+    double *row = s->ring.p + (s->loop_time % RING) * D_COLS;
+    const double *prev = s->loop_time > 1 ? s->ring.p + ((s->loop_time - 1) % RING) * D_COLS : nullptr;
+    femx_status r = launch_supg(s, true, s->press_pending, euler_step, s->time, s->loop_time, prev);
+    if (r != FEMX_OK) return r;
+    if ((r = launch_fix(s, euler_step, s->time)) != FEMX_OK) return r;
+    s->cur ^= 1;
+    s->press_pending = true;
+    const int nnb = grid_for(s->mesh->N, 256);
+    supg_node_diag<<<nnb, 256, 0, s->st>>>(s->mesh->N, s->U[s->cur].p, s->nblk.p, s->halt.p);
+    supg_diag_reduce<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0, row,
+                                           s->halt.p);
+    FEMX_HIP(hipGetLastError());
+  }
+  // the exit condition of the last step, and any halt inside the batch
+  int halt = INT_MAX;
+  double last[D_COLS];
+  FEMX_HIP(hipMemcpyAsync(&halt, s->halt.p, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipMemcpyAsync(last, s->ring.p + (s->loop_time % RING) * D_COLS, sizeof(last), hipMemcpyDeviceToHost,
+                          s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  femx_status r = check_bc_err(s);
+  if (r != FEMX_OK) return r;
+  long stop = -1;
+  if (halt != INT_MAX) {
+    stop = halt;
+  } else if (nsteps > 0 && last[D_UMOD] > s->p.exit_condition * s->p.exit_condition) {
+    stop = s->loop_time;
+  }
+  if (stop >= 0) {
+    // rewind the host counters to the step that exceeded the threshold
+    const long done = stop - first;
+    s->loop_time = stop;
+    s->time = t0;
+    for (long k = 0; k < done; k++) s->time += s->p.dt;
+    s->cur = (int)((cur0 + done) & 1);
+    if (steps_done) *steps_done = (int)done;
+    s->halted = true;
+    set_error("ERROR: Maximum velocity module larger than admissible threshold.");
+    return FEMX_EXIT_CONDITION;
+  }
+  if (steps_done) *steps_done = nsteps;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_get_state(femx_supg *s, double *sol, double *pres, double *lam, double *regime,
+                                           double *time) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  Mesh &m = *s->mesh;
+  femx_status r = flush_pressure(s);
+  if (r != FEMX_OK) return r;
+  if (sol) {
+    std::vector<double4> h(m.N);
+    FEMX_HIP(hipMemcpyAsync(h.data(), s->U[s->cur].p, m.N * sizeof(double4), hipMemcpyDeviceToHost, s->st));
+    FEMX_HIP(hipStreamSynchronize(s->st));
+    for (int n = 0; n < m.N; n++) {
+      sol[3L * n] = h[n].x;
+      sol[3L * n + 1] = h[n].y;
+      sol[3L * n + 2] = h[n].z;
+    }
+  }
+  double *outs[3] = {pres, lam, regime};
+  DevBuf<double> *srcs[3] = {&s->pres, &s->lam, &s->regime};
+  for (int k = 0; k < 3; k++) {
+    if (!outs[k]) continue;
+    if ((r = s->scratch.alloc(m.E)) != FEMX_OK) return r;
+    block_to_orig<<<grid_for(m.E, 256), 256, 0, s->st>>>(m.E, m.blk.d_perm.p, srcs[k]->p, s->scratch.p);
+    FEMX_HIP(hipGetLastError());
+    FEMX_HIP(hipMemcpyAsync(outs[k], s->scratch.p, m.E * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    FEMX_HIP(hipStreamSynchronize(s->st));
+  }
+  if (time) *time = s->time;
+  return check_bc_err(s);
+}
+
+extern "C" femx_status femx_supg_get_mass(femx_supg *s, double *l_M) {
+  if (!s || !l_M) return set_error("null argument"), FEMX_ERR_ARG;
+  const int N = s->mesh->N;
+  std::vector<double4> x(N);
+  FEMX_HIP(hipMemcpyAsync(x.data(), s->xyzm.p, N * sizeof(double4), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  for (int n = 0; n < N; n++) l_M[3L * n] = l_M[3L * n + 1] = l_M[3L * n + 2] = x[n].w;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_get_log(femx_supg *s, double *cols19) {
+  if (!s || !cols19) return set_error("null argument"), FEMX_ERR_ARG;
+  double k[D_COLS];
+  FEMX_HIP(hipMemcpyAsync(k, s->ring.p + (s->loop_time % RING) * D_COLS, sizeof(k), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  cols19[0] = (double)s->loop_time;
+  cols19[1] = s->time;
+  for (int i = 0; i < 17; i++) cols19[2 + i] = k[i];
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_set_diagnostics(femx_supg *s, int enable) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  s->diag_enabled = enable != 0;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_sync(femx_supg *s) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  return FEMX_OK;
+}
+
+extern "C" void *femx_supg_stream(femx_supg *s) { return s ? (void *)s->st : nullptr; }
+
+extern "C" femx_status femx_supg_kernel_timing(femx_supg *s, int enable) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  s->timer.resolve();
+  s->timer.enabled = enable != 0;
+  if (enable)
+    for (auto &a : s->timer.acc) {
+      a.launches = 0;
+      a.ms = 0.0;
+    }
+  return FEMX_OK;
+}
+
+extern "C" int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int max_entries) {
+  if (!s) return -1;
+  return s->timer.export_to(out, max_entries);
+}

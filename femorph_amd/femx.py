@@ -59,6 +59,7 @@ SIGNATURES = [
     ("femx_device_count", C.c_int, []),
     ("femx_mesh_upload", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("femx_mesh_free", C.c_int, [C.c_void_p]),
+    ("femx_mesh_block_stats", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_longlong)]),
     ("femx_vms_create", C.c_int, [C.c_void_p, C.POINTER(BCDesc), C.POINTER(VMSParams), C.POINTER(C.c_void_p)]),
     ("femx_vms_destroy", C.c_int, [C.c_void_p]),
     ("femx_vms_set_state", C.c_int, [C.c_void_p, _dp, _dp]),
@@ -75,6 +76,7 @@ SIGNATURES = [
     ("femx_supg_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     ("femx_supg_get_mass", C.c_int, [C.c_void_p, _dp]),
     ("femx_supg_get_log", C.c_int, [C.c_void_p, _dp]),
+    ("femx_supg_set_diagnostics", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_sync", C.c_int, [C.c_void_p]),
     ("femx_supg_stream", C.c_void_p, [C.c_void_p]),
     ("femx_vms_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
@@ -114,6 +116,17 @@ def _check(status):
 
 def _p(a, t=_dp):
     return a.ctypes.data_as(t)
+
+
+def block_stats(mesh: Mesh, block_elems: int) -> dict:
+    """Host-only statistics of the element-block partition (no GPU needed)."""
+    coords = np.ascontiguousarray(mesh.coords, np.float64)
+    conn = np.ascontiguousarray(mesh.conn, np.int32)
+    d = MeshDesc(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(coords), _p(conn, _ip))
+    st = (C.c_longlong * 8)()
+    _check(load().femx_mesh_block_stats(C.byref(d), block_elems, st))
+    keys = ["blocks", "elements", "local_nodes", "max_local", "shared_nodes", "partials", "complete", "max_elems"]
+    return dict(zip(keys, [int(x) for x in st]))
 
 
 def device_count() -> int:
@@ -262,6 +275,9 @@ class SUPGSolver:
         m = np.zeros(3 * self.N)
         _check(load().femx_supg_get_mass(self._h, _p(m)))
         return m
+
+    def set_diagnostics(self, enable=True):
+        _check(load().femx_supg_set_diagnostics(self._h, int(enable)))
 
     def get_log(self):
         v = np.zeros(19)

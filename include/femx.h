@@ -82,6 +82,11 @@ int femx_device_count(void);
  * chasing over model->nodeList / elementList (src/femModel.h:24-25). */
 femx_status femx_mesh_upload(const femx_mesh_desc *desc, femx_mesh **out);
 femx_status femx_mesh_free(femx_mesh *mesh);
+/* Host-only statistics of the element-block partition femx_mesh_upload
+ * builds (no device needed): stats[0..7] = blocks, elements, block-local
+ * nodes, max local nodes per block, nodes shared between blocks, partial
+ * sum slots, complete local nodes, max elements per block. */
+femx_status femx_mesh_block_stats(const femx_mesh_desc *desc, int block_elems, long long *stats);
 
 /* ------------------------------------------------------------------ VMS */
 /* vmsProps[0..4] = rho, mu, invEps, c1, c2 (src/femModel.h:32, read at
@@ -148,8 +153,13 @@ femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done);
  * el_regime; any may be NULL. */
 femx_status femx_supg_get_state(femx_supg *s, double *sol, double *pres, double *lam, double *regime, double *time);
 femx_status femx_supg_get_mass(femx_supg *s, double *l_M /*[N*3]*/);
-/* The reference's 19 log columns of the last step (:1601-1630). */
+/* The reference's 19 log columns of the last step (:1601-1630): step, time,
+ * min/max u v w, min/max tau, min/max lambda, viscous / advective / lambda
+ * CFL, min/max E, min/max nu.  Columns 8..18 are evaluated only while the
+ * element diagnostics are enabled (they cost an extra 3x3 inverse per Gauss
+ * point); otherwise they hold the reference's initial values. */
 femx_status femx_supg_get_log(femx_supg *s, double *cols19);
+femx_status femx_supg_set_diagnostics(femx_supg *s, int enable);
 femx_status femx_supg_sync(femx_supg *s);
 void *femx_supg_stream(femx_supg *s);
 

@@ -1,0 +1,148 @@
+"""SUPG (hexa8 and tet4) on the GPU vs the FP64 oracle (restatement of
+src/femSUPGExplicitFluidSolver.cpp), through the C ABI.
+
+Plain FP64 tolerance: 1e-12 relative L2 per step on the velocity, the
+post-processed element pressure and the element bulk modulus."""
+import numpy as np
+import pytest
+
+from femorph_amd import meshgen
+from oracle.oracle import SUPGOracle
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def _case(kind="hexa", n=5, jitter=0.2, seed=1, bc="cavity"):
+    m = meshgen.hex_cube(n, jitter, seed) if kind == "hexa" else meshgen.tet_cube(n, jitter, seed)
+    b = meshgen.lid_cavity_bc(m) if bc == "cavity" else meshgen.channel_bc(m)
+    params = dict(rho=1.0, mu=0.0025, alpha=1.0, cI=4.0, dt=0.1 / n)
+    ic = meshgen.random_velocity(m, 1e-3, seed + 3)[:, :3]
+    return m, b, params, ic
+
+
+def _pair(femx, m, bc, params, ic, exopts, fixed="NONE", exit_condition=1e5, precision=0):
+    o = SUPGOracle(m, bc, exopts=exopts, fixed_dofs=fixed, exit_condition=exit_condition, **params)
+    o.set_ic(ic)
+    dm = femx.DeviceMesh(m)
+    g = femx.SUPGSolver(dm, bc, exopts=exopts, fixed_dofs=fixed, exit_condition=exit_condition,
+                        precision=precision, **params)
+    g.set_state(ic)
+    return o, g, dm
+
+
+def test_lumped_mass(gpu):
+    m, bc, params, ic = _case("hexa", 6)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
+    assert rel_l2(g.get_mass(), o.get("l_M").ravel()) <= 1e-14
+
+
+def test_initial_state(gpu):
+    m, bc, params, ic = _case("hexa", 4)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
+    st = g.get_state()
+    assert np.array_equal(st["sol"], o.get("sol_p"))
+    assert not st["pres"].any() and not st["lam"].any()
+
+
+@pytest.mark.parametrize("exopts", [(0, 1, 1, 1, 1, 1), (1, 0, 1, 1, 0, 0), (0, 0, 1, 1, 1, 1), (1, 1, 0, 1, 0, 1)])
+def test_hexa_per_step_parity(gpu, exopts):
+    m, bc, params, ic = _case("hexa", 5, 0.2, seed=2)
+    o, g, _ = _pair(gpu, m, bc, params, ic, exopts)
+    for step in range(12):
+        assert o.step() == 0
+        assert g.step() == 1
+        st = g.get_state()
+        assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64, step
+        assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
+        assert rel_l2(st["lam"], o.get("el_lam")) <= 1e-12, step
+        assert np.array_equal(st["regime"], o.get("el_regime")), step
+
+
+def test_hexa_channel_fused_run(gpu):
+    """Continuous stepping, pressure fused into the next element pass."""
+    m, bc, params, ic = _case("hexa", 6, 0.2, seed=4, bc="channel")
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
+    o.step(30)
+    assert g.step(30) == 30
+    st = g.get_state()
+    assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64
+    assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11
+    assert st["time"] == o.get("scalars")[0]
+
+
+def test_tet4_per_step_parity(gpu):
+    m, bc, params, ic = _case("tet", 4, 0.1, seed=5)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 0, 1, 1, 1, 1))
+    for step in range(10):
+        o.step()
+        g.step()
+        st = g.get_state()
+        assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64, step
+        assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
+
+
+def test_tet4_bhat_rejected(gpu):
+    m, bc, params, ic = _case("tet", 3)
+    dm = gpu.DeviceMesh(m)
+    with pytest.raises(gpu.FemxError):
+        gpu.SUPGSolver(dm, bc, exopts=(0, 1, 1, 1, 1, 1), **params)
+
+
+def test_fixed_dofs_z(gpu):
+    """NODEDOF 3 Z: pressure uses the state before the zeroing (:1524 vs :1527)."""
+    m, bc, params, ic = _case("hexa", 4, 0.15, seed=6)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), fixed="Z")
+    for step in range(8):
+        o.step()
+        g.step()
+        st = g.get_state()
+        assert not st["sol"][:, 2].any()
+        assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64, step
+        assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
+
+
+def test_log_columns(gpu):
+    m, bc, params, ic = _case("hexa", 5, 0.2, seed=7)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
+    g.set_diagnostics(True)
+    o.step(3)
+    g.step(3)
+    lo, lg = o.get("log"), g.get_log()
+    assert lg[0] == lo[0] and lg[1] == lo[1]
+    np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-12, atol=0)
+
+
+def test_exit_condition(gpu):
+    """max |u| > EXITCOND stops the solve at that step (reference: exit(-1))."""
+    m, bc, params, ic = _case("hexa", 4, 0.1, seed=8)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), exit_condition=0.99)
+    assert o.step() == 1
+    done = g.step(5)
+    assert done == 1
+    st = g.get_state()
+    assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64
+    assert g.step(1) == 0
+
+
+def test_deterministic(gpu):
+    m, bc, params, ic = _case("hexa", 6, 0.2, seed=9)
+    dm = gpu.DeviceMesh(m)
+    outs = []
+    for _ in range(2):
+        g = gpu.SUPGSolver(dm, bc, **params)
+        g.set_state(ic)
+        g.step(10)
+        outs.append(g.get_state())
+    for k in ("sol", "pres", "lam"):
+        assert np.array_equal(outs[0][k], outs[1][k])
+
+
+def test_mixed_precision_error(gpu):
+    m, bc, params, ic = _case("hexa", 6, 0.2, seed=10)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), precision=1)
+    o.step(60)
+    g.step(60)
+    err = rel_l2(g.get_state()["sol"], o.get("sol_n"))
+    assert err <= 1e-5, err

@@ -30,6 +30,22 @@ sys.path.insert(0, ROOT)
 from femorph_amd import femx, meshgen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vms_block_c3_traffic.json")
+PROFILED_KERNEL = {"mixed": "vms_block<float, true, true, false>", "fp64": "vms_block<double, true, true, false>"}
+
+
+def profiled_traffic(precision: str, n: int):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy calibration),
+    or None when the profile does not cover this workload."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+    except OSError:
+        return None
+    if n != 96 or "96^3x6" not in t.get("workload", ""):
+        return None
+    k = t["kernels"].get(PROFILED_KERNEL[precision])
+    return k and k["traffic_bytes"]
 METRIC = "element-steps/sec (VMS tet4) + % HBM roofline; p50/p99 solve latency"
 
 
@@ -148,7 +164,8 @@ def main():
     step_bytes = algorithmic_bytes_per_element_step(r, mixed) * E
     step_ms_events = sum(v["total_ms"] for v in kt.values()) / args.steps
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.precision, args.n),
+            "kernel": dom_name,
             "kernel_ms": round(dom_ms, 4), "kernel_bytes": dom_bytes,
             "step_achieved": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
             "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -34,17 +34,46 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+HOST = os.path.join(CSRC, "host")
+MODEL_LIB = os.path.join(HERE, "libfemx_model.so")
+DRIVER = os.path.join(HERE, "femx_explicit")
+HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp"]
+GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-result"]
+
+
+def _run(cmd, verbose, what):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libfemx.so")
+        raise RuntimeError(f"build of {what} failed")
+
+
+def build_host(verbose: bool = False) -> None:
+    """The drop-in C++ model and solver classes (libfemx_model.so, linked to
+    libfemx.so) and the femx_explicit driver, with the host compiler."""
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", HOST]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES]
+    tmp = MODEL_LIB + ".tmp"
+    _run(["g++"] + GXX_FLAGS + ["-fPIC", "-shared"] + inc + ["-o", tmp] + srcs +
+         ["-L", HERE, "-l:libfemx.so", rpath], verbose, "libfemx_model.so")
+    os.replace(tmp, MODEL_LIB)
+    tmp = DRIVER + ".tmp"
+    _run(["g++"] + GXX_FLAGS + inc + ["-o", tmp, os.path.join(HOST, "femx_explicit.cpp"),
+                                      "-L", HERE, "-l:libfemx_model.so", "-l:libfemx.so", rpath],
+         verbose, "femx_explicit")
+    os.replace(tmp, DRIVER)
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+    _run(cmd, verbose, "libfemx.so")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

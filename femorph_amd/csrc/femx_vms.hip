@@ -44,15 +44,16 @@ namespace {
 
 // Gauss data of the reference (truncated 8-digit constants), :4-13.
 constexpr double GA = 0.58541020, GB = 0.13819660;
-constexpr int VB = 256;  // threads (= max elements and max local nodes) per block
+constexpr int VB = 256;  // threads = max elements per block (femx_mesh_upload cuts tet4 meshes so)
+constexpr int VN = 256;  // max local nodes per block (u8 local node ids)
 
 template <typename T>
 __device__ __forceinline__ T lnw(int g, int a) {
   return (T)(g == a ? GA : GB);
 }
 
-// initial_assemble geometry (:44-83) in FP64: DN[i][a], V = detJ/6 (signed),
-// h = (6V)^(1/3).
+// initial_assemble geometry (:44-83) in the reference's FP64 operation order:
+// DN[i][a], V = detJ/6 (signed), h = (6V)^(1/3).  Used for the lumped mass.
 __device__ __forceinline__ void tet_geometry(const double X[4][3], double DN[3][4], double &V, double &h) {
   double jac[3][3];
 #pragma unroll
@@ -84,152 +85,196 @@ __device__ __forceinline__ void tet_geometry(const double X[4][3], double DN[3][
   h = cbrt(6.0 * V);
 }
 
+// The same geometry for the element kernel: edge vectors formed in FP64 (no
+// cancellation), the cofactors, inverse and DN in T.  Returns V and h in T.
+template <typename T>
+__device__ __forceinline__ void tet_geometry_t(const double X[4][3], T DN[3][4], T &V, T &h) {
+  T jac[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) jac[i][j] = (T)(X[j + 1][i] - X[0][i]);
+  T cof[3][3];
+  cof[0][0] = jac[1][1] * jac[2][2] - jac[2][1] * jac[1][2];
+  cof[0][1] = jac[2][0] * jac[1][2] - jac[1][0] * jac[2][2];
+  cof[0][2] = jac[1][0] * jac[2][1] - jac[2][0] * jac[1][1];
+  cof[1][0] = jac[2][1] * jac[0][2] - jac[0][1] * jac[2][2];
+  cof[1][1] = jac[0][0] * jac[2][2] - jac[2][0] * jac[0][2];
+  cof[1][2] = jac[2][0] * jac[0][1] - jac[0][0] * jac[2][1];
+  cof[2][0] = jac[0][1] * jac[1][2] - jac[1][1] * jac[0][2];
+  cof[2][1] = jac[1][0] * jac[0][2] - jac[0][0] * jac[1][2];
+  cof[2][2] = jac[0][0] * jac[1][1] - jac[1][0] * jac[0][1];
+  const T detJ = jac[0][0] * cof[0][0] + jac[0][1] * cof[0][1] + jac[0][2] * cof[0][2];
+  const T iDet = (T)1 / detJ;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const T i0 = cof[i][0] * iDet, i1 = cof[i][1] * iDet, i2 = cof[i][2] * iDet;
+    DN[i][1] = i0;
+    DN[i][2] = i1;
+    DN[i][3] = i2;
+    DN[i][0] = -i0 - i1 - i2;
+  }
+  V = detJ * (T)(1.0 / 6.0);
+  h = cbrt(detJ);  // (6 V)^(1/3)
+}
+
 struct VmsConst {
   double nu, inv_eps, c1, c2, inv_dt, dual_step, t;
 };
 
-// Projection + subscale update of one element (assemble_projRes_explicit
-// :406-588 and update_subscales_explicit :593-646, element-based projection,
-// include_sg_p = false so the pressure subscale stays 0).
-// up: pre-BC nodal velocity; q: pressures of global nodes 0..3.
+// The reference's Gauss weights are lN[g][a] = GB + GD * (g == a) with
+// GD = GA - GB, so every Gauss-point sum of nodal data collapses to
+//   sum_b lN[g][b] f_b = GD f_g + GB sum_b f_b,
+// and the projection (element L2 projection onto the P1 space with the
+// lumped denominator, :557-577) followed by its re-interpolation (:626-630)
+// is  sum_b lN[g][b] Pi_b = (GD^2 r_g + (4 GB^2 + 2 GB GD) sum_h r_h) / S
+// with S = GA + 3 GB.  These identities hold exactly in real arithmetic;
+// only the rounding differs from the reference's loop order.
+constexpr double GD = GA - GB;
+constexpr double GS = GA + 3.0 * GB;
+constexpr double PR_SELF = GD * GD / GS - 1.0;                // coefficient of r_g in ũ' (incl. the -r_g of :548)
+constexpr double PR_SUM = (4.0 * GB * GB + 2.0 * GB * GD) / GS;  // coefficient of sum_h r_h
+constexpr double M_DIAG = GA * GA + 3.0 * GB * GB;            // sum_g lN[g][b]^2
+constexpr double M_OFF = 2.0 * GA * GB + 2.0 * GB * GB;       // sum_g lN[g][b] lN[g][c], b != c
+
+// Nodal velocities of one element in T with their sum and the (constant)
+// velocity gradient grad[i][j] = d u_i / d x_j (:259-268, :499-506).
 template <typename T>
-__device__ __forceinline__ void vms_projection(const T DN[3][4], double hd, const double up[4][3], const double q[4],
-                                               T sdu[4][3], const VmsConst &k) {
-  T u[4][3], du[3][3], dq[3];
+struct ElemVel {
+  T u[4][3], su[3], grad[3][3];
+};
+template <typename T>
+__device__ __forceinline__ void elem_vel(const T DN[3][4], const double uv[4][3], ElemVel<T> &ev) {
+  T du[3][3];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) u[a][i] = (T)up[a][i];
+    for (int i = 0; i < 3; i++) ev.u[a][i] = (T)uv[a][i];
 #pragma unroll
   for (int a = 1; a < 4; a++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) du[a - 1][i] = (T)(up[a][i] - up[0][i]);
-#pragma unroll
-  for (int a = 1; a < 4; a++) dq[a - 1] = (T)(q[a] - q[0]);
-  T gradDu[3][3], gradP[3];
+    for (int i = 0; i < 3; i++) du[a - 1][i] = (T)(uv[a][i] - uv[0][i]);  // sum_a DN[j][a] = 0
 #pragma unroll
   for (int i = 0; i < 3; i++) {
+    ev.su[i] = (ev.u[0][i] + ev.u[1][i]) + (ev.u[2][i] + ev.u[3][i]);
 #pragma unroll
-    for (int j = 0; j < 3; j++) gradDu[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
-    gradP[i] = DN[i][1] * dq[0] + DN[i][2] * dq[1] + DN[i][3] * dq[2];
+    for (int j = 0; j < 3; j++) ev.grad[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
   }
-  const T inv_dt = (T)k.inv_dt;
-  T maxA2 = (T)0, r[4][3], sdp[4][3];
+}
+
+// Projection + subscale update of one element (assemble_projRes_explicit
+// :406-588 and update_subscales_explicit :593-646, element-based projection,
+// include_sg_p = false so the pressure subscale stays 0).
+// ev: pre-BC nodal velocity; q: pressures of global nodes 0..3.
+template <typename T>
+__device__ __forceinline__ void vms_projection(const T DN[3][4], T h, const ElemVel<T> &ev, const double q[4],
+                                               T sdu[4][3], const VmsConst &k) {
+  T dq[3];
+#pragma unroll
+  for (int a = 1; a < 4; a++) dq[a - 1] = (T)(q[a] - q[0]);
+  T gradP[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) gradP[i] = DN[i][1] * dq[0] + DN[i][2] * dq[1] + DN[i][3] * dq[2];
+  const T(&u)[4][3] = ev.u;
+  const T(&su)[3] = ev.su;
+  const T(&gradDu)[3][3] = ev.grad;
+  const T gb = (T)GB, gd = (T)GD;
+  T maxA2 = (T)0, r[4][3], rs[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     T hah[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
-      hah[i] = u[0][i] * lnw<T>(g, 0) + u[1][i] * lnw<T>(g, 1) + u[2][i] * lnw<T>(g, 2) + u[3][i] * lnw<T>(g, 3) +
-               sdu[g][i];
+    for (int i = 0; i < 3; i++) hah[i] = gd * u[g][i] + (gb * su[i] + sdu[g][i]);  // convective velocity :529-535
     const T m2 = hah[0] * hah[0] + hah[1] * hah[1] + hah[2] * hah[2];
     maxA2 = m2 > maxA2 ? m2 : maxA2;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      const T agd = gradDu[i][0] * hah[0] + gradDu[i][1] * hah[1] + gradDu[i][2] * hah[2];
-      sdp[g][i] = sdu[g][i] * inv_dt - agd - gradP[i];  // :548
-      r[g][i] = agd + gradP[i];                         // projected residual integrand :557
+      r[g][i] = (gradDu[i][0] * hah[0] + gradDu[i][1] * hah[1] + gradDu[i][2] * hah[2]) + gradP[i];
+      rs[i] += r[g][i];
     }
   }
-  // Pi[b] = sum_g w r_g N_gb / sum_g w N_gb  (w = V/4 cancels)
-  const T invS = (T)(1.0 / (GA + 3.0 * GB));
-  T Pi[4][3];
-#pragma unroll
-  for (int b = 0; b < 4; b++)
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-      Pi[b][i] = (r[0][i] * lnw<T>(0, b) + r[1][i] * lnw<T>(1, b) + r[2][i] * lnw<T>(2, b) + r[3][i] * lnw<T>(3, b)) *
-                 invS;
   // tau1 = 1 / (1/dt + 1/tau_u), 1/tau_u = c1 nu / h^2 + c2 max|a| / h  (:581-585)
-  const T h = (T)hd;
-  const T itu = (T)k.c1 * (T)k.nu / (h * h) + (T)k.c2 * sqrt(maxA2) / h;
+  const T inv_dt = (T)k.inv_dt;
+  const T ih = (T)1 / h;
+  const T itu = ((T)k.c1 * (T)k.nu * ih + (T)k.c2 * sqrt(maxA2)) * ih;
   const T tau1 = (T)1 / (inv_dt + itu);
+  const T ps = (T)PR_SELF, pm = (T)PR_SUM;
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const T pg = Pi[0][i] * lnw<T>(g, 0) + Pi[1][i] * lnw<T>(g, 1) + Pi[2][i] * lnw<T>(g, 2) + Pi[3][i] * lnw<T>(g, 3);
-      sdu[g][i] = (sdp[g][i] + pg) * tau1;  // :630
-    }
+    for (int i = 0; i < 3; i++) sdu[g][i] = (sdu[g][i] * inv_dt + ps * r[g][i] + pm * rs[i]) * tau1;  // :548 + :630
 }
 
 // Element residual lRes[4][4] of assemble_RHS (:93-399) with sgps = 0 and
 // force = 0.  uq: post-BC nodal (u v w p); uprev: the "previous" velocity
 // (only read with ACC, the acceleration term of later dual iterations).
+// Diffusion, pressure gradient and subgrid convection are one 3x3 matrix K
+// applied to DN: res[b][i] = sum_j K[i][j] DN[j][b] + convection.
 template <typename T, bool ACC>
-__device__ __forceinline__ void vms_residual(const T DN[3][4], double Vd, const double uq[4][4],
-                                             const double uprev[4][3], const T sdu[4][3], T res[4][4],
-                                             const VmsConst &k) {
-  T u[4][3], p[4], du[3][3];
+__device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVel<T> &ev, const T p[4],
+                                             const double uq[4][3], const double uprev[4][3], const T sdu[4][3],
+                                             T res[4][4], const VmsConst &k) {
+  const T(&u)[4][3] = ev.u;
+  const T(&su)[3] = ev.su;
+  const T(&gradU)[3][3] = ev.grad;
+  T ss[3];
+  const T psum = (p[0] + p[1]) + (p[2] + p[3]);
 #pragma unroll
-  for (int a = 0; a < 4; a++) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) u[a][i] = (T)uq[a][i];
-    p[a] = (T)uq[a][3];
-  }
-#pragma unroll
-  for (int a = 1; a < 4; a++)
-#pragma unroll
-    for (int i = 0; i < 3; i++) du[a - 1][i] = (T)(uq[a][i] - uq[0][i]);
-  T gradU[3][3];
+  for (int i = 0; i < 3; i++) ss[i] = (sdu[0][i] + sdu[1][i]) + (sdu[2][i] + sdu[3][i]);
+  const T tr = gradU[0][0] + gradU[1][1] + gradU[2][2];
+  const T w = (T)0.25 * V;
+  const T gb = (T)GB, gd = (T)GD;
+  // Gauss-point convective velocity a_g = sum_b lN u_b + sdu_g (:298-311),
+  // (grad u) a_g (:316-322) and T = sum_g sdu_g (x) a_g
+  T agu[4][3], ags[3] = {(T)0, (T)0, (T)0}, Tm[3][3];
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) gradU[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
-  const T S = (T)(GA + 3.0 * GB);
-  const T tr = gradU[0][0] + gradU[1][1] + gradU[2][2];
-  const T V = (T)Vd, w = (T)(0.25 * Vd);
-  const T nu = (T)k.nu, ie = (T)k.inv_eps;
-  const T psum = S * (p[0] + p[1] + p[2] + p[3]);  // sum_g p_g
-  // diffusion (sum_g w = V), pressure gradient, continuity base term
+    for (int j = 0; j < 3; j++) Tm[i][j] = (T)0;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    T ag[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) ag[i] = gd * u[g][i] + (gb * su[i] + sdu[g][i]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      agu[g][i] = gradU[i][0] * ag[0] + gradU[i][1] * ag[1] + gradU[i][2] * ag[2];
+      ags[i] += agu[g][i];
+#pragma unroll
+      for (int j = 0; j < 3; j++) Tm[i][j] += sdu[g][i] * ag[j];
+    }
+  }
+  // K = V nu grad u  -  w T  -  w sum_g p_g I   (sum_g w = V, sum_g p_g = S sum_b p_b)
+  const T vnu = V * (T)k.nu, wps = w * (T)GS * psum;
+  T K[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) K[i][j] = vnu * gradU[i][j] - w * Tm[i][j] - (i == j ? wps : (T)0);
+  const T wgd = w * gd, wgb = w * gb;
+  const T c0 = w * (T)k.inv_eps * tr * (T)GS, c1 = -w * (T)k.inv_eps;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
-      res[b][i] = V * nu * (gradU[i][0] * DN[0][b] + gradU[i][1] * DN[1][b] + gradU[i][2] * DN[2][b]) -
-                  w * psum * DN[i][b];
-    res[b][3] = w * ie * tr * S;
+      res[b][i] = (K[i][0] * DN[0][b] + K[i][1] * DN[1][b] + K[i][2] * DN[2][b]) + (wgd * agu[b][i] + wgb * ags[i]);
+    res[b][3] = c0 + c1 * (ss[0] * DN[0][b] + ss[1] * DN[1][b] + ss[2] * DN[2][b]);  // continuity :357-363
   }
-  T ssum[3] = {(T)0, (T)0, (T)0};
-#pragma unroll
-  for (int g = 0; g < 4; g++) {
-    T ag[3], agu[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-      ag[i] = u[0][i] * lnw<T>(g, 0) + u[1][i] * lnw<T>(g, 1) + u[2][i] * lnw<T>(g, 2) + u[3][i] * lnw<T>(g, 3) +
-              sdu[g][i];
-#pragma unroll
-    for (int i = 0; i < 3; i++) agu[i] = gradU[i][0] * ag[0] + gradU[i][1] * ag[1] + gradU[i][2] * ag[2];
-#pragma unroll
-    for (int i = 0; i < 3; i++) ssum[i] += sdu[g][i];
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const T ahDN = ag[0] * DN[0][b] + ag[1] * DN[1][b] + ag[2] * DN[2][b];
-#pragma unroll
-      for (int i = 0; i < 3; i++) res[b][i] += w * (agu[i] * lnw<T>(g, b) - sdu[g][i] * ahDN);  // :326-347
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < 4; b++) res[b][3] -= w * ie * (ssum[0] * DN[0][b] + ssum[1] * DN[1][b] + ssum[2] * DN[2][b]);
   if (ACC) {
-    T dd[4][3];
+    // acceleration (:324-325): sum_g w lN[g][b] sum_c lN[g][c] (u_c - u'_c) / dt
+    T dd[4][3], ds[3];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int i = 0; i < 3; i++) dd[a][i] = (T)(uq[a][i] - uprev[a][i]);
-    const T inv_dt = (T)k.inv_dt;
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      T acc[3];
+    for (int i = 0; i < 3; i++) ds[i] = (dd[0][i] + dd[1][i]) + (dd[2][i] + dd[3][i]);
+    const T wi = w * (T)k.inv_dt;
+    const T md = (T)(M_DIAG - M_OFF) * wi, mo = (T)M_OFF * wi;
 #pragma unroll
-      for (int i = 0; i < 3; i++)
-        acc[i] = (dd[0][i] * lnw<T>(g, 0) + dd[1][i] * lnw<T>(g, 1) + dd[2][i] * lnw<T>(g, 2) +
-                  dd[3][i] * lnw<T>(g, 3)) * inv_dt;
+    for (int b = 0; b < 4; b++)
 #pragma unroll
-      for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) res[b][i] += w * acc[i] * lnw<T>(g, b);  // :324-325
-    }
+      for (int i = 0; i < 3; i++) res[b][i] += md * dd[b][i] + mo * ds[i];
   }
 }
 
@@ -252,6 +297,10 @@ struct BlockArgs {
   const double4 *rows;
   int *err;
   double *bmax;  // [nblk] per-block max of the pre-BC |u|^2
+  // element diagnostics (eval_centroid_*, :699-724), NULL when off:
+  const int *perm;   // block order -> original element id
+  double *diag_res;  // [E][4][4] element residual contributions, original order
+  double *diag_sdu;  // [E][3] mean over the Gauss points of the subgrid velocity the residual used
   VmsConst k;
 };
 
@@ -260,23 +309,28 @@ struct BlockArgs {
 template <int NT>
 __device__ __forceinline__ void block_max_store(double v, double *out) {
   __shared__ double wm[NT / 64];
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = wm[0];
 #pragma unroll
-    for (int i = 1; i < NT / 64; i++) m = fmax(m, wm[i]);
+    for (int i = 1; i < NT / 64; i++) m = wm[i] > m ? wm[i] : m;
     out[blockIdx.x] = m;
   }
 }
 
-// update_dual_solution of one node + its BCs; records the pre-BC |u|^2.
-__device__ __forceinline__ double node_update(const double4 base, double m, const double R[4], const VmsConst &k,
+// update_dual_solution of one node (:764; im = 1 / nd_mass) + its BCs;
+// returns the pre-BC |u|^2.
+__device__ __forceinline__ double node_update(const double4 base, double im, const double R[4], const VmsConst &k,
                                               double4 &pre, double4 &post, int ri, const BCRecord *rec,
                                               const int *row_off, const double4 *rows, int *err) {
-  double u = base.x - k.dual_step * R[0] / m, v = base.y - k.dual_step * R[1] / m;
-  double w = base.z - k.dual_step * R[2] / m, p = base.w - k.dual_step * R[3] / m;
+  const double ds = k.dual_step * im;
+  double u = base.x - ds * R[0], v = base.y - ds * R[1];
+  double w = base.z - ds * R[2], p = base.w - ds * R[3];
   pre = make_double4(u, v, w, p);
   const double m2 = u * u + v * v + w * w;
   if (ri >= 0) apply_bc(rec[ri], row_off, rows, k.t, u, v, w, &p, err);
@@ -284,21 +338,51 @@ __device__ __forceinline__ double node_update(const double4 base, double m, cons
   return m2;
 }
 
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
+// XOR swizzle of a record index inside each 256-byte LDS row group, so that
+// lanes reading records 16 (float4) or 8 (double4) apart hit different banks.
+template <typename R>
+__device__ __forceinline__ int swz(int p) {
+  constexpr int W = 256 / (int)sizeof(R);
+  return p ^ ((p / W) & (W - 1));
+}
+
+// Row `row_elems` of an SoA array, addressed as a wave-uniform base plus a
+// zero-extended 32-bit byte offset (SGPR base + VGPR offset addressing).
+template <typename T>
+__device__ __forceinline__ T *row_ptr(T *base, size_t row_elems, unsigned byte_off) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(base + row_elems) + (size_t)byte_off);
+}
+
 template <typename T, bool PROJ, bool RES, bool ACC>
 __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
-  __shared__ double sx[VB], sy[VB], sz[VB], sm[VB];
-  __shared__ double spu[VB], spv[VB], spw[VB], spp[ACC ? VB : 1];
-  __shared__ double squ[RES ? VB : 1], sqv[RES ? VB : 1], sqw[RES ? VB : 1], sqp[RES ? VB : 1];
-  // element-corner contributions in the block's node-sorted order, per dof
-  __shared__ T sres[RES ? 16 * VB : 1];
+  using T4 = typename Vec4<T>::type;
+  // staged node records as 16-byte pairs: {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}
+  __shared__ double2 snd[VN][5];
+  __shared__ double sim[VN], spp[ACC ? VN : 1];
+  __shared__ int sbc[VN];  // 1 for nodes with a BC record (pre-BC and post-BC states may differ)
+  // element-corner contributions (u v w p) in the block's node-sorted order,
+  // one T4 record per corner at a bank-swizzled position
+  __shared__ T4 sres[RES ? 4 * VB : 1];
   const int tid = threadIdx.x;
   const int blk = xcd_block(blockIdx.x, a.nblk);
   const int n0 = a.node_off[blk], nn = a.node_off[blk + 1] - n0;
   const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
   // 0. issue this thread's element loads before staging (they overlap)
   const bool has_e = tid < ne;
-  const int e = e0 + tid;
-  const long E = a.E;
+  const unsigned e = (unsigned)(e0 + tid);
+  const size_t E = (size_t)a.E;
+  const unsigned eoff = e * (unsigned)sizeof(T);  // byte offset inside a row (rows are < 4 GiB)
   uchar4 c = make_uchar4(0, 0, 0, 0);
   ushort4 pos = make_ushort4(0, 0, 0, 0);
   T sdu[4][3];
@@ -308,42 +392,35 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
 #pragma unroll
     for (int g = 0; g < 4; g++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) sdu[g][i] = a.sdus[(long)(g * 3 + i) * E + e];
+      for (int i = 0; i < 3; i++) sdu[g][i] = *row_ptr(a.sdus, (g * 3 + i) * E, eoff);
   }
-  // node-side metadata of the local node this thread owns (nn <= VB)
+  // node-side metadata of the local node this thread owns (nn <= VN <= VB)
   const bool has_n = tid < nn;
   int g_own = 0, ps_own = -1, s0_own = 0, s1_own = 0, rec_own = -1;
   if (has_n) {
     g_own = a.loc_node[n0 + tid];
+    rec_own = a.node_rec[g_own];
     if (RES) {
       ps_own = a.loc_pslot[n0 + tid];
       s0_own = a.lcsr_off[n0 + tid];
       s1_own = a.lcsr_off[n0 + tid + 1];
-      rec_own = a.node_rec[g_own];
     }
   }
   // 1. stage the block's nodes
-  for (int j = tid; j < nn; j += VB) {
-    const int g = g_own;
+  double4 q_own = make_double4(0, 0, 0, 0), p_own = make_double4(0, 0, 0, 0);
+  if (has_n) {
+    const int j = tid, g = g_own;
     const double4 x = a.xyzm[g];
-    sx[j] = x.x;
-    sy[j] = x.y;
-    sz[j] = x.z;
-    sm[j] = x.w;
-    if (PROJ || ACC) {
-      const double4 p = a.P[g];
-      spu[j] = p.x;
-      spv[j] = p.y;
-      spw[j] = p.z;
-      if (ACC) spp[j] = p.w;
-    }
-    if (RES) {
-      const double4 q = a.Q[g];
-      squ[j] = q.x;
-      sqv[j] = q.y;
-      sqw[j] = q.z;
-      sqp[j] = q.w;
-    }
+    if (PROJ || ACC) p_own = a.P[g];
+    if (RES) q_own = a.Q[g];
+    snd[j][0] = make_double2(x.x, x.y);
+    snd[j][1] = make_double2(x.z, p_own.x);
+    snd[j][2] = make_double2(p_own.y, p_own.z);
+    snd[j][3] = make_double2(q_own.x, q_own.y);
+    snd[j][4] = make_double2(q_own.z, q_own.w);
+    sim[j] = x.w;
+    sbc[j] = rec_own >= 0;
+    if (ACC) spp[j] = p_own.w;
   }
   double q4[4];
   if (PROJ) {
@@ -355,55 +432,72 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
   // 2. one element per thread
   if (has_e) {
     const int id[4] = {c.x, c.y, c.z, c.w};
-    double X[4][3];
+    double X[4][3], up[4][3], uq[4][3];
+    T p[4];
 #pragma unroll
     for (int n = 0; n < 4; n++) {
-      X[n][0] = sx[id[n]];
-      X[n][1] = sy[id[n]];
-      X[n][2] = sz[id[n]];
-    }
-    double DNd[3][4], Vd, hd;
-    tet_geometry(X, DNd, Vd, hd);
-    T DN[3][4];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int n = 0; n < 4; n++) DN[i][n] = (T)DNd[i][n];
-    if (PROJ) {
-      double up[4][3];
-#pragma unroll
-      for (int n = 0; n < 4; n++) {
-        up[n][0] = spu[id[n]];
-        up[n][1] = spv[id[n]];
-        up[n][2] = spw[id[n]];
+      const double2 r0 = snd[id[n]][0], r1 = snd[id[n]][1];
+      X[n][0] = r0.x;
+      X[n][1] = r0.y;
+      X[n][2] = r1.x;
+      if (PROJ || ACC) {
+        const double2 r2 = snd[id[n]][2];
+        up[n][0] = r1.y;
+        up[n][1] = r2.x;
+        up[n][2] = r2.y;
       }
-      vms_projection<T>(DN, hd, up, q4, sdu, a.k);
+    }
+    T DN[3][4], V, h;
+    tet_geometry_t<T>(X, DN, V, h);
+    ElemVel<T> ev;
+    if (PROJ) {
+      elem_vel<T>(DN, up, ev);
+      vms_projection<T>(DN, h, ev, q4, sdu, a.k);
 #pragma unroll
       for (int g = 0; g < 4; g++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) a.sdus[(long)(g * 3 + i) * E + e] = sdu[g][i];
+        for (int i = 0; i < 3; i++) *row_ptr(a.sdus, (g * 3 + i) * E, eoff) = sdu[g][i];
     }
     if (RES) {
-      double uq[4][4], upv[4][3];
 #pragma unroll
       for (int n = 0; n < 4; n++) {
-        uq[n][0] = squ[id[n]];
-        uq[n][1] = sqv[id[n]];
-        uq[n][2] = sqw[id[n]];
-        uq[n][3] = sqp[id[n]];
-        if (ACC) {
-          upv[n][0] = spu[id[n]];
-          upv[n][1] = spv[id[n]];
-          upv[n][2] = spw[id[n]];
+        const double2 r3 = snd[id[n]][3], r4 = snd[id[n]][4];
+        uq[n][0] = r3.x;
+        uq[n][1] = r3.y;
+        uq[n][2] = r4.x;
+        p[n] = (T)r4.y;
+      }
+      // without BC nodes the post-BC velocities equal the pre-BC ones the
+      // projection just used: reuse their gradient
+      const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
+      if (!reuse) elem_vel<T>(DN, uq, ev);
+      T res[4][4];
+      vms_residual<T, ACC>(DN, V, ev, p, uq, up, sdu, res, a.k);
+      if (a.diag_res) {
+        const size_t eo = (size_t)a.perm[e];
+        double *d = a.diag_res + eo * 16;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) d[b * 4 + i] = (double)res[b][i];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          double sm = 0.0;
+#pragma unroll
+          for (int g = 0; g < 4; g++) sm += (double)sdu[g][i] * 0.25;
+          a.diag_sdu[eo * 3 + i] = sm;
         }
       }
-      T res[4][4];
-      vms_residual<T, ACC>(DN, Vd, uq, upv, sdu, res, a.k);
       const int ps[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
-      for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) sres[i * (4 * VB) + ps[b]] = res[b][i];
+      for (int b = 0; b < 4; b++) {
+        T4 v;
+        v.x = res[b][0];
+        v.y = res[b][1];
+        v.z = res[b][2];
+        v.w = res[b][3];
+        sres[swz<T4>(ps[b])] = v;
+      }
     }
   }
   if (!RES) return;
@@ -416,15 +510,17 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
     const int s0 = s0_own - e0 * 4, s1 = s1_own - e0 * 4;
 #pragma unroll 4
     for (int s = s0; s < s1; s++) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) R[i] += (double)sres[i * (4 * VB) + s];
+      const T4 v = sres[swz<T4>(s)];
+      R[0] += (double)v.x;
+      R[1] += (double)v.y;
+      R[2] += (double)v.z;
+      R[3] += (double)v.w;
     }
     if (ps_own < 0) {
       // base of update_dual_solution: post-BC at the first dual iteration, pre-BC after
-      const double4 base = !ACC ? make_double4(squ[j], sqv[j], sqw[j], sqp[j])
-                                : make_double4(spu[j], spv[j], spw[j], spp[j]);
+      const double4 base = !ACC ? q_own : p_own;
       double4 pre, post;
-      m2 = node_update(base, sm[j], R, a.k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, sim[j], R, a.k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -464,7 +560,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     }
     const double4 base = a.first_dual ? a.Q[g] : a.P[g];
     double4 pre, post;
-    m2 = node_update(base, a.xyzm[g].w, R, a.k, pre, post, a.node_rec[g], a.rec, a.row_off, a.rows, a.err);
+    m2 = node_update(base, a.xyzm[g].w, R, a.k, pre, post, a.node_rec[g], a.rec, a.row_off, a.rows, a.err);  // w = 1/m
     a.Pn[g] = pre;
     a.Qn[g] = post;
   }
@@ -474,7 +570,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
 // Nodal lumped mass nd_mass[n] = sum over incident elements of V/4 in
 // ascending original element order (:88-90), into xyzm.w.
 __global__ __launch_bounds__(256) void vms_mass(int N, const int *g_off, const int *g_slot, const int4 *conn,
-                                                const double4 *xyz, double4 *xyzm) {
+                                                const double4 *xyz, double4 *xyzm, double *mass) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   double acc = 0.0;
@@ -493,7 +589,8 @@ __global__ __launch_bounds__(256) void vms_mass(int N, const int *g_off, const i
     acc += V / 4.0;
   }
   const double4 x = xyz[n];
-  xyzm[n] = make_double4(x.x, x.y, x.z, acc);
+  xyzm[n] = make_double4(x.x, x.y, x.z, 1.0 / acc);
+  mass[n] = acc;
 }
 
 // BCs of the initial state (solve() :883-889).
@@ -506,6 +603,94 @@ __global__ void vms_bc_only(int nrec, const int *rec_node, const BCRecord *rec, 
   double u = q.x, v = q.y, w = q.z, p = q.w;
   apply_bc(rec[i], row_off, rows, t, u, v, w, &p, err);
   Q[n] = make_double4(u, v, w, p);
+}
+
+// Element diagnostics.  The reference scatters each element's residual into
+// the global rhs and reads the element's centroid residual right after its
+// own scatter (:1065-1090, :699-711), so node n holds, at that moment, the
+// sum of the contributions of the elements <= e that touch it.  Per node, walk
+// its elements in ascending original order and replace every contribution by
+// that running sum (slot e*4+a belongs to exactly one node, so in place).
+__global__ __launch_bounds__(256) void vms_diag_prefix(int N, const int *g_off, const int *g_slot, double4 *contrib) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+  for (int s = g_off[n]; s < g_off[n + 1]; s++) {
+    const int slot = g_slot[s];
+    const double4 v = contrib[slot];
+    r0 += v.x;
+    r1 += v.y;
+    r2 += v.z;
+    r3 += v.w;
+    contrib[slot] = make_double4(r0, r1, r2, r3);
+  }
+}
+
+// Centroid residual of every element (eval_centroid_residuals :699-711) and
+// per-block maxima of |res_u| and |res_p| (eval_residual_norms :774-793).
+__global__ __launch_bounds__(256) void vms_diag_centroid(int E, const double4 *prefix, double *el_u, double *el_p,
+                                                         double *nmax) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double un = 0.0, pn = 0.0;
+  if (e < E) {
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0, p = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const double4 v = prefix[4L * e + a];
+      u0 += v.x * 0.25;
+      u1 += v.y * 0.25;
+      u2 += v.z * 0.25;
+      p += v.w * 0.25;
+    }
+    el_u[3L * e] = u0;
+    el_u[3L * e + 1] = u1;
+    el_u[3L * e + 2] = u2;
+    el_p[e] = p;
+    un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    pn = fabs(p);
+  }
+  __shared__ double wm[2][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a = __shfl_xor(un, off), b = __shfl_xor(pn, off);
+    un = a > un ? a : un;
+    pn = b > pn ? b : pn;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wm[0][threadIdx.x >> 6] = un;
+    wm[1][threadIdx.x >> 6] = pn;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double m = wm[threadIdx.x][0];
+    for (int i = 1; i < 4; i++) m = wm[threadIdx.x][i] > m ? wm[threadIdx.x][i] : m;
+    nmax[2L * blockIdx.x + threadIdx.x] = m;
+  }
+}
+
+// One workgroup: out[0..2] = max |res_u|, max |res_p|, max pre-BC |u|^2.
+__global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nmax, int n_bmax, const double *bmax,
+                                                       double *out) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    v[0] = nmax[2L * i] > v[0] ? nmax[2L * i] : v[0];
+    v[1] = nmax[2L * i + 1] > v[1] ? nmax[2L * i + 1] : v[1];
+  }
+  for (int i = threadIdx.x; i < n_bmax; i += blockDim.x) v[2] = bmax[i] > v[2] ? bmax[i] : v[2];
+  __shared__ double wm[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v[k], off);
+      v[k] = o > v[k] ? o : v[k];
+    }
+    if ((threadIdx.x & 63) == 0) wm[k][threadIdx.x >> 6] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double m = wm[threadIdx.x][0];
+    for (int i = 1; i < 4; i++) m = wm[threadIdx.x][i] > m ? wm[threadIdx.x][i] : m;
+    out[threadIdx.x] = m;
+  }
 }
 
 // reference layout sdus[(g*3+i)*E + original e]  <->  block order
@@ -534,7 +719,8 @@ struct femx_vms {
   femx_vms_params p{};
   hipStream_t st = nullptr;
   BCDev bc;
-  DevBuf<double4> xyzm;
+  DevBuf<double4> xyzm;   // x y z and 1 / nd_mass
+  DevBuf<double> mass;    // nd_mass
   DevBuf<double4> P[2], Q[2];
   DevBuf<double4> partials;
   DevBuf<double> sdus64;
@@ -548,6 +734,13 @@ struct femx_vms {
   long iter = 0;
   double time = 0.0;
   double min_h = 0.0;
+  // element diagnostics of every dual iteration (femx_vms_set_diagnostics)
+  bool diag = false;
+  DevBuf<double> diag_res;   // [E][4][4] contributions, then their running sums
+  DevBuf<double> diag_sdu;   // [E][3]
+  DevBuf<double> diag_el;    // [E][3] centroid residual u v w, then [E] p
+  DevBuf<double> diag_nmax;  // [blocks][2]
+  DevBuf<double> diag_iter;  // [tot_dual_steps + 1][3] of the last step
   KernelTimer timer;
 };
 
@@ -607,6 +800,9 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.rows = s->bc.rows.p;
   a.err = s->bc.err.p;
   a.bmax = s->bmax.p;
+  a.perm = b.d_perm.p;
+  a.diag_res = s->diag ? s->diag_res.p : nullptr;
+  a.diag_sdu = s->diag ? s->diag_sdu.p : nullptr;
   a.k = vms_const(s, t);
   return a;
 }
@@ -661,6 +857,20 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   return FEMX_OK;
 }
 
+// Diagnostics of dual iteration d from the contributions the block kernel
+// just wrote: running sums per node, centroid residuals, maxima.
+femx_status launch_diag(femx_vms *s, int d) {
+  Mesh &m = *s->mesh;
+  TimedLaunch tl(s->timer, s->timer.kind("vms_diag", m.E * (128.0 * 3 + 32.0 + 8.0)), s->st);
+  double4 *c = reinterpret_cast<double4 *>(s->diag_res.p);
+  vms_diag_prefix<<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, c);
+  const int nb = grid_for(m.E, 256);
+  vms_diag_centroid<<<nb, 256, 0, s->st>>>(m.E, c, s->diag_el.p, s->diag_el.p + 3L * m.E, s->diag_nmax.p);
+  vms_diag_reduce<<<1, 256, 0, s->st>>>(nb, s->diag_nmax.p, s->n_bmax, s->bmax.p, s->diag_iter.p + 3L * d);
+  FEMX_HIP(hipGetLastError());
+  return FEMX_OK;
+}
+
 femx_status flush_projection(femx_vms *s) {
   if (!s->proj_pending) return FEMX_OK;
   femx_status r = launch_block(s, true, false, false, true, s->time);
@@ -685,7 +895,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   Mesh &m = mh->m;
   if (m.npe != 4) return set_error("the VMS solver runs on tet4 meshes only"), FEMX_ERR_UNSUPPORTED;
   if (m.N < 4) return set_error("the VMS projection reads global nodes 0..3: need >= 4 nodes"), FEMX_ERR_ARG;
-  if (m.blk.B > VB || m.blk.max_local > VB) return set_error("element blocks do not fit the VMS workgroup"), FEMX_ERR_ARG;
+  if (m.blk.B > VB || m.blk.max_local > VN) return set_error("element blocks do not fit the VMS workgroup"), FEMX_ERR_ARG;
   if (p->precision != FEMX_FP64 && p->precision != FEMX_MIXED) return set_error("precision"), FEMX_ERR_ARG;
   if (!(p->dt > 0.0) || p->rho == 0.0 || p->tot_dual_steps < 0) return set_error("parameters"), FEMX_ERR_ARG;
   femx_vms *s = new femx_vms();
@@ -699,7 +909,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(FEMX_ERR_HIP);
   if ((r = s->bc.build(bc, m.N, true, s->st)) != FEMX_OK) return fail(r);
   if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
-  if ((r = s->xyzm.alloc(m.N)) != FEMX_OK) return fail(r);
+  if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->mass.alloc(m.N)) != FEMX_OK) return fail(r);
   for (int k = 0; k < 2; k++)
     if ((r = s->P[k].alloc(m.N)) != FEMX_OK || (r = s->Q[k].alloc(m.N)) != FEMX_OK) return fail(r);
   const int nparts = m.blk.n_sh ? m.blk.sh_off[m.blk.n_sh] : 0;
@@ -714,7 +924,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
     if (hipMemsetAsync(s->Q[k].p, 0, m.N * sizeof(double4), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   }
   vms_mass<<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, reinterpret_cast<const int4 *>(m.conn.p),
-                                                  m.xyz.p, s->xyzm.p);
+                                                  m.xyz.p, s->xyzm.p, s->mass.p);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   // min element length (:947-952), for the CFL print
   double mh_ = 1.79769313486231570815e+308;
@@ -789,6 +999,7 @@ extern "C" femx_status femx_vms_step(femx_vms *s, int nsteps) {
       femx_status r = launch_block(s, s->proj_pending, true, d > 0, d == 0, s->time);
       if (r != FEMX_OK) return r;
       if ((r = launch_fixup(s, d == 0, s->time)) != FEMX_OK) return r;
+      if (s->diag && (r = launch_diag(s, d)) != FEMX_OK) return r;
       s->cur ^= 1;
       s->proj_pending = true;
       s->iter++;
@@ -819,11 +1030,8 @@ extern "C" femx_status femx_vms_get_state(femx_vms *s, double *sol, double *sdus
 
 extern "C" femx_status femx_vms_get_mass(femx_vms *s, double *nd_mass) {
   if (!s || !nd_mass) return set_error("null argument"), FEMX_ERR_ARG;
-  const int N = s->mesh->N;
-  std::vector<double4> x(N);
-  FEMX_HIP(hipMemcpyAsync(x.data(), s->xyzm.p, N * sizeof(double4), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipMemcpyAsync(nd_mass, s->mass.p, s->mesh->N * sizeof(double), hipMemcpyDeviceToHost, s->st));
   FEMX_HIP(hipStreamSynchronize(s->st));
-  for (int n = 0; n < N; n++) nd_mass[n] = x[n].w;
   return FEMX_OK;
 }
 
@@ -838,6 +1046,46 @@ extern "C" femx_status femx_vms_get_cfl(femx_vms *s, double *max_u, double *cfl)
   const double mu = std::sqrt(m2);
   if (max_u) *max_u = mu;
   if (cfl) *cfl = mu * s->p.dt / s->min_h;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_set_diagnostics(femx_vms *s, int enable) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (enable && !s->diag_res.p) {
+    const long E = s->mesh->E;
+    femx_status r;
+    if ((r = s->diag_res.alloc(16 * E)) != FEMX_OK || (r = s->diag_sdu.alloc(3 * E)) != FEMX_OK ||
+        (r = s->diag_el.alloc(4 * E)) != FEMX_OK || (r = s->diag_nmax.alloc(2L * grid_for(E, 256))) != FEMX_OK ||
+        (r = s->diag_iter.alloc(3L * (s->p.tot_dual_steps + 1))) != FEMX_OK)
+      return r;
+    FEMX_HIP(hipMemsetAsync(s->diag_sdu.p, 0, 3 * E * sizeof(double), s->st));
+    FEMX_HIP(hipMemsetAsync(s->diag_el.p, 0, 4 * E * sizeof(double), s->st));
+    FEMX_HIP(hipMemsetAsync(s->diag_iter.p, 0, 3L * (s->p.tot_dual_steps + 1) * sizeof(double), s->st));
+  }
+  s->diag = enable != 0;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_get_diagnostics(femx_vms *s, double *iter_norms, double *el_u_res, double *el_p_res,
+                                                double *el_u_sdus, double *el_p_sdus) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (!s->diag_iter.p) return set_error("diagnostics were never enabled"), FEMX_ERR_ARG;
+  const long E = s->mesh->E;
+  const int nd = s->p.tot_dual_steps + 1;
+  std::vector<double> it(3L * nd);
+  FEMX_HIP(hipMemcpyAsync(it.data(), s->diag_iter.p, it.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (el_u_res) FEMX_HIP(hipMemcpyAsync(el_u_res, s->diag_el.p, 3 * E * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (el_p_res)
+    FEMX_HIP(hipMemcpyAsync(el_p_res, s->diag_el.p + 3 * E, E * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (el_u_sdus) FEMX_HIP(hipMemcpyAsync(el_u_sdus, s->diag_sdu.p, 3 * E * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  if (el_p_sdus) std::fill(el_p_sdus, el_p_sdus + E, 0.0);  // include_sg_p = false (:846): sgps stay 0
+  if (iter_norms)
+    for (int d = 0; d < nd; d++) {
+      iter_norms[3 * d] = it[3 * d];
+      iter_norms[3 * d + 1] = it[3 * d + 1];
+      iter_norms[3 * d + 2] = std::sqrt(it[3 * d + 2]) * s->p.dt / s->min_h;  // CFL (:1202-1209)
+    }
   return FEMX_OK;
 }
 

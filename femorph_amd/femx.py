@@ -14,7 +14,8 @@ import numpy as np
 from .mesh import BoundaryConditions, Mesh
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfemx.so")
+# FEMX_LIB selects another build of the same library (A/B comparisons).
+LIB_PATH = os.environ.get("FEMX_LIB", os.path.join(_HERE, "libfemx.so"))
 
 FEMX_FP64, FEMX_MIXED = 0, 1
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_BC", 4: "ERR_NOMEM", 5: "ERR_UNSUPPORTED",
@@ -67,6 +68,8 @@ SIGNATURES = [
     ("femx_vms_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     ("femx_vms_get_mass", C.c_int, [C.c_void_p, _dp]),
     ("femx_vms_get_cfl", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("femx_vms_set_diagnostics", C.c_int, [C.c_void_p, C.c_int]),
+    ("femx_vms_get_diagnostics", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     ("femx_vms_sync", C.c_int, [C.c_void_p]),
     ("femx_vms_stream", C.c_void_p, [C.c_void_p]),
     ("femx_supg_create", C.c_int, [C.c_void_p, C.POINTER(BCDesc), C.POINTER(SUPGParams), C.POINTER(C.c_void_p)]),
@@ -176,6 +179,7 @@ class VMSSolver:
         L = load()
         self.dmesh = dmesh
         self.N, self.E = dmesh.mesh.n_nodes, dmesh.mesh.n_elems
+        self.tot_dual_steps = tot_dual_steps
         k = _BCKeep(bc)
         p = VMSParams(rho, mu, inv_eps, c1, c2, dt, dt if dual_step is None else dual_step, tot_dual_steps, precision)
         h = C.c_void_p()
@@ -213,6 +217,20 @@ class VMSSolver:
         a, b = np.zeros(1), np.zeros(1)
         _check(load().femx_vms_get_cfl(self._h, _p(a), _p(b)))
         return float(a[0]), float(b[0])
+
+    def set_diagnostics(self, enable=True):
+        _check(load().femx_vms_set_diagnostics(self._h, int(enable)))
+
+    def get_diagnostics(self):
+        """Element diagnostics of the last step: dict with iter_norms
+        [(tot_dual_steps+1), 3] (max |res_u|, max |res_p|, CFL per dual
+        iteration) and res_v [E,3], res_p [E], subgrid_u [E,3], subgrid_p [E]
+        of the last dual iteration."""
+        E = self.E
+        it = np.zeros((self.tot_dual_steps + 1, 3))
+        ru, rp, su, sp = np.zeros((E, 3)), np.zeros(E), np.zeros((E, 3)), np.zeros(E)
+        _check(load().femx_vms_get_diagnostics(self._h, _p(it), _p(ru), _p(rp), _p(su), _p(sp)))
+        return dict(iter_norms=it, res_v=ru, res_p=rp, subgrid_u=su, subgrid_p=sp)
 
     def sync(self):
         _check(load().femx_vms_sync(self._h))

@@ -121,6 +121,22 @@ femx_status femx_vms_get_mass(femx_vms *s, double *nd_mass);
 /* max over nodes of |u| of the pre-BC state of the last dual iteration and
  * the CFL number printed by the reference (:1202-1215). */
 femx_status femx_vms_get_cfl(femx_vms *s, double *max_u, double *cfl);
+/* Element diagnostics of every dual iteration (off by default: they cost
+ * an extra pass over E x 16 doubles per iteration and 200 B per element of
+ * device memory).  They replace eval_centroid_residuals /
+ * eval_centroid_subgrid / eval_residual_norms
+ * (src/femVMSExplicitFluidSolver.cpp:699-724,774-793,1087-1093,1171-1173)
+ * with the reference's semantics: the centroid residual of element e
+ * averages the global residual as assembled up to element e (ascending
+ * element order). */
+femx_status femx_vms_set_diagnostics(femx_vms *s, int enable);
+/* Diagnostics of the last step: iter_norms [(tot_dual_steps+1)][3] = max
+ * centroid |res_u|, max centroid |res_p|, CFL of each dual iteration (the
+ * values the reference prints at :1212); el_u_res [E][3], el_p_res [E],
+ * el_u_sdus [E][3], el_p_sdus [E] of the last dual iteration (the res_v,
+ * res_p, subgrid_u, subgrid_p results, :1269-1325).  Any pointer may be NULL. */
+femx_status femx_vms_get_diagnostics(femx_vms *s, double *iter_norms, double *el_u_res, double *el_p_res,
+                                     double *el_u_sdus, double *el_p_sdus);
 femx_status femx_vms_sync(femx_vms *s);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 void *femx_vms_stream(femx_vms *s);

@@ -140,3 +140,35 @@ def test_mixed_precision_error(gpu):
     ref = o.get("sol_n")
     err_u = rel_l2(sol[:, :3], ref[:, :3])
     assert err_u <= 1e-5, err_u
+
+
+@pytest.mark.parametrize("tot_dual", [0, 2])
+def test_element_diagnostics(gpu, tot_dual):
+    """res_v / res_p (centroid residuals of the partially assembled rhs,
+    ascending element order), subgrid_u / subgrid_p and the printed norms
+    and CFL of the last dual iteration vs the oracle, which assembles
+    element by element like the reference.  Tolerance 1e-11: the running
+    sums cancel, so their rounding is relative to the contributions."""
+    m, bc, params, ic = _case("channel", 5, 0.15, seed=5)
+    sdus = np.random.default_rng(3).uniform(-1e-3, 1e-3, (12, m.n_elems))
+    o, g, _ = _pair(gpu, m, bc, params, ic, tot_dual_steps=tot_dual, sdus=sdus)
+    g.set_diagnostics(True)
+    for step in range(4):
+        o.step()
+        g.step()
+        d = g.get_diagnostics()
+        errs = dict(res_v=rel_l2(d["res_v"], o.get("el_u_res")), res_p=rel_l2(d["res_p"], o.get("el_p_res")),
+                    subgrid_u=rel_l2(d["subgrid_u"], o.get("el_u_sdus")))
+        print(step, errs, flush=True)
+        assert max(errs.values()) <= 1e-11, errs
+        assert not d["subgrid_p"].any()
+        s = o.get("scalars")
+        last = d["iter_norms"][-1]
+        assert d["iter_norms"].shape == (tot_dual + 1, 3)
+        np.testing.assert_allclose(last, s[2:5], rtol=1e-11, atol=0)
+    # switching the diagnostics off leaves the state path untouched
+    g.set_diagnostics(False)
+    o.step()
+    g.step()
+    sol, _, _ = g.get_state()
+    assert rel_l2(sol, o.get("sol_n")) <= TOL64

@@ -118,6 +118,199 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def _c4_items(seed, precision, count=256):
+    """256 lid-driven VMS tet4 cavities, n log-uniform in [8, 48], Re uniform
+    in [50, 500], IC uniform +-1e-3 (the C1 physics per mesh)."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(8), np.log(48), count))).astype(int)
+    res = rng.uniform(50.0, 500.0, count)
+    cache, items, ics = {}, [], []
+    for i, (n, re) in enumerate(zip(ns, res)):
+        if n not in cache:
+            m = meshgen.tet_cube(int(n))
+            cache[n] = (m, meshgen.lid_cavity_bc(m), meshgen.min_tet_h(m))
+        m, bc, h = cache[n]
+        items.append((m, bc, dict(rho=1.0, mu=1.0 / re, inv_eps=1.0, c1=4.0, c2=2.0, dt=0.1 * h, precision=precision)))
+        ics.append(meshgen.random_velocity(m, 1e-3, seed + 1000 + i))
+    return ns, items, ics
+
+
+def workload_c4(args):
+    """Batch of 256 cavities: one batched launch per phase, graph replay."""
+    def note(msg):
+        print(f"c4: {msg}", file=sys.stderr, flush=True)
+
+    ns, items, ics = _c4_items(args.seed, femx.FEMX_MIXED)
+    note(f"{len(items)} meshes generated")
+    E = np.array([m.n_elems for m, _, _ in items], np.float64)
+    N = np.array([m.n_nodes for m, _, _ in items], np.float64)
+    steps = args.steps
+    order = np.argsort(ns, kind="stable")
+    sample = [items[j] for j in order[::16]]
+    t0 = time.perf_counter()
+    be, xcd, tune_ms = femx.vms_batch_autotune(sample, steps=6)
+    tune_s = time.perf_counter() - t0
+    note(f"autotune: block_elems {be}, xcd_order {xcd}, {tune_s:.1f} s")
+    t0 = time.perf_counter()
+    b = femx.VMSBatch(items, block_elems=be, xcd_order=xcd)
+    build_s = time.perf_counter() - t0
+    b.set_state(ics)
+    b.step(2)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps, graph=True)
+    b.sync()
+    secs = time.perf_counter() - t0
+    note(f"batch: {steps} steps in {secs:.3f} s")
+    info = b.info()
+    mixed_states = [b.get_state(i)[0] for i in range(len(items))]
+    del b
+    # the same batch in FP64: velocity error of the mixed path after 2 + steps steps
+    items64 = [(m, bc, dict(p, precision=femx.FEMX_FP64)) for m, bc, p in items]
+    b64 = femx.VMSBatch(items64, block_elems=be, xcd_order=xcd)
+    b64.set_state(ics)
+    b64.step(2 + steps)
+    errs = []
+    for i in range(len(items)):
+        ref = b64.get_state(i)[0]
+        errs.append(float(np.linalg.norm(mixed_states[i][:, :3] - ref[:, :3]) / np.linalg.norm(ref[:, :3])))
+    del b64
+    note("fp64 reference batch done")
+    # plain single-mesh path on the same meshes: one handle per mesh, run one after the other
+    solvers = []
+    for (m, bc, p), ic in zip(items, ics):
+        dm = femx.DeviceMesh(m)
+        g = femx.VMSSolver(dm, bc, p["rho"], p["mu"], p["inv_eps"], p["c1"], p["c2"], p["dt"], precision=p["precision"])
+        g.set_state(ic)
+        g.step(2)
+        g.sync()
+        solvers.append((dm, g))
+    t0 = time.perf_counter()
+    for _, g in solvers:
+        g.step(steps)
+        g.sync()
+    single_secs = time.perf_counter() - t0
+    note(f"single-mesh path: {single_secs:.3f} s")
+    del solvers
+    total = float(E.sum()) * steps
+    value = total / secs
+    step_bytes = float(np.sum((112.0 + 84.0 * N / E) * E))
+    achieved = step_bytes / (secs / steps) / 1e9
+    out = {
+        "metric": "element-steps/sec (VMS tet4, batch of 256 cavities)", "value": round(value, 1),
+        "unit": "element-steps/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": round(secs / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-element/f64-state", "data": "synthetic (seeded cavities)",
+        "config": {"workload": f"C4: 256 VMS tet4 lid cavities, n log-uniform [8,48], Re U[50,500] "
+                               f"({int(E.sum())} tets, {int(N.sum())} nodes)",
+                   "elements": int(E.sum()), "nodes": int(N.sum()), "meshes": len(items),
+                   "n_min": int(ns.min()), "n_max": int(ns.max())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched step (block kernel + fix-up + time tick)",
+                     "step_bytes": step_bytes},
+        "batch": dict(info, graph=True, autotune={"block_elems": be, "xcd_order": xcd,
+                                                  "ms_per_step": [round(x, 4) for x in tune_ms],
+                                                  "candidates": "256/224/192 x xcd 1/0", "sample_meshes": len(sample),
+                                                  "seconds": round(tune_s, 2)},
+                      build_seconds=round(build_s, 2)),
+        "single_mesh_path": {"value": round(total / single_secs, 1), "ms_per_step": round(single_secs / steps * 1e3, 4),
+                             "batch_speedup": round(single_secs / secs, 2)},
+        "error_vs_fp64": {"steps": 2 + steps, "max_rel_l2_velocity": max(errs),
+                          "median_rel_l2_velocity": float(np.median(errs))},
+    }
+    if not args.no_cpu:
+        from oracle.oracle import VMSOracle  # test infrastructure, checker only
+        j = int(order[len(order) // 2])
+        m, bc, p = items[j]
+        o = VMSOracle(m, bc, p["rho"], p["mu"], p["inv_eps"], p["c1"], p["c2"], p["dt"])
+        o.set_ic(ics[j])
+        k, tt = 0, 0.0
+        while tt < args.cpu_budget and k < 200:
+            t0 = time.perf_counter()
+            o.step()
+            tt += time.perf_counter() - t0
+            k += 1
+        cv = m.n_elems * k / tt
+        out["cpu_baseline"] = {"value": round(cv, 1), "unit": "element-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} FP64 steps of the median mesh ({m.n_elems} tets), oracle, 1 thread"}
+        out["speedup_vs_cpu_1t"] = round(value / cv, 1)
+    print(json.dumps(out), flush=True)
+
+
+def workload_serve(args):
+    """Poisson arrivals of tet4 (VMS) / hexa8 (SUPG) cavity solves, n
+    log-uniform in [8, 64], 50..500 steps, mixed precision; per rate the
+    latency percentiles and throughput."""
+    rng = np.random.default_rng(args.seed)
+    cache = {}
+
+    def mesh_for(kind, n):
+        if (kind, n) not in cache:
+            if kind == "vms":
+                m = meshgen.tet_cube(n, 0.15, n)
+                cache[(kind, n)] = (m, meshgen.lid_cavity_bc(m), 0.1 * meshgen.min_tet_h(m))
+            else:
+                m = meshgen.hex_cube(n, 0.2, n)
+                cache[(kind, n)] = (m, meshgen.lid_cavity_bc(m), 0.1 / n)
+        return cache[(kind, n)]
+
+    rates = [float(r) for r in args.rates.split(",")]
+    plans = []
+    for rate in rates:
+        arr = np.cumsum(rng.exponential(1.0 / rate, args.requests))
+        reqs = []
+        for _ in range(args.requests):
+            kind = "vms" if rng.random() < 0.5 else "supg"
+            n = int(np.rint(np.exp(rng.uniform(np.log(8), np.log(64)))))
+            steps = int(rng.integers(50, 501))
+            re = float(rng.uniform(50.0, 500.0))
+            mesh_for(kind, n)
+            reqs.append((kind, n, steps, re))
+        plans.append((rate, arr, reqs))
+    table = {}
+    for rate, arr, reqs in plans:
+        srv = femx.Server(budget_bytes=args.budget_gb << 30, max_active=args.max_active, prep_threads=args.prep_threads,
+                          chunk_steps=args.chunk)
+        ids = {}
+        t0 = time.perf_counter()
+        for t_arr, (kind, n, steps, re) in zip(arr, reqs):
+            wait = t0 + t_arr - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            m, bc, dt = mesh_for(kind, n)
+            if kind == "vms":
+                rid = srv.submit_vms(m, bc, steps, 1.0, 1.0 / re, 1.0, 4.0, 2.0, dt)
+            else:
+                rid = srv.submit_supg(m, bc, steps, 1.0, 1.0 / re, 1.0, 4.0, dt)
+            ids[rid] = m.n_elems * steps
+        srv.drain(1200.0)
+        st = srv.poll()
+        for s_ in st:
+            srv.result(s_["id"])
+        srv.close()
+        lat = np.array([s_["t_done"] - s_["t_submit"] for s_ in st])
+        span = max(s_["t_done"] for s_ in st) - min(s_["t_submit"] for s_ in st)
+        done = [s_ for s_ in st if s_["status"] in (0, femx.EXIT_CONDITION)]
+        es = sum(s_["elements"] * s_["steps_done"] for s_ in done)
+        table[str(int(rate))] = {"requests": len(st), "ok": len(done), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                                 "req_per_s": round(len(st) / span, 2), "element_steps_per_s": round(es / span, 1),
+                                 "offered_req_per_s": round(len(st) / float(arr[-1]), 2)}
+        print(json.dumps({"rate": rate, **table[str(int(rate))]}), file=sys.stderr, flush=True)
+    top = table[str(int(rates[-1]))]
+    out = {"metric": "p50/p99 solve latency (served VMS tet4 / SUPG hexa8 requests)",
+           "value": top["element_steps_per_s"], "unit": "element-steps/s", "n_gpus": 1, "steps": None, "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32-element/f64-state", "data": "synthetic (seeded jittered cavities)",
+           "config": {"workload": f"C5: Poisson arrivals at {args.rates} req/s, {args.requests} requests per rate, "
+                                  "tet4 (VMS) or hexa8 (SUPG) cavities, n log-uniform [8,64], steps U[50,500]",
+                      "budget_gb": args.budget_gb, "max_active": args.max_active, "chunk_steps": args.chunk,
+                      "prep_threads": args.prep_threads},
+           "latency": table}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +322,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
+    ap.add_argument("--workload", choices=["c3", "c4", "serve"], default="c3",
+                    help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request stream")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000")
+    ap.add_argument("--requests", type=int, default=40, help="requests per rate (serve)")
+    ap.add_argument("--budget-gb", type=int, default=64)
+    ap.add_argument("--max-active", type=int, default=64)
+    ap.add_argument("--prep-threads", type=int, default=8)
+    ap.add_argument("--chunk", type=int, default=25)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -140,6 +342,14 @@ def main():
     if femx.device_count() < 1:
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
+    if args.workload == "c4":
+        if args.steps == 50:
+            args.steps = 200
+        workload_c4(args)
+        return
+    if args.workload == "serve":
+        workload_serve(args)
+        return
 
     m, bc, params, ic = make_workload(args.n)
     E, N = m.n_elems, m.n_nodes

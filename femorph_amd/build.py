@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemx.so")
 
-SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip"]
+SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -24,6 +24,7 @@ HIPCC_FLAGS = [
     # keep the compiler away from packed FP32 instructions (v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    "-pthread",  # the solve server's prep and GPU threads
 ]
 
 

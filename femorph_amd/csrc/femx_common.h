@@ -105,8 +105,15 @@ struct Mesh {
 };
 
 // Builds m.blk on the host (needs h_coords, h_conn, the global CSR) and uploads it.
-void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B);
-femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B);
+// elem_group (optional): group id per element; blocks never mix groups and
+// each group is Morton-ordered in its own bounding box (batches of meshes).
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr);
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr);
+// Fills m from host arrays (coordinates [N][3], 0-based connectivity
+// [E][npe]) and uploads it: coordinates, connectivity, node->element CSR,
+// element blocks of at most B elements (0: 256 for tet4, 128 for hexa8).
+femx_status mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
+                       const std::vector<int> *elem_group = nullptr);
 
 // --------------------------------------------------- boundary conditions
 // One record per BC-touched node, merging the reference's three lists with

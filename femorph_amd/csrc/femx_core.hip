@@ -95,34 +95,49 @@ static inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
   return v;
 }
 
-void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B) {
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
   Blocks &b = m.blk;
   const long E = m.E, npe = m.npe;
   b.B = B;
-  // Morton order of element centroids
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (long n = 0; n < m.N; n++)
-    for (int k = 0; k < 3; k++) {
-      lo[k] = std::min(lo[k], m.h_coords[3 * n + k]);
-      hi[k] = std::max(hi[k], m.h_coords[3 * n + k]);
-    }
-  std::vector<std::pair<uint64_t, int>> key(E);
+  // Morton order of element centroids, inside each element group (a batch
+  // of meshes: blocks never mix meshes, each mesh gets its own bounding box)
+  const int ng = elem_group ? 1 + *std::max_element(elem_group->begin(), elem_group->end()) : 1;
+  std::vector<double> lo(3 * ng, 1e300), hi(3 * ng, -1e300);
   for (long e = 0; e < E; e++) {
+    const int gi = elem_group ? (*elem_group)[e] : 0;
+    for (int a = 0; a < npe; a++)
+      for (int k = 0; k < 3; k++) {
+        const double x = m.h_coords[3L * m.h_conn[e * npe + a] + k];
+        lo[3 * gi + k] = std::min(lo[3 * gi + k], x);
+        hi[3 * gi + k] = std::max(hi[3 * gi + k], x);
+      }
+  }
+  struct Key {
+    int group;
+    uint64_t code;
+    int e;
+    bool operator<(const Key &o) const {
+      return group != o.group ? group < o.group : (code != o.code ? code < o.code : e < o.e);
+    }
+  };
+  std::vector<Key> key(E);
+  for (long e = 0; e < E; e++) {
+    const int gi = elem_group ? (*elem_group)[e] : 0;
     double c[3] = {0, 0, 0};
     for (int a = 0; a < npe; a++)
       for (int k = 0; k < 3; k++) c[k] += m.h_coords[3L * m.h_conn[e * npe + a] + k];
     uint64_t code = 0;
     for (int k = 0; k < 3; k++) {
-      double span = hi[k] - lo[k];
-      double f = span > 0 ? (c[k] / npe - lo[k]) / span : 0.0;
+      double span = hi[3 * gi + k] - lo[3 * gi + k];
+      double f = span > 0 ? (c[k] / npe - lo[3 * gi + k]) / span : 0.0;
       uint64_t q = (uint64_t)std::min(std::max(f, 0.0) * 2097151.0, 2097151.0);
       code |= spread3(q) << k;
     }
-    key[e] = {code, (int)e};
+    key[e] = {gi, code, (int)e};
   }
   std::sort(key.begin(), key.end());
   b.perm.resize(E);
-  for (long i = 0; i < E; i++) b.perm[i] = key[i].second;
+  for (long i = 0; i < E; i++) b.perm[i] = key[i].e;
   key.clear();
   key.shrink_to_fit();
   // greedy blocks: <= B elements, <= 256 local nodes
@@ -153,7 +168,8 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B) {
     const int eo = b.perm[i];
     int fresh = 0;
     for (int a = 0; a < npe; a++) fresh += stamp[m.h_conn[(long)eo * npe + a]] != blk;
-    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256)) close_block(i);
+    const bool new_group = elem_group && (*elem_group)[eo] != (*elem_group)[b.perm[e_begin]];
+    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256 || new_group)) close_block(i);
     for (int a = 0; a < npe; a++) {
       const int n = m.h_conn[(long)eo * npe + a];
       if (stamp[n] != blk) {
@@ -217,8 +233,8 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B) {
     }
 }
 
-femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B) {
-  build_blocks_host(m, g_off, B);
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
+  build_blocks_host(m, g_off, B, elem_group);
   Blocks &b = m.blk;
   const long E = m.E;
   const long L = b.total_local;
@@ -328,6 +344,36 @@ extern "C" int femx_device_count(void) {
   return n;
 }
 
+femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
+                             const std::vector<int> *elem_group) {
+  m.N = (int)N;
+  m.E = (int)E;
+  m.npe = npe;
+  (void)hipGetDevice(&m.device);
+  m.h_coords.assign(coords, coords + 3 * N);
+  m.h_conn.assign(conn, conn + npe * E);
+  // node -> (element, local node) gather lists in ascending element order
+  std::vector<int> off(N + 1, 0), slot(npe * E);
+  for (long i = 0; i < npe * E; i++) off[m.h_conn[i] + 1]++;
+  for (long n = 0; n < N; n++) {
+    m.max_deg = std::max(m.max_deg, off[n + 1]);
+    off[n + 1] += off[n];
+  }
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (long i = 0; i < npe * E; i++) slot[fill[m.h_conn[i]]++] = (int)i;  // i = e*npe + a, ascending
+  std::vector<double4> xyz(N);
+  for (long n = 0; n < N; n++) xyz[n] = make_double4(coords[3 * n], coords[3 * n + 1], coords[3 * n + 2], 0.0);
+  hipStream_t st = nullptr;
+  femx_status s;
+  if ((s = m.xyz.upload(xyz.data(), N, st)) != FEMX_OK || (s = m.conn.upload(m.h_conn.data(), npe * E, st)) != FEMX_OK ||
+      (s = m.g_off.upload(off.data(), N + 1, st)) != FEMX_OK || (s = m.g_slot.upload(slot.data(), npe * E, st)) != FEMX_OK ||
+      (s = build_blocks(m, off, B > 0 ? B : (npe == 4 ? 256 : 128), elem_group)) != FEMX_OK)
+    return s;
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "mesh upload");
+  return FEMX_OK;
+}
+
 extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out) {
   if (!d || !out || !d->coords || !d->conn) return set_error("null argument"), FEMX_ERR_ARG;
   if (d->n_nodes <= 0 || d->n_elems <= 0 || (d->npe != 4 && d->npe != 8))
@@ -339,36 +385,10 @@ extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out
   for (long i = 0; i < npe * E; i++)
     if (d->conn[i] < 0 || d->conn[i] >= N) return set_error("connectivity out of range"), FEMX_ERR_ARG;
   femx_mesh *mh = new femx_mesh();
-  Mesh &m = mh->m;
-  m.N = (int)N;
-  m.E = (int)E;
-  m.npe = (int)npe;
-  (void)hipGetDevice(&m.device);
-  m.h_coords.assign(d->coords, d->coords + 3 * N);
-  m.h_conn.assign(d->conn, d->conn + npe * E);
-  // node -> (element, local node) gather lists in ascending element order
-  std::vector<int> off(N + 1, 0), slot(npe * E);
-  for (long i = 0; i < npe * E; i++) off[m.h_conn[i] + 1]++;
-  for (long n = 0; n < N; n++) {
-    m.max_deg = std::max(m.max_deg, off[n + 1]);
-    off[n + 1] += off[n];
-  }
-  std::vector<int> fill(off.begin(), off.end() - 1);
-  for (long i = 0; i < npe * E; i++) slot[fill[m.h_conn[i]]++] = (int)i;  // i = e*npe + a, ascending
-  std::vector<double4> xyz(N);
-  for (long n = 0; n < N; n++) xyz[n] = make_double4(d->coords[3 * n], d->coords[3 * n + 1], d->coords[3 * n + 2], 0.0);
-  hipStream_t st = nullptr;
-  femx_status s;
-  if ((s = m.xyz.upload(xyz.data(), N, st)) != FEMX_OK || (s = m.conn.upload(m.h_conn.data(), npe * E, st)) != FEMX_OK ||
-      (s = m.g_off.upload(off.data(), N + 1, st)) != FEMX_OK || (s = m.g_slot.upload(slot.data(), npe * E, st)) != FEMX_OK ||
-      (s = build_blocks(m, off, npe == 4 ? 256 : 128)) != FEMX_OK) {
+  femx_status s = mesh_setup(mh->m, d->coords, d->conn, N, E, (int)npe, 0);
+  if (s != FEMX_OK) {
     delete mh;
     return s;
-  }
-  hipError_t e = hipStreamSynchronize(st);
-  if (e != hipSuccess) {
-    delete mh;
-    return hip_fail(e, "mesh upload");
   }
   *out = mh;
   return FEMX_OK;

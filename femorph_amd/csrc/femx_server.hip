@@ -1,0 +1,402 @@
+// femx_server.hip -- a solve server over the VMS and SUPG handles.
+//
+// The reference runs one solve per process (solveExplicitNS,
+// src/main.cpp:788-805).  The server keeps many solves resident on one GPU
+// and admits new ones while others run:
+//
+//   submit()      copies the request (mesh, BCs, parameters, step count)
+//                 into the pending queue and returns an id;
+//   prep threads  take pending requests in arrival order when their HBM
+//                 footprint fits the budget and a slot is free, then build
+//                 the element blocks, upload the mesh, create the solver
+//                 handle (its own HIP stream) and set the zero initial state;
+//   GPU thread    advances every active request by `chunk` steps at a time
+//                 on the request's own stream (so the work of different
+//                 requests overlaps on the device), polls completion with
+//                 an event per request, reads finished states back and frees
+//                 their handles and budget.
+//
+// Each request's arithmetic is that of its own handle, so a result does not
+// depend on what else is being served (bitwise).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "femx_common.h"
+
+using namespace femx;
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  long long id = 0;
+  femx_request_kind_t kind = FEMX_REQ_VMS;
+  int steps = 0;
+  // owned copies of the request arrays
+  int N = 0, E = 0, npe = 0;
+  std::vector<double> coords;
+  std::vector<int> conn;
+  std::vector<int> vel_node, vel_dir, vel_off, dir_node, pres_node;
+  std::vector<double> vel_rows, dir_val, pres_val;
+  femx_vms_params vms{};
+  femx_supg_params supg{};
+  size_t bytes = 0;
+  // handles while resident
+  femx_mesh *mesh = nullptr;
+  femx_vms *vh = nullptr;
+  femx_supg *sh = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  int done = 0, last_k = 0;
+  bool inflight = false;
+  femx_request_stats stats{};
+  std::vector<double> result;
+
+  femx_bc_desc bc() const {
+    femx_bc_desc b;
+    b.n_vel = (int)vel_node.size();
+    b.vel_node = vel_node.data();
+    b.vel_dir = vel_dir.data();
+    b.vel_row_off = vel_off.data();
+    b.vel_rows = vel_rows.data();
+    b.n_dir = (int)dir_node.size();
+    b.dir_node = dir_node.data();
+    b.dir_val = dir_val.data();
+    b.n_pres = (int)pres_node.size();
+    b.pres_node = pres_node.data();
+    b.pres_val = pres_val.data();
+    return b;
+  }
+  void release() {
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+    femx_vms_destroy(vh);
+    femx_supg_destroy(sh);
+    femx_mesh_free(mesh);
+    vh = nullptr;
+    sh = nullptr;
+    mesh = nullptr;
+  }
+};
+
+// Device bytes a request will hold (upper estimate of the mesh, block
+// tables, state and per-element arrays of femx_mesh_upload + the solver).
+size_t footprint(long N, long E, int npe, bool vms, bool mixed) {
+  const double per_elem = npe * (4.0 + 4.0 + 1.0 + 2.0) + 4.0 + (vms ? 12.0 * (mixed ? 4 : 8) : 24.0 + 64.0);
+  const double per_node = 32.0 + 32.0 + 8.0 + 4 * 32.0 + 4.0 + 4.0 + 3 * 12.0 + 32.0 + 16.0;
+  return (size_t)(E * per_elem + N * per_node) + (1u << 20);
+}
+
+}  // namespace
+
+struct femx_server {
+  size_t budget = 0;
+  int max_active = 0;
+  int chunk = 0;
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::shared_ptr<Request>> pending, ready;
+  std::vector<std::shared_ptr<Request>> active;
+  std::map<long long, std::shared_ptr<Request>> finished;
+  std::vector<femx_request_stats> unpolled;
+  size_t reserved = 0, peak_reserved = 0;
+  int preparing = 0, n_resident = 0;
+  long long next_id = 1;
+  bool stop = false;
+  std::string first_error;
+  std::vector<std::thread> preps;
+  std::thread gpu;
+
+  void prep_loop();
+  void gpu_loop();
+  void fail(const std::shared_ptr<Request> &r, femx_status st);
+};
+
+void femx_server::fail(const std::shared_ptr<Request> &r, femx_status st) {
+  r->release();
+  std::lock_guard<std::mutex> lk(mu);
+  r->stats.status = st;
+  r->stats.t_done = now_s();
+  if (first_error.empty()) first_error = femx_last_error();
+  reserved -= r->bytes;
+  n_resident--;
+  finished[r->id] = r;
+  unpolled.push_back(r->stats);
+  cv.notify_all();
+}
+
+void femx_server::prep_loop() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    std::shared_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      // arrival order: the head waits until its footprint fits
+      cv.wait(lk, [&] {
+        return stop || (!pending.empty() && n_resident < max_active &&
+                        (reserved + pending.front()->bytes <= budget || n_resident == 0));
+      });
+      if (stop) return;
+      r = pending.front();
+      pending.pop_front();
+      reserved += r->bytes;
+      peak_reserved = std::max(peak_reserved, reserved);
+      n_resident++;
+      r->stats.t_admit = now_s();
+    }
+    femx_mesh_desc md{r->N, r->E, r->npe, r->coords.data(), r->conn.data()};
+    const femx_bc_desc bc = r->bc();
+    femx_status st = femx_mesh_upload(&md, &r->mesh);
+    if (st == FEMX_OK) {
+      if (r->kind == FEMX_REQ_VMS) {
+        st = femx_vms_create(r->mesh, &bc, &r->vms, &r->vh);
+        if (st == FEMX_OK) {
+          std::vector<double> zero(4L * r->N, 0.0);
+          st = femx_vms_set_state(r->vh, zero.data(), nullptr);
+          r->st = (hipStream_t)femx_vms_stream(r->vh);
+        }
+      } else {
+        st = femx_supg_create(r->mesh, &bc, &r->supg, &r->sh);
+        if (st == FEMX_OK) {
+          std::vector<double> zero(3L * r->N, 0.0);
+          st = femx_supg_set_state(r->sh, zero.data());
+          r->st = (hipStream_t)femx_supg_stream(r->sh);
+        }
+      }
+    }
+    if (st == FEMX_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) st = FEMX_ERR_HIP;
+    if (st != FEMX_OK) {
+      fail(r, st);
+      continue;
+    }
+    // release the host copies of the inputs: the device holds them now
+    r->coords = std::vector<double>();
+    r->conn = std::vector<int>();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      r->stats.t_start = now_s();
+      ready.push_back(r);
+    }
+    cv.notify_all();
+  }
+}
+
+void femx_server::gpu_loop() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait_for(lk, std::chrono::microseconds(active.empty() ? 2000 : 50),
+                  [&] { return stop || !ready.empty() || !active.empty(); });
+      if (stop) return;
+      while (!ready.empty()) {
+        active.push_back(ready.front());
+        ready.pop_front();
+      }
+    }
+    bool changed = false;
+    for (size_t i = 0; i < active.size();) {
+      auto &r = active[i];
+      femx_status st = FEMX_OK;
+      bool over = false;
+      if (r->inflight) {
+        if (hipEventQuery(r->ev) == hipErrorNotReady) {
+          i++;
+          continue;
+        }
+        r->inflight = false;
+        if (r->kind == FEMX_REQ_SUPG) {
+          int d = 0;
+          st = femx_supg_finish(r->sh, &d);  // the exit condition of the chunk
+          r->done += d;
+          over = st != FEMX_OK;
+        } else {
+          r->done += r->last_k;
+        }
+      }
+      if (!over && r->done < r->steps) {
+        const int k = std::min(chunk, r->steps - r->done);
+        st = r->kind == FEMX_REQ_VMS ? femx_vms_step(r->vh, k) : femx_supg_step_async(r->sh, k);
+        if (st == FEMX_OK && hipEventRecord(r->ev, r->st) == hipSuccess) {
+          r->last_k = k;
+          r->inflight = true;
+          changed = true;
+          i++;
+          continue;
+        }
+        if (st == FEMX_OK) st = FEMX_ERR_HIP;
+      }
+      // finished (or failed): read the final state back, free the handles and the budget
+      const femx_status end_st = st;
+      if (st == FEMX_OK || st == FEMX_EXIT_CONDITION) {
+        if (r->kind == FEMX_REQ_VMS) {
+          r->result.resize(4L * r->N);
+          st = femx_vms_get_state(r->vh, r->result.data(), nullptr, nullptr);
+        } else {
+          r->result.resize(3L * r->N);
+          st = femx_supg_get_state(r->sh, r->result.data(), nullptr, nullptr, nullptr, nullptr);
+        }
+        if (st == FEMX_OK) st = end_st;
+      }
+      r->release();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        r->stats.status = st;
+        r->stats.steps_done = r->done;
+        r->stats.t_done = now_s();
+        if (st != FEMX_OK && st != FEMX_EXIT_CONDITION && first_error.empty()) first_error = femx_last_error();
+        reserved -= r->bytes;
+        n_resident--;
+        finished[r->id] = r;
+        unpolled.push_back(r->stats);
+      }
+      cv.notify_all();
+      active.erase(active.begin() + i);
+      changed = true;
+    }
+    if (!changed) std::this_thread::yield();
+  }
+}
+
+extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int chunk_steps,
+                                          femx_server **out) {
+  if (!out || max_active < 1 || prep_threads < 1 || chunk_steps < 1 || hbm_budget_bytes == 0)
+    return set_error("bad argument"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  femx_server *s = new femx_server();
+  s->budget = hbm_budget_bytes;
+  s->max_active = max_active;
+  s->chunk = chunk_steps;
+  (void)hipGetDevice(&s->device);
+  for (int i = 0; i < prep_threads; i++) s->preps.emplace_back([s] { s->prep_loop(); });
+  s->gpu = std::thread([s] { s->gpu_loop(); });
+  *out = s;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_server_submit(femx_server *s, const femx_request *q, long long *id) {
+  if (!s || !q || !q->mesh.coords || !q->mesh.conn || q->steps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  const femx_mesh_desc &d = q->mesh;
+  if (d.n_nodes <= 0 || d.n_elems <= 0 || (d.npe != 4 && d.npe != 8)) return set_error("mesh sizes"), FEMX_ERR_ARG;
+  if (q->kind != FEMX_REQ_VMS && q->kind != FEMX_REQ_SUPG) return set_error("request kind"), FEMX_ERR_ARG;
+  if (q->kind == FEMX_REQ_VMS && d.npe != 4) return set_error("VMS requests need tet4 meshes"), FEMX_ERR_UNSUPPORTED;
+  auto r = std::make_shared<Request>();
+  r->kind = q->kind;
+  r->steps = q->steps;
+  r->N = d.n_nodes;
+  r->E = d.n_elems;
+  r->npe = d.npe;
+  r->coords.assign(d.coords, d.coords + 3L * d.n_nodes);
+  r->conn.assign(d.conn, d.conn + (long)d.npe * d.n_elems);
+  const femx_bc_desc &b = q->bc;
+  r->vel_off.assign(1, 0);
+  for (int i = 0; i < b.n_vel; i++) {
+    r->vel_node.push_back(b.vel_node[i]);
+    r->vel_dir.push_back(b.vel_dir[i]);
+    for (int k = b.vel_row_off[i]; k < b.vel_row_off[i + 1]; k++)
+      r->vel_rows.insert(r->vel_rows.end(), b.vel_rows + 4L * k, b.vel_rows + 4L * k + 4);
+    r->vel_off.push_back((int)(r->vel_rows.size() / 4));
+  }
+  r->dir_node.assign(b.dir_node, b.dir_node + b.n_dir);
+  r->dir_val.assign(b.dir_val, b.dir_val + 3L * b.n_dir);
+  r->pres_node.assign(b.pres_node, b.pres_node + b.n_pres);
+  r->pres_val.assign(b.pres_val, b.pres_val + b.n_pres);
+  r->vms = q->vms;
+  r->supg = q->supg;
+  const bool vms = q->kind == FEMX_REQ_VMS;
+  r->bytes = footprint(d.n_nodes, d.n_elems, d.npe, vms,
+                       (vms ? q->vms.precision : q->supg.precision) == FEMX_MIXED);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->next_id++;
+    r->stats.id = r->id;
+    r->stats.kind = q->kind;
+    r->stats.elements = d.n_elems;
+    r->stats.nodes = d.n_nodes;
+    r->stats.steps = q->steps;
+    r->stats.bytes = (long long)r->bytes;
+    r->stats.t_submit = now_s();
+    s->pending.push_back(r);
+  }
+  s->cv.notify_all();
+  if (id) *id = r->id;
+  return FEMX_OK;
+}
+
+extern "C" int femx_server_poll(femx_server *s, femx_request_stats *out, int max_entries) {
+  if (!s || (!out && max_entries > 0)) return -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int n = std::min((int)s->unpolled.size(), max_entries);
+  for (int i = 0; i < n; i++) out[i] = s->unpolled[i];
+  s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + n);
+  return n;
+}
+
+extern "C" femx_status femx_server_result(femx_server *s, long long id, double *sol, long long len) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->finished.find(id);
+  if (it == s->finished.end()) return set_error("request not finished (or unknown id)"), FEMX_ERR_ARG;
+  const auto &r = it->second;
+  if ((long long)r->result.size() != len) return set_error("result length: VMS N*4, SUPG N*3"), FEMX_ERR_ARG;
+  if (sol) std::copy(r->result.begin(), r->result.end(), sol);
+  s->finished.erase(it);
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_server_drain(femx_server *s, double timeout_s) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+    return s->pending.empty() && s->ready.empty() && s->n_resident == 0;
+  });
+  if (!ok) return set_error("drain timed out"), FEMX_ERR_ARG;
+  if (!s->first_error.empty()) return set_error(s->first_error), FEMX_ERR_HIP;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_server_info(femx_server *s, long long *info) {
+  if (!s || !info) return set_error("null argument"), FEMX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  info[0] = (long long)s->pending.size();
+  info[1] = s->n_resident;
+  info[2] = (long long)s->reserved;
+  info[3] = (long long)s->peak_reserved;
+  info[4] = (long long)s->budget;
+  info[5] = s->max_active;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_server_destroy(femx_server *s) {
+  if (!s) return FEMX_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  for (auto &t : s->preps) t.join();
+  s->gpu.join();
+  for (auto &r : s->active) {
+    if (r->st) (void)hipStreamSynchronize(r->st);
+    r->release();
+  }
+  for (auto &r : s->ready) r->release();
+  delete s;
+  return FEMX_OK;
+}

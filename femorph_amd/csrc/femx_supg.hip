@@ -651,6 +651,10 @@ struct femx_supg {
   bool press_pending = false;
   bool diag_enabled = false;
   bool halted = false;  // the exit condition fired; the solve is over
+  // steps enqueued and not yet checked for the exit condition
+  long pend_n = 0, pend_first = 0;
+  double pend_t0 = 0.0;
+  int pend_cur0 = 0;
   KernelTimer timer;
 };
 
@@ -923,6 +927,7 @@ extern "C" femx_status femx_supg_set_state(femx_supg *s, const double *sol) {
   s->loop_time = 0;
   s->time = 0.0;
   s->halted = false;
+  s->pend_n = 0;
   FEMX_HIP(hipMemcpyAsync(s->U[0].p, h.data(), m.N * sizeof(double4), hipMemcpyHostToDevice, s->st));
   FEMX_HIP(hipMemsetAsync(s->bc.err.p, 0, sizeof(int), s->st));
   const int imax = INT_MAX;
@@ -943,13 +948,15 @@ extern "C" femx_status femx_supg_set_state(femx_supg *s, const double *sol) {
   return r;
 }
 
-extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done) {
-  if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
-  if (steps_done) *steps_done = 0;
-  if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
-  const long first = s->loop_time;
-  const double t0 = s->time;
-  const int cur0 = s->cur;
+namespace {
+
+// Enqueues nsteps time steps (:1034-1734) without waiting for them.
+femx_status supg_enqueue(femx_supg *s, int nsteps) {
+  if (s->pend_n == 0) {
+    s->pend_first = s->loop_time;
+    s->pend_t0 = s->time;
+    s->pend_cur0 = s->cur;
+  }
   for (int k = 0; k < nsteps; k++) {
     s->loop_time++;
     s->time += s->p.dt;  // currentTime += timeStep (:1041)
@@ -966,8 +973,20 @@ extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done)
     supg_diag_reduce<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0, row,
                                            s->halt.p);
     FEMX_HIP(hipGetLastError());
+    s->pend_n++;
   }
-  // the exit condition of the last step, and any halt inside the batch
+  return FEMX_OK;
+}
+
+// Waits for the enqueued steps and applies the exit condition (:1633-1643):
+// when a step's max |u| exceeded it, the later steps did not run and the
+// host counters are rewound to that step.
+femx_status supg_finish(femx_supg *s, int *steps_done) {
+  const long nsteps = s->pend_n, first = s->pend_first;
+  const double t0 = s->pend_t0;
+  const int cur0 = s->pend_cur0;
+  s->pend_n = 0;
+  if (steps_done) *steps_done = 0;
   int halt = INT_MAX;
   double last[D_COLS];
   FEMX_HIP(hipMemcpyAsync(&halt, s->halt.p, sizeof(int), hipMemcpyDeviceToHost, s->st));
@@ -994,14 +1013,44 @@ extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done)
     set_error("ERROR: Maximum velocity module larger than admissible threshold.");
     return FEMX_EXIT_CONDITION;
   }
-  if (steps_done) *steps_done = nsteps;
+  if (steps_done) *steps_done = (int)nsteps;
   return FEMX_OK;
+}
+
+}  // namespace
+
+extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done) {
+  if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (steps_done) *steps_done = 0;
+  if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
+  femx_status r = supg_enqueue(s, nsteps);
+  if (r != FEMX_OK) return r;
+  return supg_finish(s, steps_done);
+}
+
+extern "C" femx_status femx_supg_step_async(femx_supg *s, int nsteps) {
+  if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
+  return supg_enqueue(s, nsteps);
+}
+
+extern "C" femx_status femx_supg_finish(femx_supg *s, int *steps_done) {
+  if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->halted && s->pend_n == 0) {
+    if (steps_done) *steps_done = 0;
+    return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
+  }
+  return supg_finish(s, steps_done);
 }
 
 extern "C" femx_status femx_supg_get_state(femx_supg *s, double *sol, double *pres, double *lam, double *regime,
                                            double *time) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
   Mesh &m = *s->mesh;
+  if (s->pend_n > 0) {
+    const femx_status f = supg_finish(s, nullptr);
+    if (f != FEMX_OK && f != FEMX_EXIT_CONDITION) return f;
+  }
   femx_status r = flush_pressure(s);
   if (r != FEMX_OK) return r;
   if (sol) {
@@ -1040,6 +1089,10 @@ extern "C" femx_status femx_supg_get_mass(femx_supg *s, double *l_M) {
 
 extern "C" femx_status femx_supg_get_log(femx_supg *s, double *cols19) {
   if (!s || !cols19) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->pend_n > 0) {
+    const femx_status f = supg_finish(s, nullptr);
+    if (f != FEMX_OK && f != FEMX_EXIT_CONDITION) return f;
+  }
   double k[D_COLS];
   FEMX_HIP(hipMemcpyAsync(k, s->ring.p + (s->loop_time % RING) * D_COLS, sizeof(k), hipMemcpyDeviceToHost, s->st));
   FEMX_HIP(hipStreamSynchronize(s->st));

@@ -85,15 +85,24 @@ __device__ __forceinline__ void tet_geometry(const double X[4][3], double DN[3][
   h = cbrt(6.0 * V);
 }
 
+// Reciprocal, square root and h^-1 = (6V)^(-1/3): IEEE-rounded in FP64; in
+// FP32 the hardware approximations (v_rcp_f32, v_sqrt_f32, v_log_f32 /
+// v_exp_f32, about 1 ulp), which are within the FP32 element arithmetic's
+// own rounding and cost one instruction instead of a correctly rounded
+// sequence.
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double inv_cbrt_t(double x) { return 1.0 / cbrt(x); }
+__device__ __forceinline__ float inv_cbrt_t(float x) {
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (-1.0f / 3.0f));
+}
+
 // The same geometry for the element kernel: edge vectors formed in FP64 (no
-// cancellation), the cofactors, inverse and DN in T.  Returns V and h in T.
+// cancellation), the cofactors, inverse and DN in T.  Returns V and 1/h in T.
 template <typename T>
-__device__ __forceinline__ void tet_geometry_t(const double X[4][3], T DN[3][4], T &V, T &h) {
-  T jac[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) jac[i][j] = (T)(X[j + 1][i] - X[0][i]);
+__device__ __forceinline__ void tet_geometry_core(const T jac[3][3], T DN[3][4], T &V, T &ih) {
   T cof[3][3];
   cof[0][0] = jac[1][1] * jac[2][2] - jac[2][1] * jac[1][2];
   cof[0][1] = jac[2][0] * jac[1][2] - jac[1][0] * jac[2][2];
@@ -105,7 +114,7 @@ __device__ __forceinline__ void tet_geometry_t(const double X[4][3], T DN[3][4],
   cof[2][1] = jac[1][0] * jac[0][2] - jac[0][0] * jac[1][2];
   cof[2][2] = jac[0][0] * jac[1][1] - jac[1][0] * jac[0][1];
   const T detJ = jac[0][0] * cof[0][0] + jac[0][1] * cof[0][1] + jac[0][2] * cof[0][2];
-  const T iDet = (T)1 / detJ;
+  const T iDet = rcp_t(detJ);
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const T i0 = cof[i][0] * iDet, i1 = cof[i][1] * iDet, i2 = cof[i][2] * iDet;
@@ -115,7 +124,16 @@ __device__ __forceinline__ void tet_geometry_t(const double X[4][3], T DN[3][4],
     DN[i][0] = -i0 - i1 - i2;
   }
   V = detJ * (T)(1.0 / 6.0);
-  h = cbrt(detJ);  // (6 V)^(1/3)
+  ih = inv_cbrt_t(detJ);  // 1 / h, h = (6 V)^(1/3)
+}
+template <typename T, typename X_t>
+__device__ __forceinline__ void tet_geometry_t(const X_t X[4][3], T DN[3][4], T &V, T &ih) {
+  T jac[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) jac[i][j] = (T)(X[j + 1][i] - X[0][i]);
+  tet_geometry_core<T>(jac, DN, V, ih);
 }
 
 struct VmsConst {
@@ -162,12 +180,34 @@ __device__ __forceinline__ void elem_vel(const T DN[3][4], const double uv[4][3]
   }
 }
 
+// The same from FP32 velocities relative to the block's reference velocity
+// ref (u = ref + rel): the differences u_a - u_0 are differences of the
+// small relative values, so they keep their significant digits.
+__device__ __forceinline__ void elem_vel_rel(const float DN[3][4], const float ref[3], const float rel[4][3],
+                                             ElemVel<float> &ev) {
+  float du[3][3];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) ev.u[a][i] = ref[i] + rel[a][i];
+#pragma unroll
+  for (int a = 1; a < 4; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) du[a - 1][i] = rel[a][i] - rel[0][i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    ev.su[i] = (ev.u[0][i] + ev.u[1][i]) + (ev.u[2][i] + ev.u[3][i]);
+#pragma unroll
+    for (int j = 0; j < 3; j++) ev.grad[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
+  }
+}
+
 // Projection + subscale update of one element (assemble_projRes_explicit
 // :406-588 and update_subscales_explicit :593-646, element-based projection,
 // include_sg_p = false so the pressure subscale stays 0).
 // ev: pre-BC nodal velocity; q: pressures of global nodes 0..3.
 template <typename T>
-__device__ __forceinline__ void vms_projection(const T DN[3][4], T h, const ElemVel<T> &ev, const double q[4],
+__device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const ElemVel<T> &ev, const double q[4],
                                                T sdu[4][3], const VmsConst &k) {
   T dq[3];
 #pragma unroll
@@ -195,9 +235,8 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T h, const Elem
   }
   // tau1 = 1 / (1/dt + 1/tau_u), 1/tau_u = c1 nu / h^2 + c2 max|a| / h  (:581-585)
   const T inv_dt = (T)k.inv_dt;
-  const T ih = (T)1 / h;
-  const T itu = ((T)k.c1 * (T)k.nu * ih + (T)k.c2 * sqrt(maxA2)) * ih;
-  const T tau1 = (T)1 / (inv_dt + itu);
+  const T itu = ((T)k.c1 * (T)k.nu * ih + (T)k.c2 * sqrt_t(maxA2)) * ih;
+  const T tau1 = rcp_t(inv_dt + itu);
   const T ps = (T)PR_SELF, pm = (T)PR_SUM;
 #pragma unroll
   for (int g = 0; g < 4; g++)
@@ -206,14 +245,14 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T h, const Elem
 }
 
 // Element residual lRes[4][4] of assemble_RHS (:93-399) with sgps = 0 and
-// force = 0.  uq: post-BC nodal (u v w p); uprev: the "previous" velocity
-// (only read with ACC, the acceleration term of later dual iterations).
+// force = 0.  dd: nodal u - u' (post-BC state minus the "previous" one),
+// read only with ACC, the acceleration term of later dual iterations.
 // Diffusion, pressure gradient and subgrid convection are one 3x3 matrix K
 // applied to DN: res[b][i] = sum_j K[i][j] DN[j][b] + convection.
 template <typename T, bool ACC>
 __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVel<T> &ev, const T p[4],
-                                             const double uq[4][3], const double uprev[4][3], const T sdu[4][3],
-                                             T res[4][4], const VmsConst &k) {
+                                             const T dd[4][3], const T sdu[4][3], T res[4][4],
+                                             const VmsConst &k) {
   const T(&u)[4][3] = ev.u;
   const T(&su)[3] = ev.su;
   const T(&gradU)[3][3] = ev.grad;
@@ -262,11 +301,7 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   }
   if (ACC) {
     // acceleration (:324-325): sum_g w lN[g][b] sum_c lN[g][c] (u_c - u'_c) / dt
-    T dd[4][3], ds[3];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) dd[a][i] = (T)(uq[a][i] - uprev[a][i]);
+    T ds[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) ds[i] = (dd[0][i] + dd[1][i]) + (dd[2][i] + dd[3][i]);
     const T wi = w * (T)k.inv_dt;
@@ -301,6 +336,12 @@ struct BlockArgs {
   const int *perm;   // block order -> original element id
   double *diag_res;  // [E][4][4] element residual contributions, original order
   double *diag_sdu;  // [E][3] mean over the Gauss points of the subgrid velocity the residual used
+  // batches of meshes (NULL for one mesh): per block its mesh, per mesh the
+  // constants (incl. its time) and its first node (the quirk's nodes 0..3)
+  const int *blk_mesh;
+  const VmsConst *kt;
+  const int *mesh_node0;
+  int xcd;  // 1: XCD-aware block order, 0: dispatch order
   VmsConst k;
 };
 
@@ -365,17 +406,29 @@ __device__ __forceinline__ T *row_ptr(T *base, size_t row_elems, unsigned byte_o
 }
 
 template <typename T, bool PROJ, bool RES, bool ACC>
-__global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
+__global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 4 : 2)) void vms_block(BlockArgs<T> a) {
   using T4 = typename Vec4<T>::type;
-  // staged node records as 16-byte pairs: {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}
-  __shared__ double2 snd[VN][5];
-  __shared__ double sim[VN], spp[ACC ? VN : 1];
+  // FP64: staged node records as 16-byte pairs {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}.
+  // FP32 element math: coordinates and velocities relative to the block's
+  // first local node, formed in FP64 and rounded once per node:
+  //   sfa = (x y z pu), sfb = (pv pw qu qv), sfc = (qw, qp)
+  constexpr bool F32 = sizeof(T) == 4;
+  __shared__ double2 snd[F32 ? 1 : VN][5];
+  __shared__ float4 sfa[F32 ? VN : 1], sfb[F32 ? VN : 1];
+  __shared__ float2 sfc[F32 ? VN : 1];
   __shared__ int sbc[VN];  // 1 for nodes with a BC record (pre-BC and post-BC states may differ)
   // element-corner contributions (u v w p) in the block's node-sorted order,
   // one T4 record per corner at a bank-swizzled position
   __shared__ T4 sres[RES ? 4 * VB : 1];
   const int tid = threadIdx.x;
-  const int blk = xcd_block(blockIdx.x, a.nblk);
+  const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
+  VmsConst k = a.k;
+  int node0 = 0;
+  if (a.kt) {
+    const int mid = a.blk_mesh[blk];
+    k = a.kt[mid];
+    node0 = a.mesh_node0[mid];
+  }
   const int n0 = a.node_off[blk], nn = a.node_off[blk + 1] - n0;
   const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
   // 0. issue this thread's element loads before staging (they overlap)
@@ -394,85 +447,151 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
 #pragma unroll
       for (int i = 0; i < 3; i++) sdu[g][i] = *row_ptr(a.sdus, (g * 3 + i) * E, eoff);
   }
-  // node-side metadata of the local node this thread owns (nn <= VN <= VB)
+  // the local node this thread owns (nn <= VN <= VB); its phase-3 data is
+  // re-read there (from L2) rather than held in registers across phase 2
   const bool has_n = tid < nn;
-  int g_own = 0, ps_own = -1, s0_own = 0, s1_own = 0, rec_own = -1;
+  int g_own = 0, rec_own = -1;
   if (has_n) {
     g_own = a.loc_node[n0 + tid];
     rec_own = a.node_rec[g_own];
-    if (RES) {
-      ps_own = a.loc_pslot[n0 + tid];
-      s0_own = a.lcsr_off[n0 + tid];
-      s1_own = a.lcsr_off[n0 + tid + 1];
-    }
+  }
+  // reference node of the FP32 relative values: the block's first local node
+  double4 xr = make_double4(0, 0, 0, 0), pr = make_double4(0, 0, 0, 0), qr = make_double4(0, 0, 0, 0);
+  if (F32) {
+    const int gr = a.loc_node[n0];
+    xr = a.xyzm[gr];
+    if (PROJ || ACC) pr = a.P[gr];
+    if (RES) qr = a.Q[gr];
   }
   // 1. stage the block's nodes
-  double4 q_own = make_double4(0, 0, 0, 0), p_own = make_double4(0, 0, 0, 0);
   if (has_n) {
     const int j = tid, g = g_own;
     const double4 x = a.xyzm[g];
+    double4 p_own = make_double4(0, 0, 0, 0), q_own = make_double4(0, 0, 0, 0);
     if (PROJ || ACC) p_own = a.P[g];
     if (RES) q_own = a.Q[g];
-    snd[j][0] = make_double2(x.x, x.y);
-    snd[j][1] = make_double2(x.z, p_own.x);
-    snd[j][2] = make_double2(p_own.y, p_own.z);
-    snd[j][3] = make_double2(q_own.x, q_own.y);
-    snd[j][4] = make_double2(q_own.z, q_own.w);
-    sim[j] = x.w;
+    if (F32) {
+      sfa[j] = make_float4((float)(x.x - xr.x), (float)(x.y - xr.y), (float)(x.z - xr.z), (float)(p_own.x - pr.x));
+      sfb[j] = make_float4((float)(p_own.y - pr.y), (float)(p_own.z - pr.z), (float)(q_own.x - qr.x),
+                           (float)(q_own.y - qr.y));
+      sfc[j] = make_float2((float)(q_own.z - qr.z), (float)q_own.w);
+    } else {
+      snd[j][0] = make_double2(x.x, x.y);
+      snd[j][1] = make_double2(x.z, p_own.x);
+      snd[j][2] = make_double2(p_own.y, p_own.z);
+      snd[j][3] = make_double2(q_own.x, q_own.y);
+      snd[j][4] = make_double2(q_own.z, q_own.w);
+    }
     sbc[j] = rec_own >= 0;
-    if (ACC) spp[j] = p_own.w;
   }
   double q4[4];
   if (PROJ) {
     // p[i] = sol_n[i][3]: pressures of GLOBAL nodes 0..3 (:489)
 #pragma unroll
-    for (int n = 0; n < 4; n++) q4[n] = a.P[n].w;
+    for (int n = 0; n < 4; n++) q4[n] = a.P[node0 + n].w;
   }
   __syncthreads();
   // 2. one element per thread
   if (has_e) {
     const int id[4] = {c.x, c.y, c.z, c.w};
-    double X[4][3], up[4][3], uq[4][3];
-    T p[4];
+    T DN[3][4], V, ih;
+    ElemVel<T> ev;
+    T p[4], dd[4][3];
+    if constexpr (F32) {
+      float X[4][3], up[4][3], uq[4][3];
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-      const double2 r0 = snd[id[n]][0], r1 = snd[id[n]][1];
-      X[n][0] = r0.x;
-      X[n][1] = r0.y;
-      X[n][2] = r1.x;
-      if (PROJ || ACC) {
-        const double2 r2 = snd[id[n]][2];
-        up[n][0] = r1.y;
-        up[n][1] = r2.x;
-        up[n][2] = r2.y;
+      for (int n = 0; n < 4; n++) {
+        const float4 ra = sfa[id[n]];
+        X[n][0] = ra.x;
+        X[n][1] = ra.y;
+        X[n][2] = ra.z;
+        if (PROJ || ACC) {
+          const float4 rb = sfb[id[n]];
+          up[n][0] = ra.w;
+          up[n][1] = rb.x;
+          up[n][2] = rb.y;
+        }
+      }
+      tet_geometry_t<float>(X, DN, V, ih);
+      const float pref[3] = {(float)pr.x, (float)pr.y, (float)pr.z};
+      if (PROJ) {
+        elem_vel_rel(DN, pref, up, ev);
+        vms_projection<float>(DN, ih, ev, q4, sdu, k);
+      }
+      if (RES) {
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float4 rb = sfb[id[n]];
+          const float2 rc = sfc[id[n]];
+          uq[n][0] = rb.z;
+          uq[n][1] = rb.w;
+          uq[n][2] = rc.x;
+          p[n] = rc.y;
+        }
+        const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
+        if (!reuse) {
+          const float qref[3] = {(float)qr.x, (float)qr.y, (float)qr.z};
+          elem_vel_rel(DN, qref, uq, ev);
+        }
+        if (ACC) {
+          // u - u' = (qref - pref) + (q_rel - p_rel)
+          const float dr[3] = {(float)(qr.x - pr.x), (float)(qr.y - pr.y), (float)(qr.z - pr.z)};
+#pragma unroll
+          for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) dd[n][i] = dr[i] + (uq[n][i] - up[n][i]);
+        }
+      }
+    } else {
+      double X[4][3], up[4][3], uq[4][3];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const double2 r0 = snd[id[n]][0], r1 = snd[id[n]][1];
+        X[n][0] = r0.x;
+        X[n][1] = r0.y;
+        X[n][2] = r1.x;
+        if (PROJ || ACC) {
+          const double2 r2 = snd[id[n]][2];
+          up[n][0] = r1.y;
+          up[n][1] = r2.x;
+          up[n][2] = r2.y;
+        }
+      }
+      tet_geometry_t<T>(X, DN, V, ih);
+      if (PROJ) {
+        elem_vel<T>(DN, up, ev);
+        vms_projection<T>(DN, ih, ev, q4, sdu, k);
+      }
+      if (RES) {
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const double2 r3 = snd[id[n]][3], r4 = snd[id[n]][4];
+          uq[n][0] = r3.x;
+          uq[n][1] = r3.y;
+          uq[n][2] = r4.x;
+          p[n] = (T)r4.y;
+        }
+        // without BC nodes the post-BC velocities equal the pre-BC ones the
+        // projection just used: reuse their gradient
+        const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
+        if (!reuse) elem_vel<T>(DN, uq, ev);
+        if (ACC) {
+#pragma unroll
+          for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) dd[n][i] = (T)(uq[n][i] - up[n][i]);
+        }
       }
     }
-    T DN[3][4], V, h;
-    tet_geometry_t<T>(X, DN, V, h);
-    ElemVel<T> ev;
     if (PROJ) {
-      elem_vel<T>(DN, up, ev);
-      vms_projection<T>(DN, h, ev, q4, sdu, a.k);
 #pragma unroll
       for (int g = 0; g < 4; g++)
 #pragma unroll
         for (int i = 0; i < 3; i++) *row_ptr(a.sdus, (g * 3 + i) * E, eoff) = sdu[g][i];
     }
     if (RES) {
-#pragma unroll
-      for (int n = 0; n < 4; n++) {
-        const double2 r3 = snd[id[n]][3], r4 = snd[id[n]][4];
-        uq[n][0] = r3.x;
-        uq[n][1] = r3.y;
-        uq[n][2] = r4.x;
-        p[n] = (T)r4.y;
-      }
-      // without BC nodes the post-BC velocities equal the pre-BC ones the
-      // projection just used: reuse their gradient
-      const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
-      if (!reuse) elem_vel<T>(DN, uq, ev);
       T res[4][4];
-      vms_residual<T, ACC>(DN, V, ev, p, uq, up, sdu, res, a.k);
+      vms_residual<T, ACC>(DN, V, ev, p, dd, sdu, res, k);
       if (a.diag_res) {
         const size_t eo = (size_t)a.perm[e];
         double *d = a.diag_res + eo * 16;
@@ -505,9 +624,10 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
   // 3. per local node: sum its contiguous contributions in a fixed order
   double m2 = 0.0;
   if (has_n) {
-    const int j = tid, g = g_own;
+    const int g = g_own;
+    const int ps_own = a.loc_pslot[n0 + tid];
     double R[4] = {0.0, 0.0, 0.0, 0.0};
-    const int s0 = s0_own - e0 * 4, s1 = s1_own - e0 * 4;
+    const int s0 = a.lcsr_off[n0 + tid] - e0 * 4, s1 = a.lcsr_off[n0 + tid + 1] - e0 * 4;
 #pragma unroll 4
     for (int s = s0; s < s1; s++) {
       const T4 v = sres[swz<T4>(s)];
@@ -518,9 +638,9 @@ __global__ __launch_bounds__(VB) void vms_block(BlockArgs<T> a) {
     }
     if (ps_own < 0) {
       // base of update_dual_solution: post-BC at the first dual iteration, pre-BC after
-      const double4 base = !ACC ? q_own : p_own;
+      const double4 base = !ACC ? a.Q[g] : a.P[g];
       double4 pre, post;
-      m2 = node_update(base, sim[j], R, a.k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, a.xyzm[g].w, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -542,6 +662,8 @@ struct FixArgs {
   const double4 *rows;
   int *err;
   double *bmax;  // [grid] per-block max of the pre-BC |u|^2
+  const int *node_mesh;  // batches: mesh of every node (NULL for one mesh)
+  const VmsConst *kt;
   VmsConst k;
 };
 
@@ -559,8 +681,9 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
       R[3] += p.w;
     }
     const double4 base = a.first_dual ? a.Q[g] : a.P[g];
+    const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
-    m2 = node_update(base, a.xyzm[g].w, R, a.k, pre, post, a.node_rec[g], a.rec, a.row_off, a.rows, a.err);  // w = 1/m
+    m2 = node_update(base, a.xyzm[g].w, R, k, pre, post, a.node_rec[g], a.rec, a.row_off, a.rows, a.err);  // w = 1/m
     a.Pn[g] = pre;
     a.Qn[g] = post;
   }
@@ -741,7 +864,21 @@ struct femx_vms {
   DevBuf<double> diag_el;    // [E][3] centroid residual u v w, then [E] p
   DevBuf<double> diag_nmax;  // [blocks][2]
   DevBuf<double> diag_iter;  // [tot_dual_steps + 1][3] of the last step
+  // batch of meshes (femx_vms_batch_*): the mesh is the union of the batch
+  int n_mesh = 1;
+  femx_mesh *owned = nullptr;
+  DevBuf<VmsConst> kt;       // per mesh constants, .t = the mesh's time
+  DevBuf<double> kdt;        // per mesh dt
+  DevBuf<int> blk_mesh, mesh_node0, node_mesh;
+  std::vector<int> h_node0, h_nnodes;
+  std::vector<double> h_time, h_dt;
+  int xcd = 1;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps, by state parity
   KernelTimer timer;
+  ~femx_vms() {
+    for (auto &g : graph)
+      if (g) (void)hipGraphExecDestroy(g);
+  }
 };
 
 namespace {
@@ -803,6 +940,11 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.perm = b.d_perm.p;
   a.diag_res = s->diag ? s->diag_res.p : nullptr;
   a.diag_sdu = s->diag ? s->diag_sdu.p : nullptr;
+  const bool batch = s->n_mesh > 1;
+  a.blk_mesh = batch ? s->blk_mesh.p : nullptr;
+  a.kt = batch ? s->kt.p : nullptr;
+  a.mesh_node0 = batch ? s->mesh_node0.p : nullptr;
+  a.xcd = s->xcd;
   a.k = vms_const(s, t);
   return a;
 }
@@ -851,6 +993,8 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.rows = s->bc.rows.p;
   a.err = s->bc.err.p;
   a.bmax = s->bmax.p + b.nb;
+  a.node_mesh = s->n_mesh > 1 ? s->node_mesh.p : nullptr;
+  a.kt = s->n_mesh > 1 ? s->kt.p : nullptr;
   a.k = vms_const(s, t);
   vms_fixup<<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
@@ -949,7 +1093,9 @@ extern "C" femx_status femx_vms_destroy(femx_vms *s) {
   if (!s) return FEMX_OK;
   hipStream_t st = s->st;
   if (st) (void)hipStreamSynchronize(st);
+  femx_mesh *own = s->owned;
   delete s;
+  delete own;
   if (st) (void)hipStreamDestroy(st);
   return FEMX_OK;
 }
@@ -1112,4 +1258,315 @@ extern "C" femx_status femx_vms_kernel_timing(femx_vms *s, int enable) {
 extern "C" int femx_vms_kernel_times(femx_vms *s, femx_kernel_time *out, int max_entries) {
   if (!s) return -1;
   return s->timer.export_to(out, max_entries);
+}
+
+// ------------------------------------------------------------ batches
+// Many independent VMS tet4 meshes stepped together.  The meshes are
+// concatenated into one union mesh whose element blocks never mix meshes;
+// every block reads its mesh's constants (nu, invEps, c1, c2, 1/dt,
+// dual_step and the mesh's own time) from a small device table, so one
+// launch of the block kernel and one of the fix-up advance every mesh by
+// one dual iteration.  Steps are replayed from a captured hipGraph of two
+// steps (the state ping-pong returns to the same parity after two steps).
+
+struct femx_vms_batch {
+  femx_vms *s = nullptr;
+};
+
+namespace {
+
+// time of every mesh: currentTime += timeStep (:1028), on the device so
+// that a captured graph advances it
+__global__ void vms_batch_tick(int n, VmsConst *kt, const double *dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) kt[i].t = kt[i].t + dt[i];
+}
+
+femx_status batch_step_once(femx_vms *s) {
+  vms_batch_tick<<<grid_for(s->n_mesh, 256), 256, 0, s->st>>>(s->n_mesh, s->kt.p, s->kdt.p);
+  FEMX_HIP(hipGetLastError());
+  for (int d = 0; d <= s->p.tot_dual_steps; d++) {
+    femx_status r = launch_block(s, s->proj_pending, true, d > 0, d == 0, 0.0);
+    if (r != FEMX_OK) return r;
+    if ((r = launch_fixup(s, d == 0, 0.0)) != FEMX_OK) return r;
+    if (s->diag && (r = launch_diag(s, d)) != FEMX_OK) return r;
+    s->cur ^= 1;
+    s->proj_pending = true;
+    s->iter++;
+  }
+  return FEMX_OK;
+}
+
+femx_status batch_capture(femx_vms *s, int par) {
+  const bool timing = s->timer.enabled;
+  s->timer.enabled = false;
+  const int cur0 = s->cur;
+  const bool pp0 = s->proj_pending;
+  const long it0 = s->iter;
+  FEMX_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  femx_status r = batch_step_once(s);
+  if (r == FEMX_OK) r = batch_step_once(s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s->st, &g);
+  // the capture recorded the launches without running them
+  s->cur = cur0;
+  s->proj_pending = pp0;
+  s->iter = it0;
+  s->timer.enabled = timing;
+  if (r != FEMX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  FEMX_HIP(e);
+  const hipError_t ie = hipGraphInstantiate(&s->graph[par], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  FEMX_HIP(ie);
+  return FEMX_OK;
+}
+
+}  // namespace
+
+extern "C" femx_status femx_vms_batch_create(int n, const femx_mesh_desc *meshes, const femx_bc_desc *bcs,
+                                             const femx_vms_params *params, int block_elems, int xcd_order,
+                                             femx_vms_batch **out) {
+  if (n < 1 || !meshes || !params || !out) return set_error("null argument or empty batch"), FEMX_ERR_ARG;
+  if (block_elems < 0 || block_elems > VB) return set_error("block_elems must be 0 or 1..256"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  long N = 0, E = 0, rows = 0;
+  for (int i = 0; i < n; i++) {
+    const femx_mesh_desc &d = meshes[i];
+    const femx_vms_params &p = params[i];
+    if (!d.coords || !d.conn || d.npe != 4 || d.n_nodes < 4 || d.n_elems <= 0)
+      return set_error("every batch mesh must be tet4 with >= 4 nodes"), FEMX_ERR_ARG;
+    if (p.precision != params[0].precision || p.tot_dual_steps != params[0].tot_dual_steps)
+      return set_error("precision and tot_dual_steps must be the same for the whole batch"), FEMX_ERR_ARG;
+    if (!(p.dt > 0.0) || p.rho == 0.0 || p.tot_dual_steps < 0) return set_error("parameters"), FEMX_ERR_ARG;
+    for (long k = 0; k < 4L * d.n_elems; k++)
+      if (d.conn[k] < 0 || d.conn[k] >= d.n_nodes) return set_error("connectivity out of range"), FEMX_ERR_ARG;
+    N += d.n_nodes;
+    E += d.n_elems;
+    if (bcs && bcs[i].n_vel > 0) rows += bcs[i].vel_row_off[bcs[i].n_vel];
+  }
+  if (4 * E > 2147483647L || N > 2147483647L) return set_error("batch too large for 32-bit slots"), FEMX_ERR_ARG;
+  // union mesh and union BCs (node ids shifted by each mesh's first node)
+  std::vector<double> coords(3 * N);
+  std::vector<int> conn(4 * E), group(E), node0(n), nnodes(n), node_mesh(N);
+  std::vector<int> vel_node, vel_dir, vel_off(1, 0), dir_node, pres_node;
+  std::vector<double> vel_rows, dir_val, pres_val;
+  long nb = 0, eb = 0;
+  for (int i = 0; i < n; i++) {
+    const femx_mesh_desc &d = meshes[i];
+    node0[i] = (int)nb;
+    nnodes[i] = d.n_nodes;
+    std::copy(d.coords, d.coords + 3L * d.n_nodes, coords.begin() + 3 * nb);
+    for (long k = 0; k < 4L * d.n_elems; k++) conn[4 * eb + k] = d.conn[k] + (int)nb;
+    std::fill(group.begin() + eb, group.begin() + eb + d.n_elems, i);
+    std::fill(node_mesh.begin() + nb, node_mesh.begin() + nb + d.n_nodes, i);
+    if (bcs) {
+      const femx_bc_desc &b = bcs[i];
+      for (int v = 0; v < b.n_vel; v++) {
+        if (b.vel_node[v] < 0 || b.vel_node[v] >= d.n_nodes) return set_error("velocity BC node out of range"), FEMX_ERR_ARG;
+        vel_node.push_back(b.vel_node[v] + (int)nb);
+        vel_dir.push_back(b.vel_dir[v]);
+        for (int r = b.vel_row_off[v]; r < b.vel_row_off[v + 1]; r++)
+          vel_rows.insert(vel_rows.end(), b.vel_rows + 4L * r, b.vel_rows + 4L * r + 4);
+        vel_off.push_back((int)(vel_rows.size() / 4));
+      }
+      for (int v = 0; v < b.n_dir; v++) {
+        if (b.dir_node[v] < 0 || b.dir_node[v] >= d.n_nodes) return set_error("Dirichlet BC node out of range"), FEMX_ERR_ARG;
+        dir_node.push_back(b.dir_node[v] + (int)nb);
+        dir_val.insert(dir_val.end(), b.dir_val + 3L * v, b.dir_val + 3L * v + 3);
+      }
+      for (int v = 0; v < b.n_pres; v++) {
+        if (b.pres_node[v] < 0 || b.pres_node[v] >= d.n_nodes) return set_error("pressure BC node out of range"), FEMX_ERR_ARG;
+        pres_node.push_back(b.pres_node[v] + (int)nb);
+        pres_val.push_back(b.pres_val[v]);
+      }
+    }
+    nb += d.n_nodes;
+    eb += d.n_elems;
+  }
+  (void)rows;
+  femx_bc_desc ub;
+  ub.n_vel = (int)vel_node.size();
+  ub.vel_node = vel_node.data();
+  ub.vel_dir = vel_dir.data();
+  ub.vel_row_off = vel_off.data();
+  ub.vel_rows = vel_rows.data();
+  ub.n_dir = (int)dir_node.size();
+  ub.dir_node = dir_node.data();
+  ub.dir_val = dir_val.data();
+  ub.n_pres = (int)pres_node.size();
+  ub.pres_node = pres_node.data();
+  ub.pres_val = pres_val.data();
+  femx_mesh *um = new femx_mesh();
+  femx_status r = mesh_setup(um->m, coords.data(), conn.data(), N, E, 4, block_elems, &group);
+  if (r != FEMX_OK) {
+    delete um;
+    return r;
+  }
+  femx_vms *s = nullptr;
+  if ((r = femx_vms_create(um, &ub, &params[0], &s)) != FEMX_OK) {
+    delete um;
+    return r;
+  }
+  s->owned = um;
+  s->n_mesh = n;
+  s->xcd = xcd_order ? 1 : 0;
+  s->h_node0 = node0;
+  s->h_nnodes = nnodes;
+  s->h_time.assign(n, 0.0);
+  s->h_dt.resize(n);
+  std::vector<VmsConst> kt(n);
+  for (int i = 0; i < n; i++) {
+    const femx_vms_params &p = params[i];
+    kt[i].nu = p.mu / p.rho;
+    kt[i].inv_eps = p.inv_eps;
+    kt[i].c1 = p.c1;
+    kt[i].c2 = p.c2;
+    kt[i].inv_dt = 1.0 / p.dt;
+    kt[i].dual_step = p.dual_step;
+    kt[i].t = 0.0;
+    s->h_dt[i] = p.dt;
+  }
+  const Blocks &bk = um->m.blk;
+  std::vector<int> bm(bk.nb);
+  for (int k = 0; k < bk.nb; k++) bm[k] = group[bk.perm[bk.elem_off[k]]];
+  femx_vms_batch *bt = new femx_vms_batch();
+  bt->s = s;
+  if ((r = s->kt.upload(kt.data(), n, s->st)) != FEMX_OK || (r = s->kdt.upload(s->h_dt.data(), n, s->st)) != FEMX_OK ||
+      (r = s->blk_mesh.upload(bm.data(), bm.size(), s->st)) != FEMX_OK ||
+      (r = s->mesh_node0.upload(node0.data(), n, s->st)) != FEMX_OK ||
+      (r = s->node_mesh.upload(node_mesh.data(), N, s->st)) != FEMX_OK) {
+    femx_vms_destroy(s);
+    delete bt;
+    return r;
+  }
+  *out = bt;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_batch_destroy(femx_vms_batch *b) {
+  if (!b) return FEMX_OK;
+  femx_vms_destroy(b->s);
+  delete b;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_batch_set_state(femx_vms_batch *b, const double *const *sols) {
+  if (!b) return set_error("null argument"), FEMX_ERR_ARG;
+  femx_vms *s = b->s;
+  std::vector<double> u(4L * s->mesh->N, 0.0);
+  for (int i = 0; sols && i < s->n_mesh; i++)
+    if (sols[i]) std::copy(sols[i], sols[i] + 4L * s->h_nnodes[i], u.begin() + 4L * s->h_node0[i]);
+  femx_status r = femx_vms_set_state(s, u.data(), nullptr);
+  if (r != FEMX_OK) return r;
+  std::vector<VmsConst> kt(s->n_mesh);
+  FEMX_HIP(hipMemcpyAsync(kt.data(), s->kt.p, s->n_mesh * sizeof(VmsConst), hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  for (auto &k : kt) k.t = 0.0;
+  FEMX_HIP(hipMemcpyAsync(s->kt.p, kt.data(), s->n_mesh * sizeof(VmsConst), hipMemcpyHostToDevice, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  std::fill(s->h_time.begin(), s->h_time.end(), 0.0);
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_batch_step(femx_vms_batch *b, int nsteps, int use_graph) {
+  if (!b || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  femx_vms *s = b->s;
+  int k = 0;
+  while (k < nsteps) {
+    int done;
+    if (use_graph && s->proj_pending && nsteps - k >= 2) {
+      const int par = s->cur;
+      if (!s->graph[par]) {
+        femx_status r = batch_capture(s, par);
+        if (r != FEMX_OK) return r;
+      }
+      FEMX_HIP(hipGraphLaunch(s->graph[par], s->st));
+      s->iter += 2L * (s->p.tot_dual_steps + 1);
+      done = 2;
+    } else {
+      femx_status r = batch_step_once(s);
+      if (r != FEMX_OK) return r;
+      done = 1;
+    }
+    for (int j = 0; j < done; j++)
+      for (int i = 0; i < s->n_mesh; i++) s->h_time[i] += s->h_dt[i];
+    k += done;
+  }
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_batch_get_state(femx_vms_batch *b, int i, double *sol, double *time) {
+  if (!b || i < 0 || i >= b->s->n_mesh) return set_error("bad argument"), FEMX_ERR_ARG;
+  femx_vms *s = b->s;
+  femx_status r = flush_projection(s);
+  if (r != FEMX_OK) return r;
+  if (sol)
+    FEMX_HIP(hipMemcpyAsync(sol, s->Q[s->cur].p + s->h_node0[i], s->h_nnodes[i] * sizeof(double4),
+                            hipMemcpyDeviceToHost, s->st));
+  FEMX_HIP(hipStreamSynchronize(s->st));
+  if (time) *time = s->h_time[i];
+  return check_err(s);
+}
+
+extern "C" femx_status femx_vms_batch_info(femx_vms_batch *b, long long *info) {
+  if (!b || !info) return set_error("null argument"), FEMX_ERR_ARG;
+  const Mesh &m = *b->s->mesh;
+  info[0] = b->s->n_mesh;
+  info[1] = m.N;
+  info[2] = m.E;
+  info[3] = m.blk.nb;
+  info[4] = m.blk.n_sh;
+  info[5] = m.blk.B;
+  info[6] = b->s->xcd;
+  info[7] = m.blk.max_local;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_batch_sync(femx_vms_batch *b) {
+  if (!b) return set_error("null argument"), FEMX_ERR_ARG;
+  FEMX_HIP(hipStreamSynchronize(b->s->st));
+  return check_err(b->s);
+}
+
+extern "C" void *femx_vms_batch_stream(femx_vms_batch *b) { return b ? (void *)b->s->st : nullptr; }
+
+// Candidates (element-block size x block order) timed on the given meshes:
+// 2 warm-up steps, then `steps` graph-replayed steps between HIP events.
+extern "C" femx_status femx_vms_batch_autotune(int n, const femx_mesh_desc *meshes, const femx_bc_desc *bcs,
+                                               const femx_vms_params *params, int steps, int *best_block_elems,
+                                               int *best_xcd, double *ms_per_step /*[6]*/) {
+  if (!best_block_elems || !best_xcd || steps < 2) return set_error("bad argument"), FEMX_ERR_ARG;
+  static const int cand_b[3] = {256, 224, 192};
+  double best = 1e300;
+  for (int ib = 0; ib < 3; ib++)
+    for (int x = 1; x >= 0; x--) {
+      femx_vms_batch *b = nullptr;
+      femx_status r = femx_vms_batch_create(n, meshes, bcs, params, cand_b[ib], x, &b);
+      if (r != FEMX_OK) return r;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      float ms = 0.f;
+      if ((r = femx_vms_batch_set_state(b, nullptr)) == FEMX_OK && (r = femx_vms_batch_step(b, 3, 1)) == FEMX_OK &&
+          hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        (void)hipEventRecord(e0, b->s->st);
+        r = femx_vms_batch_step(b, steps & ~1, 1);
+        (void)hipEventRecord(e1, b->s->st);
+        if (r == FEMX_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+      }
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (r == FEMX_OK) r = femx_vms_batch_sync(b);
+      femx_vms_batch_destroy(b);
+      if (r != FEMX_OK) return r;
+      const double per = ms / (steps & ~1);
+      if (ms_per_step) ms_per_step[ib * 2 + (1 - x)] = per;
+      if (per < best) {
+        best = per;
+        *best_block_elems = cand_b[ib];
+        *best_xcd = x;
+      }
+    }
+  return FEMX_OK;
 }

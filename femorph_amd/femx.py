@@ -49,6 +49,17 @@ class SUPGParams(C.Structure):
                 ("exit_condition", C.c_double), ("precision", C.c_int)]
 
 
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("steps", C.c_int), ("mesh", MeshDesc), ("bc", BCDesc), ("vms", VMSParams),
+                ("supg", SUPGParams)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("kind", C.c_int), ("status", C.c_int), ("steps", C.c_int),
+                ("steps_done", C.c_int), ("elements", C.c_longlong), ("nodes", C.c_longlong), ("bytes", C.c_longlong),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int), ("total_ms", C.c_double),
                 ("bytes_per_launch", C.c_double)]
@@ -72,6 +83,26 @@ SIGNATURES = [
     ("femx_vms_get_diagnostics", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     ("femx_vms_sync", C.c_int, [C.c_void_p]),
     ("femx_vms_stream", C.c_void_p, [C.c_void_p]),
+    ("femx_vms_batch_create", C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(BCDesc), C.POINTER(VMSParams),
+                                        C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("femx_vms_batch_destroy", C.c_int, [C.c_void_p]),
+    ("femx_vms_batch_set_state", C.c_int, [C.c_void_p, C.POINTER(_dp)]),
+    ("femx_vms_batch_step", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("femx_vms_batch_get_state", C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    ("femx_vms_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
+    ("femx_vms_batch_sync", C.c_int, [C.c_void_p]),
+    ("femx_vms_batch_stream", C.c_void_p, [C.c_void_p]),
+    ("femx_vms_batch_autotune", C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(BCDesc), C.POINTER(VMSParams),
+                                          C.c_int, _ip, _ip, _dp]),
+    ("femx_supg_step_async", C.c_int, [C.c_void_p, C.c_int]),
+    ("femx_supg_finish", C.c_int, [C.c_void_p, _ip]),
+    ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("femx_server_submit", C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)]),
+    ("femx_server_poll", C.c_int, [C.c_void_p, C.POINTER(RequestStats), C.c_int]),
+    ("femx_server_result", C.c_int, [C.c_void_p, C.c_longlong, _dp, C.c_longlong]),
+    ("femx_server_drain", C.c_int, [C.c_void_p, C.c_double]),
+    ("femx_server_info", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
+    ("femx_server_destroy", C.c_int, [C.c_void_p]),
     ("femx_supg_create", C.c_int, [C.c_void_p, C.POINTER(BCDesc), C.POINTER(SUPGParams), C.POINTER(C.c_void_p)]),
     ("femx_supg_destroy", C.c_int, [C.c_void_p]),
     ("femx_supg_set_state", C.c_int, [C.c_void_p, _dp]),
@@ -104,10 +135,20 @@ def load() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m femorph_amd.build`")
         L = C.CDLL(LIB_PATH)
+        missing = []
         for name, res, args in SIGNATURES:
-            f = getattr(L, name)
+            try:
+                f = getattr(L, name)
+            except AttributeError:
+                missing.append(name)
+                continue
             f.restype = res
             f.argtypes = args
+        # the in-tree library must export every entry point; an explicitly
+        # chosen other build (FEMX_LIB, e.g. an older one being compared)
+        # may lack newer ones, which then fail when called
+        if missing and "FEMX_LIB" not in os.environ:
+            raise RuntimeError(f"{LIB_PATH} lacks {missing}: rebuild it with `python -m femorph_amd.build`")
         _LIB = L
     return _LIB
 
@@ -169,6 +210,90 @@ def _kernel_times(fn, h):
     n = fn(h, buf, 32)
     return {buf[i].name.decode(): dict(launches=buf[i].launches, total_ms=buf[i].total_ms,
                                        bytes_per_launch=buf[i].bytes_per_launch) for i in range(min(n, 32))}
+
+
+class _BatchDescs:
+    """C descriptors of a list of (mesh, bc, params dict) kept alive together."""
+
+    def __init__(self, items):
+        n = len(items)
+        self.keep = []
+        self.meshes = (MeshDesc * n)()
+        self.bcs = (BCDesc * n)()
+        self.params = (VMSParams * n)()
+        for i, (mesh, bc, prm) in enumerate(items):
+            if mesh.npe != 4:
+                raise ValueError("batches hold tet4 meshes")
+            c = np.ascontiguousarray(mesh.coords, np.float64)
+            e = np.ascontiguousarray(mesh.conn, np.int32)
+            k = _BCKeep(bc)
+            self.keep += [c, e, k]
+            self.meshes[i] = MeshDesc(mesh.n_nodes, mesh.n_elems, 4, _p(c), _p(e, _ip))
+            self.bcs[i] = k.desc
+            dt = prm["dt"]
+            self.params[i] = VMSParams(prm["rho"], prm["mu"], prm["inv_eps"], prm["c1"], prm["c2"], dt,
+                                       prm.get("dual_step", dt), prm.get("tot_dual_steps", 0),
+                                       prm.get("precision", FEMX_FP64))
+
+
+def vms_batch_autotune(items, steps=6):
+    """Time block sizes {256, 224, 192} x xcd order {1, 0} on `items`;
+    returns (block_elems, xcd_order, ms_per_step[6])."""
+    d = _BatchDescs(items)
+    b, x, ms = np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(6)
+    _check(load().femx_vms_batch_autotune(len(items), d.meshes, d.bcs, d.params, steps, _p(b, _ip), _p(x, _ip),
+                                          _p(ms)))
+    return int(b[0]), int(x[0]), ms
+
+
+class VMSBatch:
+    """Many VMS tet4 solves advanced together (femx_vms_batch_*).
+
+    items: list of (Mesh, BoundaryConditions, dict(rho, mu, inv_eps, c1, c2,
+    dt[, dual_step, tot_dual_steps, precision]))."""
+
+    def __init__(self, items, block_elems=0, xcd_order=1):
+        self.sizes = [m.n_nodes for m, _, _ in items]
+        d = _BatchDescs(items)
+        h = C.c_void_p()
+        _check(load().femx_vms_batch_create(len(items), d.meshes, d.bcs, d.params, block_elems, xcd_order,
+                                            C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.femx_vms_batch_destroy(self._h)
+            self._h = None
+
+    def set_state(self, sols=None):
+        if sols is None:
+            _check(load().femx_vms_batch_set_state(self._h, None))
+            return
+        arrs = [None if s is None else np.ascontiguousarray(s, np.float64).reshape(n, 4)
+                for s, n in zip(sols, self.sizes)]
+        ptrs = (_dp * len(arrs))(*[_p(a) if a is not None else None for a in arrs])
+        self._keep = arrs
+        _check(load().femx_vms_batch_set_state(self._h, ptrs))
+
+    def step(self, n=1, graph=True):
+        _check(load().femx_vms_batch_step(self._h, n, int(graph)))
+
+    def get_state(self, i):
+        sol, t = np.zeros((self.sizes[i], 4)), np.zeros(1)
+        _check(load().femx_vms_batch_get_state(self._h, i, _p(sol), _p(t)))
+        return sol, float(t[0])
+
+    def info(self):
+        a = (C.c_longlong * 8)()
+        _check(load().femx_vms_batch_info(self._h, a))
+        return dict(zip(("meshes", "nodes", "elements", "blocks", "shared_nodes", "block_elems", "xcd_order",
+                         "max_local"), list(a)))
+
+    def sync(self):
+        _check(load().femx_vms_batch_sync(self._h))
+
+    def stream(self):
+        return load().femx_vms_batch_stream(self._h)
 
 
 class VMSSolver:
@@ -313,3 +438,69 @@ class SUPGSolver:
 
     def kernel_times(self):
         return _kernel_times(load().femx_supg_kernel_times, self._h)
+
+
+class Server:
+    """Solve server (femx_server_*): admission by device footprint, resident
+    state, every resident request advanced on its own stream."""
+
+    STATS = ("id", "kind", "status", "steps", "steps_done", "elements", "nodes", "bytes",
+             "t_submit", "t_admit", "t_start", "t_done")
+
+    def __init__(self, budget_bytes=64 << 30, max_active=32, prep_threads=4, chunk_steps=25):
+        h = C.c_void_p()
+        _check(load().femx_server_create(int(budget_bytes), max_active, prep_threads, chunk_steps, C.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.femx_server_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _submit(self, kind, mesh, bc, steps, vms=None, supg=None):
+        c = np.ascontiguousarray(mesh.coords, np.float64)
+        e = np.ascontiguousarray(mesh.conn, np.int32)
+        k = _BCKeep(bc)
+        r = Request(kind, steps, MeshDesc(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(c), _p(e, _ip)), k.desc,
+                    vms if vms is not None else VMSParams(), supg if supg is not None else SUPGParams())
+        rid = C.c_longlong()
+        _check(load().femx_server_submit(self._h, C.byref(r), C.byref(rid)))  # arrays are copied here
+        self._n[rid.value] = (mesh.n_nodes, 4 if kind == 0 else 3)
+        return rid.value
+
+    def submit_vms(self, mesh, bc, steps, rho, mu, inv_eps, c1, c2, dt, dual_step=None, tot_dual_steps=0,
+                   precision=FEMX_MIXED):
+        p = VMSParams(rho, mu, inv_eps, c1, c2, dt, dt if dual_step is None else dual_step, tot_dual_steps, precision)
+        return self._submit(0, mesh, bc, steps, vms=p)
+
+    def submit_supg(self, mesh, bc, steps, rho, mu, alpha, cI, dt, exopts=(0, 1, 1, 1, 1, 1), fixed_dofs="NONE",
+                    exit_condition=1.0e5, precision=FEMX_MIXED):
+        e = [int(bool(x)) for x in exopts]
+        if mesh.npe == 4:
+            e[1] = 0  # B-hat is hexa8-only in the reference
+        p = SUPGParams(rho, mu, alpha, cI, dt, e[0], e[1], e[2], e[3], e[4], e[5], FIXED_DOFS[fixed_dofs],
+                       exit_condition, precision)
+        return self._submit(1, mesh, bc, steps, supg=p)
+
+    def poll(self, max_entries=4096):
+        buf = (RequestStats * max_entries)()
+        n = load().femx_server_poll(self._h, buf, max_entries)
+        return [{f: getattr(buf[i], f) for f in self.STATS} for i in range(n)]
+
+    def result(self, rid):
+        """Final state of a finished request: VMS [N, 4], SUPG [N, 3]."""
+        nn, dofs = self._n.pop(rid)
+        out = np.zeros((nn, dofs))
+        _check(load().femx_server_result(self._h, rid, _p(out), nn * dofs))
+        return out
+
+    def drain(self, timeout_s=600.0):
+        _check(load().femx_server_drain(self._h, timeout_s))
+
+    def info(self):
+        a = (C.c_longlong * 6)()
+        _check(load().femx_server_info(self._h, a))
+        return dict(zip(("queued", "resident", "reserved", "peak_reserved", "budget", "max_active"), list(a)))

@@ -141,6 +141,41 @@ femx_status femx_vms_sync(femx_vms *s);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 void *femx_vms_stream(femx_vms *s);
 
+/* ------------------------------------------------------- VMS batches */
+/* Many independent VMS tet4 solves advanced together (the reference runs
+ * one femVMSExplicitFluidSolver::solve per model; a batch runs the same time
+ * loop, src/femVMSExplicitFluidSolver.cpp:1025-1235, for n models at once).
+ * Each mesh keeps its own BCs, parameters and time; precision and
+ * tot_dual_steps are common.  One launch of the element-block kernel and
+ * one of the fix-up advance every mesh by one dual iteration; steps are
+ * replayed from a captured hipGraph.  Results are bitwise those of n
+ * separate femx_vms handles built with the same block size.
+ * block_elems: elements per block (1..256, 0 = 256); xcd_order: 1 = blocks
+ * mapped to XCDs in contiguous ranges, 0 = dispatch order. */
+typedef struct femx_vms_batch femx_vms_batch;
+femx_status femx_vms_batch_create(int n, const femx_mesh_desc *meshes, const femx_bc_desc *bcs,
+                                  const femx_vms_params *params, int block_elems, int xcd_order,
+                                  femx_vms_batch **out);
+femx_status femx_vms_batch_destroy(femx_vms_batch *b);
+/* sols[i] = [N_i][4] initial state of mesh i before its t = 0 BCs (NULL
+ * array or NULL entry: zeros, as the reference starts). */
+femx_status femx_vms_batch_set_state(femx_vms_batch *b, const double *const *sols);
+/* nsteps time steps of every mesh; use_graph = 1 replays a captured graph
+ * of two steps (no host work per step). */
+femx_status femx_vms_batch_step(femx_vms_batch *b, int nsteps, int use_graph);
+femx_status femx_vms_batch_get_state(femx_vms_batch *b, int i, double *sol /*[N_i][4]*/, double *time);
+/* info[0..7] = meshes, nodes, elements, blocks, shared nodes, block size,
+ * xcd_order, max local nodes per block. */
+femx_status femx_vms_batch_info(femx_vms_batch *b, long long *info);
+femx_status femx_vms_batch_sync(femx_vms_batch *b);
+void *femx_vms_batch_stream(femx_vms_batch *b);
+/* Autotuner: times block sizes {256, 224, 192} x xcd_order {1, 0} on the
+ * given meshes (a representative sample of a batch) and returns the
+ * fastest; ms_per_step[6] (may be NULL) in that order. */
+femx_status femx_vms_batch_autotune(int n, const femx_mesh_desc *meshes, const femx_bc_desc *bcs,
+                                    const femx_vms_params *params, int steps, int *best_block_elems,
+                                    int *best_xcd, double *ms_per_step);
+
 /* ----------------------------------------------------------------- SUPG */
 /* vmsProps[0..3] = rho, mu, alpha, cI (src/femSUPGExplicitFluidSolver.cpp:835-838),
  * exOpts[0..5] = euler, B-hat, K1..K4 (:797-802), glob_fixed_dofs
@@ -178,6 +213,55 @@ femx_status femx_supg_get_log(femx_supg *s, double *cols19);
 femx_status femx_supg_set_diagnostics(femx_supg *s, int enable);
 femx_status femx_supg_sync(femx_supg *s);
 void *femx_supg_stream(femx_supg *s);
+
+/* Asynchronous stepping: femx_supg_step = femx_supg_step_async +
+ * femx_supg_finish.  step_async enqueues nsteps without waiting (steps
+ * after one whose max |u| exceeds EXITCOND do not run on the device);
+ * finish waits for them and returns FEMX_EXIT_CONDITION / *steps_done
+ * exactly as femx_supg_step would have for the same steps. */
+femx_status femx_supg_step_async(femx_supg *s, int nsteps);
+femx_status femx_supg_finish(femx_supg *s, int *steps_done);
+
+/* -------------------------------------------------------------- server */
+/* Keeps many solves resident on one GPU and admits new ones while others
+ * run (the reference runs one solve per process, src/main.cpp:788-805).
+ * Requests wait in arrival order until their device footprint fits the
+ * budget and one of max_active slots is free; prep threads then upload the
+ * mesh and create the handle; one GPU thread advances every resident
+ * request chunk_steps at a time on its own stream (their work overlaps on
+ * the device) and retires finished ones.  A request's result is bitwise
+ * the result of running it alone. */
+typedef enum { FEMX_REQ_VMS = 0, FEMX_REQ_SUPG = 1 } femx_request_kind_t;
+typedef struct {
+  femx_request_kind_t kind;
+  int steps;                 /* time steps to run from the zero initial state */
+  femx_mesh_desc mesh;       /* arrays are copied at submit */
+  femx_bc_desc bc;           /* copied at submit */
+  femx_vms_params vms;       /* kind == FEMX_REQ_VMS */
+  femx_supg_params supg;     /* kind == FEMX_REQ_SUPG */
+} femx_request;
+typedef struct {
+  long long id;
+  int kind, status;          /* status: FEMX_OK, FEMX_EXIT_CONDITION or an error */
+  int steps, steps_done;
+  long long elements, nodes, bytes;        /* bytes: the admission footprint */
+  double t_submit, t_admit, t_start, t_done;  /* seconds on the server's steady clock */
+} femx_request_stats;
+typedef struct femx_server femx_server;
+femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int chunk_steps,
+                               femx_server **out);
+femx_status femx_server_submit(femx_server *s, const femx_request *r, long long *id);
+/* Stats of requests finished since the last poll; returns their number. */
+int femx_server_poll(femx_server *s, femx_request_stats *out, int max_entries);
+/* Final state of a finished request (VMS [N][4], SUPG [N][3]; len = that
+ * many doubles); the server then forgets the request. */
+femx_status femx_server_result(femx_server *s, long long id, double *sol, long long len);
+/* Waits until every submitted request has finished. */
+femx_status femx_server_drain(femx_server *s, double timeout_s);
+/* info[0..5] = queued, resident, reserved bytes, peak reserved bytes,
+ * budget, max_active. */
+femx_status femx_server_info(femx_server *s, long long *info);
+femx_status femx_server_destroy(femx_server *s);
 
 /* --------------------------------------------------------- measurement */
 /* Per-kernel HIP-event timing on the handle's stream (off by default).

@@ -1,0 +1,92 @@
+"""The solve server (femx_server_*): requests admitted by device footprint
+while others run, every resident request advanced on its own stream.
+
+Tolerance: bitwise.  A served request runs the arithmetic of its own
+handle, so its result must equal the same request run alone (and the
+single-handle path is pinned to the oracle by the VMS / SUPG tests)."""
+import numpy as np
+import pytest
+
+from femorph_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _vms_req(n, seed, steps):
+    m = meshgen.tet_cube(n, 0.15, seed)
+    bc = meshgen.lid_cavity_bc(m)
+    re = 50.0 + 40.0 * seed
+    return ("vms", m, bc, steps, dict(rho=1.0, mu=1.0 / re, inv_eps=1.0, c1=4.0, c2=2.0,
+                                      dt=0.1 * meshgen.min_tet_h(m)))
+
+
+def _supg_req(n, seed, steps, exit_condition=1.0e5):
+    m = meshgen.hex_cube(n, 0.2, seed)
+    bc = meshgen.lid_cavity_bc(m)
+    return ("supg", m, bc, steps, dict(rho=1.0, mu=0.0025, alpha=1.0, cI=4.0, dt=0.1 / n,
+                                       exit_condition=exit_condition))
+
+
+def _submit(srv, req):
+    kind, m, bc, steps, p = req
+    if kind == "vms":
+        return srv.submit_vms(m, bc, steps, **p)
+    return srv.submit_supg(m, bc, steps, **p)
+
+
+def _alone(femx, req):
+    """The request on its own handle, no server."""
+    kind, m, bc, steps, p = req
+    dm = femx.DeviceMesh(m)
+    if kind == "vms":
+        g = femx.VMSSolver(dm, bc, p["rho"], p["mu"], p["inv_eps"], p["c1"], p["c2"], p["dt"],
+                           precision=femx.FEMX_MIXED)
+        g.set_state(np.zeros((m.n_nodes, 4)))
+        g.step(steps)
+        return g.get_state()[0]
+    g = femx.SUPGSolver(dm, bc, p["rho"], p["mu"], p["alpha"], p["cI"], p["dt"],
+                        exit_condition=p["exit_condition"], precision=femx.FEMX_MIXED)
+    g.set_state(np.zeros((m.n_nodes, 3)))
+    g.step(steps)
+    return g.get_state()["sol"]
+
+
+def test_results_alone_equal_in_the_mix(gpu):
+    probe_v = _vms_req(6, 1, 37)
+    probe_s = _supg_req(5, 2, 23)
+    mix = [_vms_req(4, 3, 40), _supg_req(4, 4, 31), _vms_req(8, 5, 25), probe_v, _supg_req(6, 6, 19),
+           _vms_req(5, 7, 52), probe_s, _vms_req(3, 8, 60), _supg_req(3, 9, 44)]
+    print("server mix of", len(mix), "requests", flush=True)
+    srv = gpu.Server(budget_bytes=24 << 20, max_active=4, prep_threads=2, chunk_steps=7)
+    ids = [_submit(srv, r) for r in mix]
+    srv.drain(300.0)
+    stats = {s["id"]: s for s in srv.poll()}
+    assert sorted(stats) == sorted(ids) and all(s["status"] == 0 for s in stats.values())
+    info = srv.info()
+    assert info["resident"] == 0 and info["queued"] == 0
+    # admission: the footprint budget and the slot count were respected
+    assert info["peak_reserved"] <= info["budget"]
+    ev = sorted([(s["t_admit"], 1) for s in stats.values()] + [(s["t_done"], -1) for s in stats.values()])
+    live = peak = 0
+    for _, d in ev:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 4
+    results = {i: srv.result(i) for i in ids}
+    srv.close()
+    for req, i in ((probe_v, ids[3]), (probe_s, ids[6])):
+        assert stats[i]["steps_done"] == req[3]
+        assert np.array_equal(results[i], _alone(gpu, req)), req[0]
+
+
+def test_exit_condition_request(gpu):
+    srv = gpu.Server(max_active=2, chunk_steps=4)
+    bad = _supg_req(4, 1, 30, exit_condition=0.5)  # the lid speed 1.0 exceeds it at step 1
+    good = _vms_req(4, 2, 12)
+    ib, ig = _submit(srv, bad), _submit(srv, good)
+    srv.drain(120.0)
+    st = {s["id"]: s for s in srv.poll()}
+    assert st[ib]["status"] == gpu.EXIT_CONDITION and st[ib]["steps_done"] == 1
+    assert st[ig]["status"] == 0 and st[ig]["steps_done"] == 12
+    assert np.array_equal(srv.result(ig), _alone(gpu, good))
+    srv.close()

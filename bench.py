@@ -311,6 +311,132 @@ def workload_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def workload_c2(args):
+    """SUPG hexa8 on a jittered 128^3 cube (2,097,152 hexa, 2,146,689 nodes),
+    Re 400 lid cavity, AB2 + B-hat + K1..K4, mixed and FP64."""
+    n = args.n if args.n != 96 else 128
+    m = meshgen.hex_cube(n, 0.2, args.seed)
+    bc = meshgen.lid_cavity_bc(m)
+    dm = femx.DeviceMesh(m)
+    p = dict(rho=1.0, mu=1.0 / 400.0, alpha=1.0, cI=4.0, dt=0.1 / n)
+    ic = meshgen.random_velocity(m, 1e-3, args.seed)[:, :3]
+    E, N = m.n_elems, m.n_nodes
+    res = {}
+    for name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64)):
+        g = femx.SUPGSolver(dm, bc, precision=prec, **p)
+        g.set_state(ic)
+        g.step(args.warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        g.sync()
+        secs = time.perf_counter() - t0
+        g.kernel_timing(True)
+        g.step(args.steps)
+        g.sync()
+        kt = g.kernel_times()
+        g.kernel_timing(False)
+        res[name] = (secs, kt)
+        print(f"c2: {name} {secs / args.steps * 1e3:.4f} ms/step", file=sys.stderr, flush=True)
+    secs, kt = res[args.precision]
+    dom_name = max(kt, key=lambda k: kt[k]["total_ms"])
+    dom = kt[dom_name]
+    dom_ms = dom["total_ms"] / max(dom["launches"], 1)
+    achieved = dom["bytes_per_launch"] / (dom_ms * 1e-3) / 1e9
+    other = "fp64" if args.precision == "mixed" else "mixed"
+    out = {"metric": "element-steps/sec (SUPG hexa8)", "value": round(E * args.steps / secs, 1),
+           "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32-element/f64-state" if args.precision == "mixed" else "f64",
+           "data": "synthetic (seeded jittered cube)",
+           "config": {"workload": f"C2: SUPG hexa8 {n}^3 jitter0.2 lid cavity Re400, AB2+Bhat+K1-K4 "
+                                  f"({E} hexa, {N} nodes)", "elements": E, "nodes": N},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                        "kernel_ms": round(dom_ms, 4), "kernel_bytes": dom["bytes_per_launch"],
+                        "note": "VALU-bound kernel (about 15k FP64/FP32 operations per element-step); "
+                                "the HBM fraction is reported for reference",
+                        "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4))
+                                    for k, v in kt.items()}},
+           other: {"value": round(E * args.steps / res[other][0], 1),
+                   "ms_per_step": round(res[other][0] / args.steps * 1e3, 4)}}
+    print(json.dumps(out), flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def workload_accuracy(args):
+    """Error of the FP32-element / FP64-state path after `steps` steps:
+    C1 against the FP64 oracle itself, C3 (VMS) and C2 (SUPG) against the
+    plain FP64 GPU path (pinned to the oracle at <= 1e-12 per step)."""
+    steps = args.steps
+    out = {"metric": "relative L2 error of the mixed path after N steps", "steps": steps, "configs": {}}
+
+    def note(msg):
+        print(f"accuracy: {msg}", file=sys.stderr, flush=True)
+
+    # C1: 20^3 x 6 lid cavity, Re 100, vs the oracle
+    from oracle.oracle import VMSOracle  # the checker
+    m = meshgen.tet_cube(20)
+    bc = meshgen.lid_cavity_bc(m)
+    p = dict(rho=1.0, mu=0.01, inv_eps=1.0, c1=4.0, c2=2.0, dt=0.1 * meshgen.min_tet_h(m))
+    ic = meshgen.random_velocity(m, 1e-3, args.seed)
+    o = VMSOracle(m, bc, p["rho"], p["mu"], p["inv_eps"], p["c1"], p["c2"], p["dt"])
+    o.set_ic(ic)
+    dm = femx.DeviceMesh(m)
+    res = {}
+    for prec in (femx.FEMX_MIXED, femx.FEMX_FP64):
+        g = femx.VMSSolver(dm, bc, precision=prec, **p)
+        g.set_state(ic)
+        res[prec] = g
+    done = 0
+    for chunk in (100,) * (steps // 100) + ((steps % 100,) if steps % 100 else ()):
+        o.step(chunk)
+        for g in res.values():
+            g.step(chunk)
+        done += chunk
+        note(f"C1 {done} steps")
+    ref = o.get("sol_n")
+    c1 = {}
+    for prec, g in res.items():
+        sol = g.get_state()[0]
+        c1["mixed" if prec == femx.FEMX_MIXED else "fp64"] = {
+            "rel_l2_velocity": _rel(sol[:, :3], ref[:, :3]), "rel_l2_pressure": _rel(sol[:, 3], ref[:, 3])}
+    out["configs"]["C1 VMS tet4 20^3x6 cavity Re100 (vs FP64 oracle)"] = c1
+    del res, dm, o
+    # C3: 96^3 x 6 channel Re 200, mixed vs FP64 GPU path
+    m, bc, params, ic = make_workload(96)
+    dm = femx.DeviceMesh(m)
+    gm = femx.VMSSolver(dm, bc, precision=femx.FEMX_MIXED, **params)
+    gd = femx.VMSSolver(dm, bc, precision=femx.FEMX_FP64, **params)
+    for g in (gm, gd):
+        g.set_state(ic)
+        g.step(steps)
+    a, b = gm.get_state()[0], gd.get_state()[0]
+    out["configs"]["C3 VMS tet4 96^3x6 channel Re200 (vs FP64 GPU path)"] = {
+        "rel_l2_velocity": _rel(a[:, :3], b[:, :3]), "rel_l2_pressure": _rel(a[:, 3], b[:, 3])}
+    note("C3 done")
+    del gm, gd, dm
+    # C2: SUPG hexa8 128^3 jittered lid cavity Re 400, AB2 + B-hat + K1..K4
+    mh = meshgen.hex_cube(128, 0.2, args.seed)
+    bch = meshgen.lid_cavity_bc(mh)
+    dmh = femx.DeviceMesh(mh)
+    ph = dict(rho=1.0, mu=1.0 / 400.0, alpha=1.0, cI=4.0, dt=0.1 / 128)
+    ich = meshgen.random_velocity(mh, 1e-3, args.seed)[:, :3]
+    sm = femx.SUPGSolver(dmh, bch, precision=femx.FEMX_MIXED, **ph)
+    sd = femx.SUPGSolver(dmh, bch, precision=femx.FEMX_FP64, **ph)
+    for g in (sm, sd):
+        g.set_state(ich)
+        g.step(steps)
+    a, b = sm.get_state(), sd.get_state()
+    out["configs"]["C2 SUPG hexa8 128^3 jitter0.2 cavity Re400 (vs FP64 GPU path)"] = {
+        "rel_l2_velocity": _rel(a["sol"], b["sol"]), "rel_l2_pressure": _rel(a["pres"], b["pres"])}
+    note("C2 done")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,7 +448,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
-    ap.add_argument("--workload", choices=["c3", "c4", "serve"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy"], default="c3",
                     help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request stream")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
@@ -349,6 +475,14 @@ def main():
         return
     if args.workload == "serve":
         workload_serve(args)
+        return
+    if args.workload == "c2":
+        workload_c2(args)
+        return
+    if args.workload == "accuracy":
+        if args.steps == 50:
+            args.steps = 1000
+        workload_accuracy(args)
         return
 
     m, bc, params, ic = make_workload(args.n)

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemx.so")
 
-SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip"]
+SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip", "femx_ffd.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -38,7 +38,7 @@ def hipcc() -> str:
 HOST = os.path.join(CSRC, "host")
 MODEL_LIB = os.path.join(HERE, "libfemx_model.so")
 DRIVER = os.path.join(HERE, "femx_explicit")
-HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp"]
+HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp", "femPointGrid.cpp"]
 GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-result"]
 
 

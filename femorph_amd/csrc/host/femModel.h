@@ -133,4 +133,6 @@ class femModel {
   void setDirichletBC(femDoubleMat &sol);
   // src/femModel.cpp:1294-1530 (ASCII VTK legacy, '%e')
   void ExportToVTKLegacy(std::string fileName, const bool write_message);
+  // src/femModel.cpp:2844-3185 (nodes, tri3/tet4/tet10 cells, point results)
+  void ReadFromVTKLegacy(std::string fileName);
 };

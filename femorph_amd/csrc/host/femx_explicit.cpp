@@ -5,6 +5,8 @@
 //   femx_explicit [-y] [--vms] [--mixed] [model.txt]
 //   femx_explicit --write-vtk out.vtk [model.txt]    (read + export, no solve)
 //   femx_explicit --dump-model out.txt [model.txt]   (read + dump the parsed fields)
+//   femx_explicit -j -f commandfile.txt [-o out.vtk] (free-form deformation, runFFD
+//                                                     src/main.cpp:742-781)
 //
 // Default: the SUPG solver (what solveExplicitNS runs) on ./model.txt in
 // FP64; --vms runs femVMSExplicitFluidSolver; --mixed selects FP32 element
@@ -14,6 +16,7 @@
 #include <string>
 
 #include "femExplicitFluidSolvers.h"
+#include "femPointGrid.h"
 #include "femx.h"
 
 namespace {
@@ -57,11 +60,45 @@ void dump_model(const femModel *m, const char *path) {
 
 }  // namespace
 
+// runFFD (src/main.cpp:742-781): read the command file and the VTK mesh,
+// morph with every box in turn (each overwrites the node displacements),
+// write bernsteinGrid_<k>.vtk per box and the morphed model.
+int run_ffd(const std::string &cmd, const std::string &out) {
+  femFFDOptions opts;
+  opts.readFromFile(cmd);
+  femModel *model = new femModel();
+  model->ReadFromVTKLegacy(opts.inputFileName);
+  std::vector<femPointGrid *> grids;
+  for (auto &g : opts.ffdData)
+    grids.push_back(new femPointGrid(g.minPoint, g.totPoints, g.gridAxis_S, g.gridAxis_T, g.gridAxis_U, g.dispNodes,
+                                     g.dispVals));
+  for (size_t k = 0; k < grids.size(); k++) {
+    grids[k]->morphModel(model);
+    grids[k]->ExportToVTKLegacy("bernsteinGrid_" + std::to_string(k + 1) + ".vtk");
+  }
+  model->ExportToVTKLegacy(out, false);
+  for (auto *g : grids) delete g;
+  delete model;
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  std::string input = "model.txt", write_vtk, dump;
-  bool vms = false, mixed = false;
+  std::string input = "model.txt", write_vtk, dump, ffd_cmd, output = "morphed.vtk";
+  bool vms = false, mixed = false, ffd = false;
   for (int i = 1; i < argc; i++) {
     if (!strcmp(argv[i], "-y")) continue;
+    if (!strcmp(argv[i], "-j")) {
+      ffd = true;
+      continue;
+    }
+    if (!strcmp(argv[i], "-f") && i + 1 < argc) {
+      ffd_cmd = argv[++i];
+      continue;
+    }
+    if (!strcmp(argv[i], "-o") && i + 1 < argc) {
+      output = argv[++i];
+      continue;
+    }
     if (!strcmp(argv[i], "--vms")) {
       vms = true;
     } else if (!strcmp(argv[i], "--mixed")) {
@@ -81,6 +118,11 @@ int main(int argc, char **argv) {
     }
   }
   try {
+    if (ffd) {
+      run_ffd(ffd_cmd.empty() ? input : ffd_cmd, output);
+      printf("\nProgram Completed.\n\n");
+      return 0;
+    }
     femModel *model = new femModel();
     model->ReadFromFEMTextFile(input);
     if (!write_vtk.empty() || !dump.empty()) {

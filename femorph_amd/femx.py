@@ -60,6 +60,12 @@ class RequestStats(C.Structure):
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
 
 
+class FFDBox(C.Structure):
+    _fields_ = [("min_point", C.c_double * 3), ("tot_points", C.c_int * 3), ("axis_s", C.c_double * 3),
+                ("axis_t", C.c_double * 3), ("axis_u", C.c_double * 3), ("n_disp", C.c_int),
+                ("disp_node", _ip), ("disp_val", _dp)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int), ("total_ms", C.c_double),
                 ("bytes_per_launch", C.c_double)]
@@ -94,6 +100,8 @@ SIGNATURES = [
     ("femx_vms_batch_stream", C.c_void_p, [C.c_void_p]),
     ("femx_vms_batch_autotune", C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(BCDesc), C.POINTER(VMSParams),
                                           C.c_int, _ip, _ip, _dp]),
+    ("femx_ffd_control_points", C.c_int, [C.POINTER(FFDBox), _dp, _dp]),
+    ("femx_ffd_morph", C.c_int, [C.POINTER(FFDBox), C.c_int, _dp, _dp]),
     ("femx_supg_step_async", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_finish", C.c_int, [C.c_void_p, _ip]),
     ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
@@ -504,3 +512,31 @@ class Server:
         a = (C.c_longlong * 6)()
         _check(load().femx_server_info(self._h, a))
         return dict(zip(("queued", "resident", "reserved", "peak_reserved", "budget", "max_active"), list(a)))
+
+
+def _ffd_box(min_point, n, S, T, U, disp_node, disp_val):
+    dn = np.ascontiguousarray(disp_node, np.int32)
+    dv = np.ascontiguousarray(disp_val, np.float64).reshape(-1, 3)
+    b = FFDBox((C.c_double * 3)(*min_point), (C.c_int * 3)(*n), (C.c_double * 3)(*S), (C.c_double * 3)(*T),
+               (C.c_double * 3)(*U), len(dn), _p(dn, _ip), _p(dv))
+    return b, (dn, dv)
+
+
+def ffd_morph(min_point, n, S, T, U, disp_node, disp_val, coords):
+    """Free-form deformation of points by one Bernstein box on the GPU
+    (femPointGrid::morphPoints); returns [n, 3] displacements (box
+    coordinates for points outside the box, as the reference stores)."""
+    b, keep = _ffd_box(min_point, n, S, T, U, disp_node, disp_val)
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    out = np.zeros_like(c)
+    _check(load().femx_ffd_morph(C.byref(b), len(c), _p(c), _p(out)))
+    return out
+
+
+def ffd_control_points(min_point, n, S, T, U, disp_node, disp_val):
+    """Control-point coordinates and displacements of one box (host only)."""
+    b, keep = _ffd_box(min_point, n, S, T, U, disp_node, disp_val)
+    tot = int(n[0]) * int(n[1]) * int(n[2])
+    c, d = np.zeros((tot, 3)), np.zeros((tot, 3))
+    _check(load().femx_ffd_control_points(C.byref(b), _p(c), _p(d)))
+    return c, d

@@ -263,6 +263,29 @@ femx_status femx_server_drain(femx_server *s, double timeout_s);
 femx_status femx_server_info(femx_server *s, long long *info);
 femx_status femx_server_destroy(femx_server *s);
 
+/* ------------------------------------------------ free-form deformation */
+/* One Bernstein box of femPointGrid (src/femPointGrid.cpp:4-114), as the
+ * FFD command file gives it (femFFDOptions::readFromFile,
+ * src/femFFDOptions.cpp): origin, control points per axis (2..13), the
+ * three box axes, and displacements of chosen control points along the
+ * normalised axes. */
+typedef struct {
+  double min_point[3];
+  int tot_points[3];
+  double axis_s[3], axis_t[3], axis_u[3];
+  int n_disp;
+  const int *disp_node;     /* [n_disp] control point index, x fastest */
+  const double *disp_val;   /* [n_disp][3] along the S, T, U axes */
+} femx_ffd_box;
+/* Host-only: control-point coordinates [n][3] and displacements [n][3]
+ * (n = product of tot_points; either pointer may be NULL). */
+femx_status femx_ffd_control_points(const femx_ffd_box *box, double *coords, double *disps);
+/* femPointGrid::morphPoints of the box coordinates of n points
+ * (getBoxCoords + morphPoints, :170-254) on the GPU: disp [n][3] = sum of
+ * B_i(s) B_j(t) B_k(u) d_ijk inside the box; points outside get their box
+ * coordinates (s, t, u), as the reference stores them. */
+femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, const double *coords, double *disp);
+
 /* --------------------------------------------------------- measurement */
 /* Per-kernel HIP-event timing on the handle's stream (off by default).
  * While enabled, every kernel launch is bracketed by events; femx_*_kernel_times

@@ -1083,3 +1083,77 @@ int ox_supg_get(const oxSUPG *s, int which, double *out) {
   }
   return -1;
 }
+
+/* ===================================================================== */
+/* FFD (femPointGrid)                                                     */
+/* ===================================================================== */
+
+/* factorial / binomialCoeff / eval1DBernsteinPolynomials, :121-133 (int
+ * factorial, exact up to 12!). */
+static int ox_factorial(int n) { return (n == 1 || n == 0) ? 1 : ox_factorial(n - 1) * n; }
+static double ox_binomial(int n, int k) { return ox_factorial(n) / (ox_factorial(n - k) * ox_factorial(k)); }
+static double ox_bernstein(double x, int i, int n) { return ox_binomial(n, i) * pow(x, i) * pow(1.0 - x, n - i); }
+
+/* femUtils Do3DExternalProduct / Do3DInternalProduct (src/femUtils.cpp:57-66) */
+static void ox_cross(const double *v1, const double *v2, double *v3) {
+  v3[0] = v1[1] * v2[2] - v2[1] * v1[2];
+  v3[1] = v1[2] * v2[0] - v2[2] * v1[0];
+  v3[2] = v1[0] * v2[1] - v2[0] * v1[1];
+}
+static double ox_dot(const double *v1, const double *v2) { return v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2]; }
+
+int ox_ffd_morph(const double min_point[3], const int n[3], const double S[3], const double T[3],
+                 const double U[3], int n_disp, const int *disp_node, const double *disp_val, int npts,
+                 const double *coords, double *out) {
+  const int tot = n[0] * n[1] * n[2];
+  double *disps = zalloc(3L * tot);
+  double nS[3], nT[3], nU[3];
+  double mS = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  double mT = sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+  double mU = sqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+  for (int k = 0; k < 3; k++) {
+    nS[k] = mS > 0.0 ? S[k] / mS : S[k];
+    nT[k] = mT > 0.0 ? T[k] / mT : T[k];
+    nU[k] = mU > 0.0 ? U[k] / mU : U[k];
+  }
+  for (int a = 0; a < n_disp; a++) {
+    int id = disp_node[a];
+    if (id < 0 || id >= tot) {
+      free(disps);
+      return -1;
+    }
+    for (int k = 0; k < 3; k++) disps[3 * id + k] += disp_val[3 * a] * nS[k];
+    for (int k = 0; k < 3; k++) disps[3 * id + k] += disp_val[3 * a + 1] * nT[k];
+    for (int k = 0; k < 3; k++) disps[3 * id + k] += disp_val[3 * a + 2] * nU[k];
+  }
+  double TU[3], SU[3], ST[3];
+  ox_cross(T, U, TU);
+  ox_cross(S, U, SU);
+  ox_cross(S, T, ST);
+  double TUS = ox_dot(TU, S), SUT = ox_dot(SU, T), STU = ox_dot(ST, U);
+  for (long p = 0; p < npts; p++) {
+    double aux[3] = {coords[3 * p] - min_point[0], coords[3 * p + 1] - min_point[1], coords[3 * p + 2] - min_point[2]};
+    double c[3] = {ox_dot(TU, aux) / TUS, ox_dot(SU, aux) / SUT, ox_dot(ST, aux) / STU};
+    int inside = (c[0] >= 0.0) && (c[0] <= 1.0) && (c[1] >= 0.0) && (c[1] <= 1.0) && (c[2] >= 0.0) && (c[2] <= 1.0);
+    if (!inside) {
+      for (int k = 0; k < 3; k++) out[3 * p + k] = c[k];
+      continue;
+    }
+    for (int a = 0; a < 3; a++) {
+      double r = 0.0;
+      int count = 0;
+      for (int z = 0; z < n[2]; z++)
+        for (int y = 0; y < n[1]; y++)
+          for (int x = 0; x < n[0]; x++) {
+            double bx = ox_bernstein(c[0], x, n[0] - 1);
+            double by = ox_bernstein(c[1], y, n[1] - 1);
+            double bz = ox_bernstein(c[2], z, n[2] - 1);
+            r += bx * by * bz * disps[3 * count + a];
+            count++;
+          }
+      out[3 * p + a] = r;
+    }
+  }
+  free(disps);
+  return 0;
+}

@@ -128,6 +128,17 @@ void ox_supg_set_elem_range(oxSUPG *s, int e0, int e1);
  * 9 scalars [time, loop_time]. */
 int ox_supg_get(const oxSUPG *s, int which, double *out);
 
+/* ------------------------------------------------------------------ FFD */
+/* femPointGrid (src/femPointGrid.cpp): control-point displacements from the
+ * displaced points along the normalised box axes (:81-113), box
+ * coordinates (getBoxCoords :207-254) and morphPoints (:170-200) with the
+ * per-term Bernstein evaluation of evalTrilinearBernsteinPolynomials
+ * (:135-158).  n[3] = control points per axis; disp_val [n_disp][3];
+ * out [npts][3].  Returns 0, or -1 for a bad control-point index. */
+int ox_ffd_morph(const double min_point[3], const int n[3], const double S[3], const double T[3],
+                 const double U[3], int n_disp, const int *disp_node, const double *disp_val, int npts,
+                 const double *coords, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -213,6 +213,22 @@ def rule_second_order(npe):
     return c.reshape(8, 4)[:n], w[:n]
 
 
+def ffd_morph(min_point, n, S, T, U, disp_node, disp_val, coords):
+    """femPointGrid morph of points (box coordinates, Bernstein sum)."""
+    L = lib()
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    out = np.zeros_like(c)
+    dn = np.ascontiguousarray(disp_node, np.int32)
+    dv = np.ascontiguousarray(disp_val, np.float64).reshape(-1, 3)
+    v = [np.ascontiguousarray(x, np.float64) for x in (min_point, S, T, U)]
+    nn = np.ascontiguousarray(n, np.int32)
+    r = L.ox_ffd_morph(_p(v[0]), _p(nn, _ip), _p(v[1]), _p(v[2]), _p(v[3]), len(dn), _p(dn, _ip), _p(dv), len(c),
+                       _p(c), _p(out))
+    if r != 0:
+        raise ValueError("control point index out of range")
+    return out
+
+
 def interp_table(t, rows):
     rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 4)
     out = np.zeros(3)

@@ -1,0 +1,70 @@
+"""Free-form deformation on the GPU (femx_ffd_morph) against the oracle's
+restatement of femPointGrid (pinned by closed forms in
+tests/test_oracle_ffd.py), and the -j driver end to end (runFFD,
+src/main.cpp:742-781).  Tolerance 1e-13 relative per point: the device and
+host pow differ in the last bits."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from femorph_amd import meshgen, modelfile
+from femorph_amd.build import DRIVER
+from oracle import oracle
+from tests.vtkio import read_vtk_legacy
+
+pytestmark = pytest.mark.gpu
+
+BOXES = [
+    ((0.0, 0.0, 0.0), (3, 3, 3), (1.0, 0, 0), (0, 1.0, 0), (0, 0, 1.0)),
+    ((0.1, -0.2, 0.05), (5, 4, 6), (0.9, 0.1, 0), (0.05, 1.1, 0.1), (0, -0.1, 0.8)),
+    ((0.0, 0.0, 0.0), (13, 2, 7), (1.2, 0, 0), (0, 0.7, 0), (0, 0, 1.3)),
+]
+
+
+@pytest.mark.parametrize("box", BOXES)
+def test_morph_matches_oracle(gpu, box):
+    X0, n, S, T, U = box
+    tot = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(sum(n))
+    nodes = rng.choice(tot, size=min(tot, 9), replace=False)
+    vals = rng.uniform(-0.2, 0.2, (len(nodes), 3))
+    pts = rng.uniform(-0.3, 1.4, (4000, 3))  # inside and outside the box
+    got = gpu.ffd_morph(X0, n, S, T, U, nodes, vals, pts)
+    ref = oracle.ffd_morph(X0, n, S, T, U, nodes, vals, pts)
+    err = np.abs(got - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)
+    assert err.max() <= 1e-13, err.max()
+    c, d = gpu.ffd_control_points(X0, n, S, T, U, nodes, vals)
+    assert c.shape == (tot, 3) and np.abs(d).sum() > 0
+
+
+def test_ffd_driver_end_to_end(gpu, tmp_path):
+    """mesh.vtk (tet4) + command file -> morphed VTK whose DXYZ is the
+    morph of every node, and one bernsteinGrid_<k>.vtk per box; the model
+    keeps the last box's displacements (each box overwrites them)."""
+    d = str(tmp_path)
+    mesh = meshgen.tet_cube(4, 0.1, 3)
+    modelfile.write_model_txt(os.path.join(d, "model.txt"), mesh, meshgen.lid_cavity_bc(mesh))
+    r = subprocess.run([DRIVER, "--write-vtk", "mesh.vtk", "model.txt"], cwd=d, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    box2 = ((0.2, 0.2, 0.2), (3, 4, 3), (0.6, 0, 0), (0, 0.6, 0), (0, 0, 0.6), [(5, 0.1, 0.0, -0.05), (14, 0.0, 0.2, 0.0)])
+    lines = ["mesh.vtk", "2", "0.0 0.0 0.0", "3 3 3", "1.0 0.0 0.0", "0.0 1.0 0.0", "0.0 0.0 1.0", "1",
+             "13 0.0 0.1 0.0",
+             " ".join(map(str, box2[0])), " ".join(map(str, box2[1])), " ".join(map(str, box2[2])),
+             " ".join(map(str, box2[3])), " ".join(map(str, box2[4])), str(len(box2[5]))]
+    lines += [" ".join(map(str, v)) for v in box2[5]]
+    open(os.path.join(d, "cmd.txt"), "w").write("\n".join(lines) + "\n")
+    print("femx_explicit -j", flush=True)
+    r = subprocess.run([DRIVER, "-j", "-f", "cmd.txt", "-o", "out.vtk"], cwd=d, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert os.path.exists(os.path.join(d, "bernsteinGrid_1.vtk")) and os.path.exists(os.path.join(d, "bernsteinGrid_2.vtk"))
+    v = read_vtk_legacy(os.path.join(d, "out.vtk"))
+    X0, n, S, T, U, disp = box2
+    ref = oracle.ffd_morph(X0, n, S, T, U, [q[0] for q in disp], [q[1:] for q in disp], mesh.coords)
+    got = v["point_data"]["DXYZ"]
+    tol = 1.5e-6 * np.abs(ref) + 1e-12
+    assert (np.abs(got - ref) <= tol).all()
+    g = read_vtk_legacy(os.path.join(d, "bernsteinGrid_2.vtk"))
+    assert g["points"].shape == (36, 3) and (g["cell_types"] == 12).all()

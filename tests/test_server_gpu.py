@@ -57,7 +57,7 @@ def test_results_alone_equal_in_the_mix(gpu):
     mix = [_vms_req(4, 3, 40), _supg_req(4, 4, 31), _vms_req(8, 5, 25), probe_v, _supg_req(6, 6, 19),
            _vms_req(5, 7, 52), probe_s, _vms_req(3, 8, 60), _supg_req(3, 9, 44)]
     print("server mix of", len(mix), "requests", flush=True)
-    srv = gpu.Server(budget_bytes=24 << 20, max_active=4, prep_threads=2, chunk_steps=7)
+    srv = gpu.Server(budget_bytes=4 << 20, max_active=4, prep_threads=2, chunk_steps=7)
     ids = [_submit(srv, r) for r in mix]
     srv.drain(300.0)
     stats = {s["id"]: s for s in srv.poll()}

@@ -290,13 +290,19 @@ def workload_serve(args):
             srv.result(s_["id"])
         srv.close()
         lat = np.array([s_["t_done"] - s_["t_submit"] for s_ in st])
+        wait = np.array([s_["t_admit"] - s_["t_submit"] for s_ in st])
+        prep = np.array([s_["t_start"] - s_["t_admit"] for s_ in st])
+        run = np.array([s_["t_done"] - s_["t_start"] for s_ in st])
         span = max(s_["t_done"] for s_ in st) - min(s_["t_submit"] for s_ in st)
         done = [s_ for s_ in st if s_["status"] in (0, femx.EXIT_CONDITION)]
         es = sum(s_["elements"] * s_["steps_done"] for s_ in done)
         table[str(int(rate))] = {"requests": len(st), "ok": len(done), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
                                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
                                  "req_per_s": round(len(st) / span, 2), "element_steps_per_s": round(es / span, 1),
-                                 "offered_req_per_s": round(len(st) / float(arr[-1]), 2)}
+                                 "offered_req_per_s": round(len(st) / float(arr[-1]), 2),
+                                 "mean_ms": {"queue": round(float(wait.mean()) * 1e3, 2),
+                                             "prep": round(float(prep.mean()) * 1e3, 2),
+                                             "resident": round(float(run.mean()) * 1e3, 2)}}
         print(json.dumps({"rate": rate, **table[str(int(rate))]}), file=sys.stderr, flush=True)
     top = table[str(int(rates[-1]))]
     out = {"metric": "p50/p99 solve latency (served VMS tet4 / SUPG hexa8 requests)",
@@ -306,7 +312,8 @@ def workload_serve(args):
            "config": {"workload": f"C5: Poisson arrivals at {args.rates} req/s, {args.requests} requests per rate, "
                                   "tet4 (VMS) or hexa8 (SUPG) cavities, n log-uniform [8,64], steps U[50,500]",
                       "budget_gb": args.budget_gb, "max_active": args.max_active, "chunk_steps": args.chunk,
-                      "prep_threads": args.prep_threads},
+                      "prep_threads": args.prep_threads,
+                      "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")},
            "latency": table}
     print(json.dumps(out), flush=True)
 
@@ -457,7 +464,12 @@ def main():
     ap.add_argument("--max-active", type=int, default=64)
     ap.add_argument("--prep-threads", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=25)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="serve: hardware queues of the process (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()
+    if args.workload == "serve" and args.hw_queues:
+        # before the first HIP call: resident requests each run on their own stream
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

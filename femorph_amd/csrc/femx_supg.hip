@@ -515,29 +515,64 @@ __global__ __launch_bounds__(256) void supg_node_diag(int N, const double4 *U, d
 }
 
 // Reduce the per-block rows into this step's ring row (one workgroup).
-__global__ __launch_bounds__(256) void supg_diag_reduce(const double *eblk, int ne_blk, const double *nblk, int nn_blk,
-                                                        int with_elem, double *row, const int *halt) {
+// Reduction of the per-block rows to the step's log row, in two stages: 32
+// workgroups each fold a strided share of the rows, all columns at once in
+// registers, into one partial row; one wave folds the partial rows with all
+// of its loads in flight together.
+constexpr int DIAG_PARTS = 32;
+__global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int ne_blk, const double *nblk,
+                                                         int nn_blk, int with_elem, double *part, const int *halt) {
   if (halted(halt)) return;
-  __shared__ double red[256];
+  double m[D_COLS];
+#pragma unroll
+  for (int c = 0; c < D_COLS; c++) m[c] = d_identity(c);
+  const int stride = gridDim.x * blockDim.x;
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nn_blk; b += stride) {
+#pragma unroll
+    for (int c = 0; c < D_COLS; c++)
+      if (c < 6 || c == D_UMOD) {
+        const double x = nblk[(long)b * D_COLS + c];
+        m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
+      }
+  }
+  if (with_elem)
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ne_blk; b += stride) {
+#pragma unroll
+      for (int c = 0; c < D_COLS; c++)
+        if (!(c < 6 || c == D_UMOD)) {
+          const double x = eblk[(long)b * D_COLS + c];
+          m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
+        }
+    }
+  __shared__ double red[4][D_COLS];
+#pragma unroll
   for (int c = 0; c < D_COLS; c++) {
-    const bool node_col = c < 6 || c == D_UMOD;
-    const double *src = node_col ? nblk : eblk;
-    const int nb = node_col ? nn_blk : (with_elem ? ne_blk : 0);
-    const bool mn = d_is_min(c);
-    double m = d_identity(c);
-    for (int b = threadIdx.x; b < nb; b += 256) {
-      const double x = src[(long)b * D_COLS + c];
-      m = mn ? fmin(m, x) : fmax(m, x);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(m[c], off);
+      m[c] = d_is_min(c) ? fmin(m[c], o) : fmax(m[c], o);
     }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] = mn ? fmin(red[threadIdx.x], red[threadIdx.x + w])
-                                                      : fmax(red[threadIdx.x], red[threadIdx.x + w]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) row[c] = red[0];
-    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = m[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < D_COLS) {
+    const int c = threadIdx.x;
+    double v = red[0][c];
+    for (int w = 1; w < 4; w++) v = d_is_min(c) ? fmin(v, red[w][c]) : fmax(v, red[w][c]);
+    part[(long)blockIdx.x * D_COLS + c] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np, double *row, const int *halt) {
+  if (halted(halt)) return;
+  if (threadIdx.x < D_COLS) {
+    const int c = threadIdx.x;
+    double x[DIAG_PARTS];
+#pragma unroll
+    for (int q = 0; q < DIAG_PARTS; q++) x[q] = q < np ? part[(long)q * D_COLS + c] : d_identity(c);
+    double v = d_identity(c);
+#pragma unroll
+    for (int q = 0; q < DIAG_PARTS; q++) v = d_is_min(c) ? fmin(v, x[q]) : fmax(v, x[q]);
+    row[c] = v;
   }
 }
 
@@ -644,6 +679,7 @@ struct femx_supg {
   DevBuf<double4> xyzm, U[2], Up[2], Fprev, partials;
   DevBuf<double> lam, regime, pres, scratch;
   DevBuf<double> ring, eblk, nblk;  // [RING][D_COLS] reduced rows; per-block rows
+  DevBuf<double> dpart;             // [DIAG_PARTS][D_COLS] partial rows of the reduction
   DevBuf<int> halt;
   int cur = 0;
   long loop_time = 0;
@@ -899,6 +935,7 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
   if ((r = s->ring.alloc((size_t)RING * D_COLS)) != FEMX_OK || (r = s->halt.alloc(1)) != FEMX_OK) return fail(r);
   if ((r = s->eblk.alloc((size_t)m.blk.nb * D_COLS)) != FEMX_OK) return fail(r);
   if ((r = s->nblk.alloc((size_t)grid_for(m.N, 256) * D_COLS)) != FEMX_OK) return fail(r);
+  if ((r = s->dpart.alloc((size_t)DIAG_PARTS * D_COLS)) != FEMX_OK) return fail(r);
   if (hipMemsetAsync(s->ring.p, 0, (size_t)RING * D_COLS * sizeof(double), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   if (m.npe == 8)
     supg_mass<8><<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, m.conn.p, m.xyz.p, p->rho, s->xyzm.p);
@@ -970,8 +1007,9 @@ femx_status supg_enqueue(femx_supg *s, int nsteps) {
     s->press_pending = true;
     const int nnb = grid_for(s->mesh->N, 256);
     supg_node_diag<<<nnb, 256, 0, s->st>>>(s->mesh->N, s->U[s->cur].p, s->nblk.p, s->halt.p);
-    supg_diag_reduce<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0, row,
-                                           s->halt.p);
+    supg_diag_partial<<<DIAG_PARTS, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb,
+                                                     s->diag_enabled ? 1 : 0, s->dpart.p, s->halt.p);
+    supg_diag_final<<<1, 64, 0, s->st>>>(s->dpart.p, DIAG_PARTS, row, s->halt.p);
     FEMX_HIP(hipGetLastError());
     s->pend_n++;
   }

@@ -447,13 +447,19 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 #pragma unroll
       for (int i = 0; i < 3; i++) sdu[g][i] = *row_ptr(a.sdus, (g * 3 + i) * E, eoff);
   }
-  // the local node this thread owns (nn <= VN <= VB); its phase-3 data is
-  // re-read there (from L2) rather than held in registers across phase 2
+  // the local node this thread owns (nn <= VN <= VB); with FP32 element
+  // math its phase-3 data is re-read there (from L2) to keep phase 2 within
+  // 96 VGPRs, with FP64 it stays in registers
   const bool has_n = tid < nn;
-  int g_own = 0, rec_own = -1;
+  int g_own = 0, rec_own = -1, ps_keep = -1, s0_keep = 0, s1_keep = 0;
   if (has_n) {
     g_own = a.loc_node[n0 + tid];
     rec_own = a.node_rec[g_own];
+    if (!F32 && RES) {
+      ps_keep = a.loc_pslot[n0 + tid];
+      s0_keep = a.lcsr_off[n0 + tid];
+      s1_keep = a.lcsr_off[n0 + tid + 1];
+    }
   }
   // reference node of the FP32 relative values: the block's first local node
   double4 xr = make_double4(0, 0, 0, 0), pr = make_double4(0, 0, 0, 0), qr = make_double4(0, 0, 0, 0);
@@ -464,12 +470,18 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     if (RES) qr = a.Q[gr];
   }
   // 1. stage the block's nodes
+  double4 base_keep = make_double4(0, 0, 0, 0);
+  double im_keep = 0.0;
   if (has_n) {
     const int j = tid, g = g_own;
     const double4 x = a.xyzm[g];
     double4 p_own = make_double4(0, 0, 0, 0), q_own = make_double4(0, 0, 0, 0);
     if (PROJ || ACC) p_own = a.P[g];
     if (RES) q_own = a.Q[g];
+    if (!F32) {
+      base_keep = !ACC ? q_own : p_own;
+      im_keep = x.w;
+    }
     if (F32) {
       sfa[j] = make_float4((float)(x.x - xr.x), (float)(x.y - xr.y), (float)(x.z - xr.z), (float)(p_own.x - pr.x));
       sfb[j] = make_float4((float)(p_own.y - pr.y), (float)(p_own.z - pr.z), (float)(q_own.x - qr.x),
@@ -625,9 +637,10 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   double m2 = 0.0;
   if (has_n) {
     const int g = g_own;
-    const int ps_own = a.loc_pslot[n0 + tid];
+    const int ps_own = F32 ? a.loc_pslot[n0 + tid] : ps_keep;
     double R[4] = {0.0, 0.0, 0.0, 0.0};
-    const int s0 = a.lcsr_off[n0 + tid] - e0 * 4, s1 = a.lcsr_off[n0 + tid + 1] - e0 * 4;
+    const int s0 = (F32 ? a.lcsr_off[n0 + tid] : s0_keep) - e0 * 4;
+    const int s1 = (F32 ? a.lcsr_off[n0 + tid + 1] : s1_keep) - e0 * 4;
 #pragma unroll 4
     for (int s = s0; s < s1; s++) {
       const T4 v = sres[swz<T4>(s)];
@@ -638,9 +651,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     }
     if (ps_own < 0) {
       // base of update_dual_solution: post-BC at the first dual iteration, pre-BC after
-      const double4 base = !ACC ? a.Q[g] : a.P[g];
+      const double4 base = F32 ? (!ACC ? a.Q[g] : a.P[g]) : base_keep;
       double4 pre, post;
-      m2 = node_update(base, a.xyzm[g].w, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, F32 ? a.xyzm[g].w : im_keep, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -653,10 +666,11 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 struct FixArgs {
   int n_sh;
   const int *sh_node, *sh_off;
-  const double4 *partials, *xyzm, *Q, *P;
+  const double *sh_im;  // 1 / nd_mass of every shared node, contiguous
+  const int *sh_rec;    // its BC record or -1, contiguous
+  const double4 *partials, *Q, *P;
   double4 *Qn, *Pn;
   int first_dual;
-  const int *node_rec;
   const BCRecord *rec;
   const int *row_off;
   const double4 *rows;
@@ -683,7 +697,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     const double4 base = a.first_dual ? a.Q[g] : a.P[g];
     const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
-    m2 = node_update(base, a.xyzm[g].w, R, k, pre, post, a.node_rec[g], a.rec, a.row_off, a.rows, a.err);  // w = 1/m
+    m2 = node_update(base, a.sh_im[s], R, k, pre, post, a.sh_rec[s], a.rec, a.row_off, a.rows, a.err);
     a.Pn[g] = pre;
     a.Qn[g] = post;
   }
@@ -714,6 +728,17 @@ __global__ __launch_bounds__(256) void vms_mass(int N, const int *g_off, const i
   const double4 x = xyz[n];
   xyzm[n] = make_double4(x.x, x.y, x.z, 1.0 / acc);
   mass[n] = acc;
+}
+
+// Per shared node, its 1/m and BC record in one contiguous array each
+// (the fix-up then reads no node-indexed metadata).
+__global__ __launch_bounds__(256) void vms_shared_meta(int n_sh, const int *sh_node, const double4 *xyzm,
+                                                       const int *node_rec, double *sh_im, int *sh_rec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sh) return;
+  const int g = sh_node[s];
+  sh_im[s] = xyzm[g].w;
+  sh_rec[s] = node_rec[g];
 }
 
 // BCs of the initial state (solve() :883-889).
@@ -846,6 +871,8 @@ struct femx_vms {
   DevBuf<double> mass;    // nd_mass
   DevBuf<double4> P[2], Q[2];
   DevBuf<double4> partials;
+  DevBuf<double> sh_im;  // per shared node: 1 / nd_mass
+  DevBuf<int> sh_rec;    // per shared node: BC record or -1
   DevBuf<double> sdus64;
   DevBuf<float> sdus32;
   DevBuf<double> bmax;  // per-block maxima of the last iteration: block kernel, then fix-up
@@ -866,6 +893,7 @@ struct femx_vms {
   DevBuf<double> diag_iter;  // [tot_dual_steps + 1][3] of the last step
   // batch of meshes (femx_vms_batch_*): the mesh is the union of the batch
   int n_mesh = 1;
+  bool batch = false;  // constants and time per mesh from the device tables (also for n = 1)
   femx_mesh *owned = nullptr;
   DevBuf<VmsConst> kt;       // per mesh constants, .t = the mesh's time
   DevBuf<double> kdt;        // per mesh dt
@@ -940,7 +968,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.perm = b.d_perm.p;
   a.diag_res = s->diag ? s->diag_res.p : nullptr;
   a.diag_sdu = s->diag ? s->diag_sdu.p : nullptr;
-  const bool batch = s->n_mesh > 1;
+  const bool batch = s->batch;
   a.blk_mesh = batch ? s->blk_mesh.p : nullptr;
   a.kt = batch ? s->kt.p : nullptr;
   a.mesh_node0 = batch ? s->mesh_node0.p : nullptr;
@@ -980,21 +1008,21 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.n_sh = b.n_sh;
   a.sh_node = b.d_sh_node.p;
   a.sh_off = b.d_sh_off.p;
+  a.sh_im = s->sh_im.p;
+  a.sh_rec = s->sh_rec.p;
   a.partials = s->partials.p;
-  a.xyzm = s->xyzm.p;
   a.Q = s->Q[s->cur].p;
   a.P = s->P[s->cur].p;
   a.Qn = s->Q[s->cur ^ 1].p;
   a.Pn = s->P[s->cur ^ 1].p;
   a.first_dual = first_dual ? 1 : 0;
-  a.node_rec = s->bc.node_rec.p;
   a.rec = s->bc.rec.p;
   a.row_off = s->bc.row_off.p;
   a.rows = s->bc.rows.p;
   a.err = s->bc.err.p;
   a.bmax = s->bmax.p + b.nb;
-  a.node_mesh = s->n_mesh > 1 ? s->node_mesh.p : nullptr;
-  a.kt = s->n_mesh > 1 ? s->kt.p : nullptr;
+  a.node_mesh = s->batch ? s->node_mesh.p : nullptr;
+  a.kt = s->batch ? s->kt.p : nullptr;
   a.k = vms_const(s, t);
   vms_fixup<<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
@@ -1069,6 +1097,11 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   }
   vms_mass<<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, reinterpret_cast<const int4 *>(m.conn.p),
                                                   m.xyz.p, s->xyzm.p, s->mass.p);
+  if (m.blk.n_sh > 0) {
+    if ((r = s->sh_im.alloc(m.blk.n_sh)) != FEMX_OK || (r = s->sh_rec.alloc(m.blk.n_sh)) != FEMX_OK) return fail(r);
+    vms_shared_meta<<<grid_for(m.blk.n_sh, 256), 256, 0, s->st>>>(m.blk.n_sh, m.blk.d_sh_node.p, s->xyzm.p,
+                                                                  s->bc.node_rec.p, s->sh_im.p, s->sh_rec.p);
+  }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   // min element length (:947-952), for the CFL print
   double mh_ = 1.79769313486231570815e+308;
@@ -1412,6 +1445,7 @@ extern "C" femx_status femx_vms_batch_create(int n, const femx_mesh_desc *meshes
   }
   s->owned = um;
   s->n_mesh = n;
+  s->batch = true;
   s->xcd = xcd_order ? 1 : 0;
   s->h_node0 = node0;
   s->h_nnodes = nnodes;

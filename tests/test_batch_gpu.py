@@ -93,3 +93,29 @@ def test_autotune_picks_a_candidate(gpu):
     print("autotune", be, xcd, ms, flush=True)
     assert be in (256, 224, 192) and xcd in (0, 1)
     assert (ms > 0).all() and ms.min() == ms[[256, 224, 192].index(be) * 2 + (1 - xcd)]
+
+
+def test_one_mesh_batch_time_dependent_bc(gpu):
+    """A batch of one mesh reads its time from the device table like any
+    batch: with a lid velocity ramped in time, it must equal the
+    single-mesh solver (which passes the host time) bitwise."""
+    from femorph_amd.mesh import BoundaryConditions
+    m = meshgen.tet_cube(4, 0.1, 7)
+    base = meshgen.lid_cavity_bc(m)
+    dt = 0.1 * meshgen.min_tet_h(m)
+    n_lid = len(base.vel_node)
+    rows = np.zeros((2 * n_lid, 4))
+    rows[1::2] = [1.0, 0.0, 0.0, 10 * dt]  # u = t / (10 dt) up to t = 10 dt
+    bc = BoundaryConditions(vel_node=base.vel_node, vel_dir=base.vel_dir,
+                            vel_row_off=np.arange(0, 2 * n_lid + 1, 2, dtype=np.int32), vel_rows=rows,
+                            dir_node=base.dir_node, dir_val=base.dir_val)
+    p = dict(rho=1.0, mu=0.01, inv_eps=1.0, c1=4.0, c2=2.0, dt=dt, tot_dual_steps=0, precision=0)
+    ic = meshgen.random_velocity(m, 1e-3, 5)
+    b = gpu.VMSBatch([(m, bc, p)])
+    b.set_state([ic])
+    b.step(6)
+    sol_b, t_b = b.get_state(0)
+    ref = _singles(gpu, [(m, bc, p)], [ic], 6)[0]
+    assert t_b == ref[1] and np.array_equal(sol_b, ref[0])
+    lid = sol_b[np.setdiff1d(bc.vel_node, bc.dir_node), 0]  # lid edges are walls (walls win)
+    assert len(lid) > 0 and np.allclose(lid, t_b / (10 * dt))  # the ramp at t = 6 dt

@@ -73,7 +73,11 @@ def test_vms_solve_files_and_prints(gpu, tmp_path):
             _close(v["point_data"]["pressure"], sol[:, 3])
             _close(v["cell_data"]["res_v"], o.get("el_u_res"))
             _close(v["cell_data"]["res_p"], o.get("el_p_res"))
-            _close(v["cell_data"]["subgrid_u"], o.get("el_u_sdus"))
+            # the orthogonal subscale has zero element mean in exact arithmetic, so
+            # subgrid_u is rounding noise of the Gauss-point values: bound it by them
+            scale = np.abs(o.get("sdus")).max()
+            assert np.abs(v["cell_data"]["subgrid_u"]).max() <= 1e-13 * scale
+            assert np.abs(o.get("el_u_sdus")).max() <= 1e-13 * scale
             assert not v["cell_data"]["subgrid_p"].any()
             assert [x[2] for x in v["order"]] == ["DXYZ", "velocity", "pressure", "dirBC", "neuBC", "normal",
                                                   "source", "res_v", "res_p", "subgrid_u", "subgrid_p"]

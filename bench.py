@@ -271,7 +271,7 @@ def workload_serve(args):
     table = {}
     for rate, arr, reqs in plans:
         srv = femx.Server(budget_bytes=args.budget_gb << 30, max_active=args.max_active, prep_threads=args.prep_threads,
-                          chunk_steps=args.chunk)
+                          chunk_steps=args.chunk, gpu_threads=args.gpu_threads)
         ids = {}
         t0 = time.perf_counter()
         for t_arr, (kind, n, steps, re) in zip(arr, reqs):
@@ -312,7 +312,7 @@ def workload_serve(args):
            "config": {"workload": f"C5: Poisson arrivals at {args.rates} req/s, {args.requests} requests per rate, "
                                   "tet4 (VMS) or hexa8 (SUPG) cavities, n log-uniform [8,64], steps U[50,500]",
                       "budget_gb": args.budget_gb, "max_active": args.max_active, "chunk_steps": args.chunk,
-                      "prep_threads": args.prep_threads,
+                      "prep_threads": args.prep_threads, "gpu_threads": args.gpu_threads,
                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")},
            "latency": table}
     print(json.dumps(out), flush=True)
@@ -464,6 +464,7 @@ def main():
     ap.add_argument("--max-active", type=int, default=64)
     ap.add_argument("--prep-threads", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=25)
+    ap.add_argument("--gpu-threads", type=int, default=4, help="serve: host threads issuing the resident requests' work")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="serve: hardware queues of the process (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()

@@ -10,11 +10,13 @@
 //                 footprint fits the budget and a slot is free, then build
 //                 the element blocks, upload the mesh, create the solver
 //                 handle (its own HIP stream) and set the zero initial state;
-//   GPU thread    advances every active request by `chunk` steps at a time
-//                 on the request's own stream (so the work of different
-//                 requests overlaps on the device), polls completion with
-//                 an event per request, reads finished states back and frees
-//                 their handles and budget.
+//   GPU threads   each owns a share of the resident requests and advances
+//                 every one of them by `chunk` steps at a time on the
+//                 request's own stream (so the work of different requests
+//                 overlaps on the device and kernel launches are issued from
+//                 several host threads), polls completion with an event per
+//                 request, reads finished states back and frees their
+//                 handles and budget.
 //
 // Each request's arithmetic is that of its own handle, so a result does not
 // depend on what else is being served (bitwise).
@@ -112,7 +114,7 @@ struct femx_server {
   std::mutex mu;
   std::condition_variable cv;
   std::deque<std::shared_ptr<Request>> pending, ready;
-  std::vector<std::shared_ptr<Request>> active;
+  std::vector<std::vector<std::shared_ptr<Request>>> active;  // per GPU thread
   std::map<long long, std::shared_ptr<Request>> finished;
   std::vector<femx_request_stats> unpolled;
   size_t reserved = 0, peak_reserved = 0;
@@ -120,11 +122,10 @@ struct femx_server {
   long long next_id = 1;
   bool stop = false;
   std::string first_error;
-  std::vector<std::thread> preps;
-  std::thread gpu;
+  std::vector<std::thread> preps, gpus;
 
   void prep_loop();
-  void gpu_loop();
+  void gpu_loop(int t);
   void fail(const std::shared_ptr<Request> &r, femx_status st);
 };
 
@@ -197,15 +198,17 @@ void femx_server::prep_loop() {
   }
 }
 
-void femx_server::gpu_loop() {
+void femx_server::gpu_loop(int t) {
   (void)hipSetDevice(device);
+  std::vector<std::shared_ptr<Request>> &active = this->active[t];
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(mu);
       cv.wait_for(lk, std::chrono::microseconds(active.empty() ? 2000 : 50),
                   [&] { return stop || !ready.empty() || !active.empty(); });
       if (stop) return;
-      while (!ready.empty()) {
+      // take one newly prepared request per pass, so the GPU threads share them
+      if (!ready.empty()) {
         active.push_back(ready.front());
         ready.pop_front();
       }
@@ -274,9 +277,10 @@ void femx_server::gpu_loop() {
   }
 }
 
-extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int chunk_steps,
-                                          femx_server **out) {
-  if (!out || max_active < 1 || prep_threads < 1 || chunk_steps < 1 || hbm_budget_bytes == 0)
+extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int gpu_threads,
+                                          int chunk_steps, femx_server **out) {
+  if (!out || max_active < 1 || prep_threads < 1 || gpu_threads < 1 || gpu_threads > 64 || chunk_steps < 1 ||
+      hbm_budget_bytes == 0)
     return set_error("bad argument"), FEMX_ERR_ARG;
   if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
   femx_server *s = new femx_server();
@@ -284,8 +288,9 @@ extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_activ
   s->max_active = max_active;
   s->chunk = chunk_steps;
   (void)hipGetDevice(&s->device);
+  s->active.resize(gpu_threads);
   for (int i = 0; i < prep_threads; i++) s->preps.emplace_back([s] { s->prep_loop(); });
-  s->gpu = std::thread([s] { s->gpu_loop(); });
+  for (int i = 0; i < gpu_threads; i++) s->gpus.emplace_back([s, i] { s->gpu_loop(i); });
   *out = s;
   return FEMX_OK;
 }
@@ -380,6 +385,7 @@ extern "C" femx_status femx_server_info(femx_server *s, long long *info) {
   info[3] = (long long)s->peak_reserved;
   info[4] = (long long)s->budget;
   info[5] = s->max_active;
+  info[6] = (long long)s->gpus.size();
   return FEMX_OK;
 }
 
@@ -391,11 +397,12 @@ extern "C" femx_status femx_server_destroy(femx_server *s) {
   }
   s->cv.notify_all();
   for (auto &t : s->preps) t.join();
-  s->gpu.join();
-  for (auto &r : s->active) {
-    if (r->st) (void)hipStreamSynchronize(r->st);
-    r->release();
-  }
+  for (auto &t : s->gpus) t.join();
+  for (auto &list : s->active)
+    for (auto &r : list) {
+      if (r->st) (void)hipStreamSynchronize(r->st);
+      r->release();
+    }
   for (auto &r : s->ready) r->release();
   delete s;
   return FEMX_OK;

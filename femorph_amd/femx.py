@@ -104,7 +104,7 @@ SIGNATURES = [
     ("femx_ffd_morph", C.c_int, [C.POINTER(FFDBox), C.c_int, _dp, _dp]),
     ("femx_supg_step_async", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_finish", C.c_int, [C.c_void_p, _ip]),
-    ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("femx_server_submit", C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)]),
     ("femx_server_poll", C.c_int, [C.c_void_p, C.POINTER(RequestStats), C.c_int]),
     ("femx_server_result", C.c_int, [C.c_void_p, C.c_longlong, _dp, C.c_longlong]),
@@ -455,9 +455,10 @@ class Server:
     STATS = ("id", "kind", "status", "steps", "steps_done", "elements", "nodes", "bytes",
              "t_submit", "t_admit", "t_start", "t_done")
 
-    def __init__(self, budget_bytes=64 << 30, max_active=32, prep_threads=4, chunk_steps=25):
+    def __init__(self, budget_bytes=64 << 30, max_active=32, prep_threads=4, chunk_steps=25, gpu_threads=4):
         h = C.c_void_p()
-        _check(load().femx_server_create(int(budget_bytes), max_active, prep_threads, chunk_steps, C.byref(h)))
+        _check(load().femx_server_create(int(budget_bytes), max_active, prep_threads, gpu_threads, chunk_steps,
+                                         C.byref(h)))
         self._h = h
         self._n = {}
 
@@ -509,9 +510,10 @@ class Server:
         _check(load().femx_server_drain(self._h, timeout_s))
 
     def info(self):
-        a = (C.c_longlong * 6)()
+        a = (C.c_longlong * 7)()
         _check(load().femx_server_info(self._h, a))
-        return dict(zip(("queued", "resident", "reserved", "peak_reserved", "budget", "max_active"), list(a)))
+        return dict(zip(("queued", "resident", "reserved", "peak_reserved", "budget", "max_active", "gpu_threads"),
+                        list(a)))
 
 
 def _ffd_box(min_point, n, S, T, U, disp_node, disp_val):

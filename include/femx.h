@@ -227,9 +227,9 @@ femx_status femx_supg_finish(femx_supg *s, int *steps_done);
  * run (the reference runs one solve per process, src/main.cpp:788-805).
  * Requests wait in arrival order until their device footprint fits the
  * budget and one of max_active slots is free; prep threads then upload the
- * mesh and create the handle; one GPU thread advances every resident
- * request chunk_steps at a time on its own stream (their work overlaps on
- * the device) and retires finished ones.  A request's result is bitwise
+ * mesh and create the handle; gpu_threads host threads, each owning a share
+ * of the resident requests, advance every one chunk_steps at a time on its
+ * own stream (their work overlaps on the device) and retire finished ones.  A request's result is bitwise
  * the result of running it alone. */
 typedef enum { FEMX_REQ_VMS = 0, FEMX_REQ_SUPG = 1 } femx_request_kind_t;
 typedef struct {
@@ -248,8 +248,8 @@ typedef struct {
   double t_submit, t_admit, t_start, t_done;  /* seconds on the server's steady clock */
 } femx_request_stats;
 typedef struct femx_server femx_server;
-femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int chunk_steps,
-                               femx_server **out);
+femx_status femx_server_create(size_t hbm_budget_bytes, int max_active, int prep_threads, int gpu_threads,
+                               int chunk_steps, femx_server **out);
 femx_status femx_server_submit(femx_server *s, const femx_request *r, long long *id);
 /* Stats of requests finished since the last poll; returns their number. */
 int femx_server_poll(femx_server *s, femx_request_stats *out, int max_entries);
@@ -258,8 +258,8 @@ int femx_server_poll(femx_server *s, femx_request_stats *out, int max_entries);
 femx_status femx_server_result(femx_server *s, long long id, double *sol, long long len);
 /* Waits until every submitted request has finished. */
 femx_status femx_server_drain(femx_server *s, double timeout_s);
-/* info[0..5] = queued, resident, reserved bytes, peak reserved bytes,
- * budget, max_active. */
+/* info[0..6] = queued, resident, reserved bytes, peak reserved bytes,
+ * budget, max_active, GPU threads. */
 femx_status femx_server_info(femx_server *s, long long *info);
 femx_status femx_server_destroy(femx_server *s);
 

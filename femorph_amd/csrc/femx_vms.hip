@@ -819,10 +819,13 @@ __global__ __launch_bounds__(256) void vms_diag_centroid(int E, const double4 *p
 __global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nmax, int n_bmax, const double *bmax,
                                                        double *out) {
   double v[3] = {0.0, 0.0, 0.0};
+  // unrolled so that many independent loads are in flight per thread
+#pragma unroll 16
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     v[0] = nmax[2L * i] > v[0] ? nmax[2L * i] : v[0];
     v[1] = nmax[2L * i + 1] > v[1] ? nmax[2L * i + 1] : v[1];
   }
+#pragma unroll 16
   for (int i = threadIdx.x; i < n_bmax; i += blockDim.x) v[2] = bmax[i] > v[2] ? bmax[i] : v[2];
   __shared__ double wm[3][4];
 #pragma unroll

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemx.so")
 
-SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip", "femx_ffd.hip"]
+SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip", "femx_ffd.hip", "femx_ins.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -198,6 +198,7 @@ struct KernelTimer {
   struct Pending {
     int kind;
     hipEvent_t a, b;
+    double bytes;
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
@@ -205,7 +206,8 @@ struct KernelTimer {
     std::string name;
     int launches = 0;
     double ms = 0.0;
-    double bytes = 0.0;
+    double bytes = 0.0;      // bytes of the launch being recorded
+    double bytes_sum = 0.0;  // summed over the timed launches
   };
   std::vector<Acc> acc;
   int kind(const char *name, double bytes);
@@ -213,6 +215,7 @@ struct KernelTimer {
   void end(int k, hipStream_t st, hipEvent_t a);
   void resolve();
   int export_to(femx_kernel_time *out, int max_entries);
+  void reset();  // resolves pending launches, then zeroes the counts
   ~KernelTimer();
 };
 

@@ -293,7 +293,7 @@ void KernelTimer::end(int k, hipStream_t st, hipEvent_t a) {
     pool.pop_back();
   }
   (void)hipEventRecord(b, st);
-  pending.push_back({k, a, b});
+  pending.push_back({k, a, b, acc[k].bytes});
 }
 
 void KernelTimer::resolve() {
@@ -303,6 +303,7 @@ void KernelTimer::resolve() {
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
       acc[p.kind].ms += ms;
       acc[p.kind].launches += 1;
+      acc[p.kind].bytes_sum += p.bytes;
     }
     pool.push_back(p.a);
     pool.push_back(p.b);
@@ -318,9 +319,18 @@ int KernelTimer::export_to(femx_kernel_time *out, int max_entries) {
     std::strncpy(out[i].name, acc[i].name.c_str(), sizeof(out[i].name) - 1);
     out[i].launches = acc[i].launches;
     out[i].total_ms = acc[i].ms;
-    out[i].bytes_per_launch = acc[i].bytes;
+    out[i].bytes_per_launch = acc[i].launches ? acc[i].bytes_sum / acc[i].launches : acc[i].bytes;
   }
   return (int)acc.size();
+}
+
+void KernelTimer::reset() {
+  resolve();
+  for (auto &a : acc) {
+    a.launches = 0;
+    a.ms = 0.0;
+    a.bytes_sum = 0.0;
+  }
 }
 
 KernelTimer::~KernelTimer() {

@@ -27,6 +27,7 @@
 #include <cstring>
 
 #include "femx_common.h"
+#include "femx_elem.h"
 
 namespace femx {
 namespace {
@@ -36,29 +37,6 @@ namespace {
 template <int NPE>
 constexpr int threads_for() { return NPE == 4 ? 256 : 128; }
 constexpr int RING = 64; // per-step diagnostic rows kept on the device
-
-// Gauss points: shape functions and local derivatives of the reference's
-// irSecondOrder rule (femHexa8 / femTetra4), filled from the host.
-__constant__ double cN8[8][8], cdN8[8][8][3];
-__constant__ double cN4[4][4], cdN4[4][4][3];
-__constant__ double cW8, cW4;
-
-template <int NPE>
-struct Shape;
-template <>
-struct Shape<8> {
-  static constexpr int NGP = 8;
-  __device__ static double N(int g, int a) { return cN8[g][a]; }
-  __device__ static double dN(int g, int a, int k) { return cdN8[g][a][k]; }
-  __device__ static double W() { return cW8; }
-};
-template <>
-struct Shape<4> {
-  static constexpr int NGP = 4;
-  __device__ static double N(int g, int a) { return cN4[g][a]; }
-  __device__ static double dN(int g, int a, int k) { return cdN4[g][a][k]; }
-  __device__ static double W() { return cW4; }
-};
 
 // diagnostic row layout: 0..5 min/max u v w, 6,7 min/max tau, 8,9 min/max
 // lambda, 10 v-CFL max, 11 a-CFL max, 12 l-CFL max, 13,14 min/max E,
@@ -106,47 +84,6 @@ struct SupgConst {
   int fixed;        // glob_fixed_dofs code
   int k1, k2, k3, k4;
 };
-
-// femUtils::invert3x3Matrix (src/femUtils.cpp:716-770).
-__device__ __forceinline__ void inv3(const double m[3][3], double iv[3][3], double &det) {
-  det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
-        m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-  const double id = 1.0 / det;
-  iv[0][0] = (m[1][1] * m[2][2] - m[2][1] * m[1][2]) * id;
-  iv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
-  iv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
-  iv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
-  iv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
-  iv[1][2] = (m[1][0] * m[0][2] - m[0][0] * m[1][2]) * id;
-  iv[2][0] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]) * id;
-  iv[2][1] = (m[2][0] * m[0][1] - m[0][0] * m[2][1]) * id;
-  iv[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) * id;
-}
-
-// evalJacobianMatrixLocal + invert + evalGlobalShapeFunctionDerivative +
-// evalGeometricMatrix (src/femElement.cpp:104-354) at Gauss point g, with
-// coordinates relative to node 0 (the Jacobian is translation invariant).
-template <int NPE>
-__device__ __forceinline__ void kinematics(int g, const double dx[NPE][3], double DN[NPE][3], double &detJ,
-                                           double iJ[3][3]) {
-  double J[3][3];
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      double t = 0.0;
-#pragma unroll
-      for (int c = 1; c < NPE; c++) t += Shape<NPE>::dN(g, c, a) * dx[c][b];
-      J[a][b] = t;
-    }
-  inv3(J, iJ, detJ);
-#pragma unroll
-  for (int a = 0; a < NPE; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-      DN[a][b] = iJ[b][0] * Shape<NPE>::dN(g, a, 0) + iJ[b][1] * Shape<NPE>::dN(g, a, 1) +
-                 iJ[b][2] * Shape<NPE>::dN(g, a, 2);
-}
 
 template <typename T>
 struct SupgArgs {
@@ -696,49 +633,6 @@ struct femx_supg {
 
 namespace {
 
-bool g_tables_ready[64] = {false};
-
-femx_status upload_tables(int dev) {
-  if (dev >= 0 && dev < 64 && g_tables_ready[dev]) return FEMX_OK;
-  // femIntegrationRule irSecondOrder (src/femIntegrationRule.cpp:204-252, :320-329)
-  const double q = 1.0 / sqrt(3.0);
-  const double gp8[8][3] = {{-q, -q, q}, {q, -q, q}, {q, q, q}, {-q, q, q},
-                            {-q, -q, -q}, {q, -q, -q}, {q, q, -q}, {-q, q, -q}};
-  const double c1[8] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0};
-  const double c2[8] = {-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0};
-  const double c3[8] = {-1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0};
-  double N8[8][8], dN8[8][8][3];
-  for (int g = 0; g < 8; g++)
-    for (int a = 0; a < 8; a++) {
-      const double x = gp8[g][0], y = gp8[g][1], z = gp8[g][2];
-      N8[g][a] = (1.0 / 8.0) * (1.0 + x * c1[a]) * (1.0 + y * c2[a]) * (1.0 + z * c3[a]);  // femHexa8.cpp:21-23
-      dN8[g][a][0] = (1.0 / 8.0) * c1[a] * (1.0 + y * c2[a]) * (1.0 + z * c3[a]);        // femHexa8.cpp:36-38
-      dN8[g][a][1] = (1.0 / 8.0) * c2[a] * (1.0 + x * c1[a]) * (1.0 + z * c3[a]);
-      dN8[g][a][2] = (1.0 / 8.0) * c3[a] * (1.0 + x * c1[a]) * (1.0 + y * c2[a]);
-    }
-  const double A = 0.58541020, B = 0.13819660;
-  const double gp4[4][3] = {{A, B, B}, {B, A, B}, {B, B, A}, {B, B, B}};
-  double N4[4][4], dN4[4][4][3];
-  const double d4[4][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}, {-1.0, -1.0, -1.0}};
-  for (int g = 0; g < 4; g++) {
-    N4[g][0] = gp4[g][0];  // femTetra4.cpp:295-301
-    N4[g][1] = gp4[g][1];
-    N4[g][2] = gp4[g][2];
-    N4[g][3] = 1.0 - gp4[g][0] - gp4[g][1] - gp4[g][2];
-    for (int a = 0; a < 4; a++)
-      for (int k = 0; k < 3; k++) dN4[g][a][k] = d4[a][k];
-  }
-  const double w8 = 8.0 * (1.0 / 8.0), w4 = (1.0 / 6.0) * (1.0 / 4.0);
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cN8), N8, sizeof(N8)));
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cdN8), dN8, sizeof(dN8)));
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cN4), N4, sizeof(N4)));
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cdN4), dN4, sizeof(dN4)));
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW8), &w8, sizeof(w8)));
-  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW4), &w4, sizeof(w4)));
-  if (dev >= 0 && dev < 64) g_tables_ready[dev] = true;
-  return FEMX_OK;
-}
-
 SupgConst supg_const(const femx_supg *s, double t, bool euler_step) {
   SupgConst k;
   k.rho = s->p.rho;
@@ -1156,13 +1050,9 @@ extern "C" void *femx_supg_stream(femx_supg *s) { return s ? (void *)s->st : nul
 
 extern "C" femx_status femx_supg_kernel_timing(femx_supg *s, int enable) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
-  s->timer.resolve();
+  if (enable) s->timer.reset();
+  else s->timer.resolve();
   s->timer.enabled = enable != 0;
-  if (enable)
-    for (auto &a : s->timer.acc) {
-      a.launches = 0;
-      a.ms = 0.0;
-    }
   return FEMX_OK;
 }
 

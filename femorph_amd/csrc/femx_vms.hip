@@ -1281,13 +1281,9 @@ extern "C" void *femx_vms_stream(femx_vms *s) { return s ? (void *)s->st : nullp
 
 extern "C" femx_status femx_vms_kernel_timing(femx_vms *s, int enable) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
-  s->timer.resolve();
+  if (enable) s->timer.reset();
+  else s->timer.resolve();
   s->timer.enabled = enable != 0;
-  if (enable)
-    for (auto &a : s->timer.acc) {
-      a.launches = 0;
-      a.ms = 0.0;
-    }
   return FEMX_OK;
 }
 

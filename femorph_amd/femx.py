@@ -66,6 +66,15 @@ class FFDBox(C.Structure):
                 ("disp_node", _ip), ("disp_val", _dp)]
 
 
+class INSParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("alpha_m", C.c_double), ("alpha_f", C.c_double), ("gamma", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int)]
+
+
+class INSStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int), ("total_ms", C.c_double),
                 ("bytes_per_launch", C.c_double)]
@@ -125,6 +134,18 @@ SIGNATURES = [
     ("femx_vms_kernel_times", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int]),
     ("femx_supg_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_kernel_times", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int]),
+    ("femx_ins_create", C.c_int, [C.c_void_p, _dp, C.c_int, _ip, _dp, C.POINTER(INSParams), C.POINTER(C.c_void_p)]),
+    ("femx_ins_destroy", C.c_int, [C.c_void_p]),
+    ("femx_ins_set_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("femx_ins_prescribe_velocity", C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp]),
+    ("femx_ins_advdiff_solve", C.c_int, [C.c_void_p, C.POINTER(INSStats)]),
+    ("femx_ins_update", C.c_int, [C.c_void_p]),
+    ("femx_ins_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("femx_ins_get_system", C.c_int, [C.c_void_p, _ip, _ip, _dp, _dp, _dp, C.POINTER(C.c_longlong)]),
+    ("femx_ins_sync", C.c_int, [C.c_void_p]),
+    ("femx_ins_stream", C.c_void_p, [C.c_void_p]),
+    ("femx_ins_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("femx_ins_kernel_times", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int]),
 ]
 
 _LIB = None
@@ -446,6 +467,74 @@ class SUPGSolver:
 
     def kernel_times(self):
         return _kernel_times(load().femx_supg_kernel_times, self._h)
+
+
+class INSSolver:
+    """Implicit transient advection-diffusion stage of femIncompressibleSolver
+    (assembly, Dirichlet rows, GMRES, generalized-alpha update) on the GPU."""
+
+    def __init__(self, dmesh: DeviceMesh, dt, alpha_m, alpha_f, gamma, el_diff=None, dir_node=(), dir_val=None,
+                 tol=1.0e-5, max_iter=1000, restart=30):
+        L = load()
+        self.dmesh = dmesh
+        self.N, self.E = dmesh.mesh.n_nodes, dmesh.mesh.n_elems
+        dn = np.ascontiguousarray(dir_node, np.int32)
+        dv = np.ascontiguousarray(np.zeros((len(dn), 3)) if dir_val is None else dir_val, np.float64).reshape(-1, 3)
+        ed = None if el_diff is None else np.ascontiguousarray(el_diff, np.float64).reshape(self.E, 3)
+        p = INSParams(dt, alpha_m, alpha_f, gamma, tol, max_iter, restart)
+        h = C.c_void_p()
+        _check(L.femx_ins_create(dmesh._h, None if ed is None else _p(ed), len(dn), _p(dn, _ip), _p(dv),
+                                 C.byref(p), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.femx_ins_destroy(self._h)
+            self._h = None
+
+    def set_state(self, phi=None, phi_dot=None, vel=None):
+        a = [None if x is None else np.ascontiguousarray(x, np.float64).reshape(self.N, k)
+             for x, k in ((phi, 1), (phi_dot, 1), (vel, 3))]
+        _check(load().femx_ins_set_state(self._h, *[None if x is None else _p(x) for x in a]))
+
+    def prescribe_velocity(self, vel_type, t, node_vel=None):
+        nv = None if node_vel is None else np.ascontiguousarray(node_vel, np.float64).reshape(self.N, 3)
+        _check(load().femx_ins_prescribe_velocity(self._h, int(vel_type), float(t), None if nv is None else _p(nv)))
+
+    def advdiff_solve(self):
+        st = INSStats()
+        _check(load().femx_ins_advdiff_solve(self._h, C.byref(st)))
+        return dict(iterations=st.iterations, converged=bool(st.converged), rel_residual=st.rel_residual)
+
+    def update(self):
+        _check(load().femx_ins_update(self._h))
+
+    def get_state(self):
+        phi, phid, vel = np.zeros(self.N), np.zeros(self.N), np.zeros((self.N, 3))
+        _check(load().femx_ins_get_state(self._h, _p(phi), _p(phid), _p(vel)))
+        return phi, phid, vel
+
+    def get_system(self):
+        """(row_ptr, col, val, rhs, phid_new) of the last solve."""
+        nnz = C.c_longlong(0)
+        L = load()
+        _check(L.femx_ins_get_system(self._h, None, None, None, None, None, C.byref(nnz)))
+        rp, col = np.zeros(self.N + 1, np.int32), np.zeros(nnz.value, np.int32)
+        val, rhs, x = np.zeros(nnz.value), np.zeros(self.N), np.zeros(self.N)
+        _check(L.femx_ins_get_system(self._h, _p(rp, _ip), _p(col, _ip), _p(val), _p(rhs), _p(x), None))
+        return rp, col, val, rhs, x
+
+    def sync(self):
+        _check(load().femx_ins_sync(self._h))
+
+    def stream(self):
+        return load().femx_ins_stream(self._h)
+
+    def kernel_timing(self, enable=True):
+        _check(load().femx_ins_kernel_timing(self._h, int(enable)))
+
+    def kernel_times(self):
+        return _kernel_times(load().femx_ins_kernel_times, self._h)
 
 
 class Server:

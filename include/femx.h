@@ -286,6 +286,63 @@ femx_status femx_ffd_control_points(const femx_ffd_box *box, double *coords, dou
  * coordinates (s, t, u), as the reference stores them. */
 femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, const double *coords, double *disp);
 
+/* ------------------------------- implicit advection-diffusion (gen-alpha) */
+/* femIncompressibleSolver (src/femIncompressibleSolver.h:6-11,
+ * src/femIncompressibleSolver.cpp:289-496), the reference's implicit
+ * solver.  Its working stage is advanceAdvectionDiffusion (:129-283). That
+ * stage assembles the SUPG transient advection-diffusion system of
+ * formTransientAdvDiffLHS/RHS (src/femElement.cpp:1471-1689) for the time
+ * derivative of node dof 4, with the velocity of dofs 0..2.  It applies
+ * the Dirichlet rows and solves the system with AztecOO
+ * (femSolver::solveLinearSystem src/femSolver.cpp:146-177).  The update
+ * phi_n1 = phi_n + dt phid_n + gamma dt (phid_n1 - phid_n) follows (:471-485).
+ * Here the system lives on the GPU as CSR over the reference's pattern
+ * (femSparseMatrix, src/femMatrix.cpp:29-105).  It is assembled in the
+ * reference's element order and solved by restarted GMRES with right
+ * Jacobi preconditioning, all on the handle's stream. */
+typedef struct femx_ins femx_ins;
+typedef struct {
+  double dt;            /* timeStep                                        */
+  double alpha_m;       /* TIMEINTEGRATION alphaM, alphaF, gamma            */
+  double alpha_f;
+  double gamma;
+  double tol;           /* ||b - A x|| <= tol ||b||; the reference asks AztecOO for 1e-5 */
+  int max_iter;         /* the reference: 1000                              */
+  int restart;          /* Krylov space per cycle, 1..64 (AztecOO's default: 30) */
+} femx_ins_params;
+typedef struct {
+  int iterations;       /* Krylov iterations of the last solve              */
+  int converged;        /* 1 if the tolerance was met                       */
+  double rel_residual;  /* true ||b - A x|| / ||b|| at exit                 */
+} femx_ins_stats;
+/* el_diff [E][3]: model->elDiffusivity (ELDIFF), only |d| enters tau
+ * (evalQuadraticTau src/femElement.cpp:237-268); NULL = zero.
+ * dir_node/dir_val [n_dir][3]: diricheletBCNode/Values; row := identity,
+ * rhs := dir_val[i][0] (later entries win). */
+femx_status femx_ins_create(femx_mesh *mesh, const double *el_diff, int n_dir, const int *dir_node,
+                            const double *dir_val, const femx_ins_params *p, femx_ins **out);
+femx_status femx_ins_destroy(femx_ins *s);
+/* phi, phi_dot [N] (dof 4 of solution_n / solution_Dot_n); vel [N][3]
+ * (dofs 0..2).  NULL leaves that field unchanged. */
+femx_status femx_ins_set_state(femx_ins *s, const double *phi, const double *phi_dot, const double *vel);
+/* femModel::prescribeNodeVels (src/femModel.cpp:4515-4546) at time t:
+ * type 0 the time-periodic vortex, 1 the steady cellular flow, 2 the given
+ * node_vel [N][3] (the reference copies elVelocity row i to node i). */
+femx_status femx_ins_prescribe_velocity(femx_ins *s, int type, double t, const double *node_vel);
+/* One advection-diffusion stage: assemble, Dirichlet rows, solve for the
+ * new time derivative (kept until femx_ins_update).  stats may be NULL. */
+femx_status femx_ins_advdiff_solve(femx_ins *s, femx_ins_stats *stats);
+/* The generalized-alpha update with the last solve's derivative. */
+femx_status femx_ins_update(femx_ins *s);
+femx_status femx_ins_get_state(femx_ins *s, double *phi, double *phi_dot, double *vel);
+/* The last assembled system (after the Dirichlet rows): row_ptr [N+1],
+ * col/val [nnz], rhs [N], and the solve's result phid_new [N]; any pointer
+ * may be NULL.  *nnz (if not NULL) receives the number of non-zeros. */
+femx_status femx_ins_get_system(femx_ins *s, int *row_ptr, int *col, double *val, double *rhs, double *phid_new,
+                                long long *nnz);
+femx_status femx_ins_sync(femx_ins *s);
+void *femx_ins_stream(femx_ins *s);
+
 /* --------------------------------------------------------- measurement */
 /* Per-kernel HIP-event timing on the handle's stream (off by default).
  * While enabled, every kernel launch is bracketed by events; femx_*_kernel_times
@@ -300,6 +357,8 @@ femx_status femx_vms_kernel_timing(femx_vms *s, int enable);
 int femx_vms_kernel_times(femx_vms *s, femx_kernel_time *out, int max_entries);
 femx_status femx_supg_kernel_timing(femx_supg *s, int enable);
 int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int max_entries);
+femx_status femx_ins_kernel_timing(femx_ins *s, int enable);
+int femx_ins_kernel_times(femx_ins *s, femx_kernel_time *out, int max_entries);
 
 #ifdef __cplusplus
 }

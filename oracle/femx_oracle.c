@@ -34,6 +34,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846 /* the value of glibc <math.h> */
+#endif
+
 /* ===================================================================== */
 /* Tables                                                                 */
 /* ===================================================================== */
@@ -1156,4 +1160,315 @@ int ox_ffd_morph(const double min_point[3], const int n[3], const double S[3], c
   }
   free(disps);
   return 0;
+}
+
+/* ===================================================================== */
+/* Implicit transient advection-diffusion (femIncompressibleSolver)       */
+/* ===================================================================== */
+
+static int cmp_int(const void *a, const void *b) {
+  const int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* femSparseMatrix::femSparseMatrix, src/femMatrix.cpp:29-105: the diagonal,
+ * then every other node of each element of the node (duplicates skipped),
+ * sorted ascending. */
+long ox_ins_pattern(int N, int E, int npe, const int *conn, int *row_ptr, int *col) {
+  int *deg = (int *)calloc((size_t)N + 1, sizeof(int));
+  for (long e = 0; e < E; e++)
+    for (int a = 0; a < npe; a++) deg[conn[e * npe + a]] += npe;
+  long *off = (long *)malloc(((size_t)N + 1) * sizeof(long));
+  off[0] = 0;
+  for (int n = 0; n < N; n++) off[n + 1] = off[n] + deg[n] + 1;
+  int *tmp = (int *)malloc((size_t)off[N] * sizeof(int));
+  int *cnt = (int *)calloc((size_t)N, sizeof(int));
+  for (int n = 0; n < N; n++) tmp[off[n] + cnt[n]++] = n;
+  for (long e = 0; e < E; e++)
+    for (int a = 0; a < npe; a++) {
+      const int r = conn[e * npe + a];
+      for (int b = 0; b < npe; b++) {
+        const int c = conn[e * npe + b];
+        if (c == r) continue;
+        int found = 0;
+        for (int k = 0; k < cnt[r]; k++)
+          if (tmp[off[r] + k] == c) {
+            found = 1;
+            break;
+          }
+        if (!found) tmp[off[r] + cnt[r]++] = c;
+      }
+    }
+  long nnz = 0;
+  for (int n = 0; n < N; n++) {
+    qsort(tmp + off[n], (size_t)cnt[n], sizeof(int), cmp_int);
+    row_ptr[n] = (int)nnz;
+    if (col)
+      for (int k = 0; k < cnt[n]; k++) col[nnz + k] = tmp[off[n] + k];
+    nnz += cnt[n];
+  }
+  row_ptr[N] = (int)nnz;
+  free(deg);
+  free(off);
+  free(tmp);
+  free(cnt);
+  return nnz;
+}
+
+/* femModel::prescribeNodeVels, src/femModel.cpp:4515-4546. */
+int ox_ins_prescribe(int type, double t, int N, const double *coords, const double *node_vel, double *vel) {
+  if (type == 2 && !node_vel) return -1;
+  if (type < 0 || type > 2) return -1;
+  for (long a = 0; a < N; a++) {
+    const double x = coords[3 * a], y = coords[3 * a + 1];
+    if (type == 0) {
+      vel[3 * a] = cos(M_PI * t / 8.0) * sin(2.0 * M_PI * y) * sin(M_PI * x) * sin(M_PI * x);
+      vel[3 * a + 1] = -cos(M_PI * t / 8.0) * sin(2.0 * M_PI * x) * sin(M_PI * y) * sin(M_PI * y);
+      vel[3 * a + 2] = 0.0;
+    } else if (type == 1) {
+      vel[3 * a] = sin(M_PI * x) * cos(M_PI * y);
+      vel[3 * a + 1] = -cos(M_PI * x) * sin(M_PI * y);
+      vel[3 * a + 2] = 0.0;
+    } else {
+      for (int k = 0; k < 3; k++) vel[3 * a + k] = node_vel[3 * a + k];
+    }
+  }
+  return 0;
+}
+
+/* evalQuadraticTau, src/femElement.cpp:237-268 (c = 1). */
+static double ins_tau(double dt, const double G[9], const double u[3], const double diff[3]) {
+  const double timeTau = 4.0 / (dt * dt);
+  const double cConst = 1.0;
+  const double isoDiff = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+  double firstTau = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) firstTau += u[a] * G[a * 3 + b] * u[b];
+  double secondTau = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) secondTau += G[a * 3 + b] * G[a * 3 + b];
+  secondTau *= cConst * cConst * isoDiff * isoDiff;
+  return 1.0 / sqrt(timeTau + firstTau + secondTau);
+}
+
+/* formTransientAdvDiffLHS :1471-1563 and formTransientAdvDiffRHS :1568-1689.
+ * Both loop the same Gauss points with the same kinematics, so one pass
+ * forms both. */
+void ox_ins_element(int npe, const double *xel, const double *vel, const double *phi, const double *phid,
+                    const double diff[3], double dt, double am, double af, double gm, double *elmat,
+                    double *elrhs) {
+  double gc[32], gw[8];
+  const int ngp = ox_rule_second_order(npe, gc, gw);
+  for (int b = 0; b < npe * npe; b++) elmat[b] = 0.0;
+  for (int b = 0; b < npe; b++) elrhs[b] = 0.0;
+  for (int g = 0; g < ngp; g++) {
+    const double xi[3] = {gc[g * 4], gc[g * 4 + 1], gc[g * 4 + 2]};
+    double N[8], DN[24], G[9], detJ;
+    ox_shape(npe, xi, N);
+    ox_global_deriv(npe, xel, xi, &detJ, DN);
+    ox_geom_matrix(npe, xel, xi, G);
+    double u[3];
+    for (int d = 0; d < 3; d++) {
+      u[d] = 0.0;
+      for (int b = 0; b < npe; b++) u[d] += N[b] * vel[b * 3 + d];
+    }
+    const double tau = ins_tau(dt, G, u, diff);
+    /* LHS :1539-1561 */
+    for (int B = 0; B < npe; B++)
+      for (int C = 0; C < npe; C++) {
+        double currK = am * N[B] * N[C];
+        double prod1 = 0.0, prod2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+          currK += N[B] * u[d] * DN[C * 3 + d] * af * dt * gm;
+          prod1 += u[d] * DN[B * 3 + d];
+          prod2 += u[d] * DN[C * 3 + d];
+        }
+        const double stabK = tau * prod1 * am * N[C] + tau * prod1 * prod2 * af * dt * gm;
+        elmat[B * npe + C] += (currK + stabK) * detJ * gw[g];
+      }
+    /* RHS :1641-1687 */
+    double phid_g = 0.0;
+    for (int b = 0; b < npe; b++) phid_g += N[b] * phid[b];
+    double gphi[3], gphid[3];
+    for (int d = 0; d < 3; d++) {
+      gphi[d] = 0.0;
+      gphid[d] = 0.0;
+      for (int b = 0; b < npe; b++) {
+        gphi[d] += DN[b * 3 + d] * phi[b];
+        gphid[d] += DN[b * 3 + d] * phid[b];
+      }
+    }
+    for (int B = 0; B < npe; B++) {
+      double currF = -N[B] * (1.0 - am) * phid_g;
+      double prod1 = 0.0, prod2 = 0.0, prod3 = 0.0;
+      for (int d = 0; d < 3; d++) {
+        currF += -N[B] * u[d] * gphi[d] - N[B] * u[d] * af * dt * (1.0 - gm) * gphid[d];
+        prod1 += u[d] * DN[B * 3 + d];
+        prod2 += u[d] * gphi[d];
+        prod3 += u[d] * gphid[d];
+      }
+      const double stabF = -tau * prod1 * (1.0 - am) * phid_g - tau * prod1 * prod2 -
+                           tau * prod1 * af * dt * (1.0 - gm) * prod3;
+      elrhs[B] += (currF + stabF) * detJ * gw[g];
+    }
+  }
+}
+
+/* Position of column c in row r (binary search over the sorted row). */
+static long csr_find(const int *row_ptr, const int *col, int r, int c) {
+  long lo = row_ptr[r], hi = row_ptr[r + 1] - 1;
+  while (lo <= hi) {
+    const long mid = (lo + hi) / 2;
+    if (col[mid] == c) return mid;
+    if (col[mid] < c) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+int ox_ins_assemble(const oxMesh *m, const int *row_ptr, const int *col, const double *vel, const double *phi,
+                    const double *phid, const double *el_diff, double dt, double am, double af, double gm,
+                    int n_dir, const int *dir_node, const double *dir_val, double *val, double *rhs) {
+  const int npe = m->npe, N = m->n_nodes;
+  for (long k = 0; k < row_ptr[N]; k++) val[k] = 0.0;
+  for (long n = 0; n < N; n++) rhs[n] = 0.0;
+  for (long e = 0; e < m->n_elems; e++) {
+    double xel[24], ve[24], pe[8], pde[8], emat[64], erhs[8];
+    const int *cn = m->conn + e * npe;
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) {
+        xel[a * 3 + k] = m->coords[(long)cn[a] * 3 + k];
+        ve[a * 3 + k] = vel[(long)cn[a] * 3 + k];
+      }
+      pe[a] = phi[cn[a]];
+      pde[a] = phid[cn[a]];
+    }
+    ox_ins_element(npe, xel, ve, pe, pde, el_diff + 3 * e, dt, am, af, gm, emat, erhs);
+    for (int a = 0; a < npe; a++)
+      for (int b = 0; b < npe; b++) {
+        const long k = csr_find(row_ptr, col, cn[a], cn[b]);
+        if (k < 0) return -1;
+        val[k] += emat[a * npe + b];
+      }
+    for (int a = 0; a < npe; a++) rhs[cn[a]] += erhs[a];
+  }
+  for (int i = 0; i < n_dir; i++) {
+    const int r = dir_node[i];
+    for (long k = row_ptr[r]; k < row_ptr[r + 1]; k++) val[k] = (col[k] == r) ? 1.0 : 0.0;
+    rhs[r] = dir_val[3 * i];
+  }
+  return 0;
+}
+
+static void csr_mv(int N, const int *row_ptr, const int *col, const double *val, const double *x, double *y) {
+  for (long r = 0; r < N; r++) {
+    double s = 0.0;
+    for (long k = row_ptr[r]; k < row_ptr[r + 1]; k++) s += val[k] * x[col[k]];
+    y[r] = s;
+  }
+}
+
+static double vdot(long n, const double *a, const double *b) {
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+int ox_gmres(int N, const int *row_ptr, const int *col, const double *val, const double *b, double *x,
+             double tol, int max_iter, int restart, int *iters, double *relres) {
+  const int m = restart;
+  double *dinv = zalloc(N), *r = zalloc(N), *w = zalloc(N), *z = zalloc(N);
+  double *V = zalloc((long)(m + 1) * N);
+  double *H = zalloc((long)(m + 1) * m), *cs = zalloc(m), *sn = zalloc(m), *gv = zalloc(m + 1), *y = zalloc(m);
+  double *h = zalloc(m + 1);
+  for (long i = 0; i < N; i++) {
+    x[i] = 0.0;
+    double d = 0.0;
+    for (long k = row_ptr[i]; k < row_ptr[i + 1]; k++)
+      if (col[k] == i) d = val[k];
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+  }
+  for (long i = 0; i < N; i++) r[i] = b[i];
+  const double r0 = sqrt(vdot(N, b, b));
+  double beta = r0;
+  int it = 0, conv = (beta == 0.0);
+  while (!conv && it < max_iter) {
+    for (long i = 0; i < N; i++) V[i] = r[i] / beta;
+    for (int i = 0; i <= m; i++) gv[i] = 0.0;
+    gv[0] = beta;
+    int j = 0, jend = -1;
+    for (j = 0; j < m; j++) {
+      const double *vj = V + (long)j * N;
+      for (long i = 0; i < N; i++) z[i] = dinv[i] * vj[i];
+      csr_mv(N, row_ptr, col, val, z, w);
+      for (int pass = 0; pass < 2; pass++) {
+        for (int i = 0; i <= j; i++) h[i] = vdot(N, w, V + (long)i * N);
+        for (long q = 0; q < N; q++) {
+          double s = w[q];
+          for (int i = 0; i <= j; i++) s -= h[i] * V[(long)i * N + q];
+          w[q] = s;
+        }
+        for (int i = 0; i <= j; i++) H[i * m + j] = (pass ? H[i * m + j] : 0.0) + h[i];
+      }
+      const double hn = sqrt(vdot(N, w, w));
+      H[(j + 1) * m + j] = hn;
+      for (int i = 0; i < j; i++) {
+        const double t0 = H[i * m + j], t1 = H[(i + 1) * m + j];
+        H[i * m + j] = cs[i] * t0 + sn[i] * t1;
+        H[(i + 1) * m + j] = -sn[i] * t0 + cs[i] * t1;
+      }
+      {
+        const double a = H[j * m + j], c = H[(j + 1) * m + j];
+        const double den = sqrt(a * a + c * c);
+        cs[j] = den != 0.0 ? a / den : 1.0;
+        sn[j] = den != 0.0 ? c / den : 0.0;
+        H[j * m + j] = den;
+        H[(j + 1) * m + j] = 0.0;
+        gv[j + 1] = -sn[j] * gv[j];
+        gv[j] = cs[j] * gv[j];
+      }
+      it++;
+      jend = j;
+      if (fabs(gv[j + 1]) <= tol * r0 || it >= max_iter || hn == 0.0) break;
+      double *vn = V + (long)(j + 1) * N;
+      for (long i = 0; i < N; i++) vn[i] = w[i] / hn;
+    }
+    /* back substitution H y = g, then x += D^-1 V y */
+    for (int i = jend; i >= 0; i--) {
+      double s = gv[i];
+      for (int k = i + 1; k <= jend; k++) s -= H[i * m + k] * y[k];
+      y[i] = H[i * m + i] != 0.0 ? s / H[i * m + i] : 0.0;
+    }
+    for (long q = 0; q < N; q++) {
+      double s = 0.0;
+      for (int i = 0; i <= jend; i++) s += y[i] * V[(long)i * N + q];
+      x[q] += dinv[q] * s;
+    }
+    csr_mv(N, row_ptr, col, val, x, w);
+    for (long i = 0; i < N; i++) r[i] = b[i] - w[i];
+    beta = sqrt(vdot(N, r, r));
+    conv = beta <= tol * r0;
+  }
+  *iters = it;
+  *relres = r0 > 0.0 ? beta / r0 : 0.0;
+  free(dinv);
+  free(r);
+  free(w);
+  free(z);
+  free(V);
+  free(H);
+  free(cs);
+  free(sn);
+  free(gv);
+  free(y);
+  free(h);
+  return conv;
+}
+
+/* solve() :471-485: phi_n1 = phi_n + dt phid_n + gamma dt (phid_n1 - phid_n). */
+void ox_ins_update(int N, double dt, double gm, double *phi, double *phid, const double *phid_new) {
+  for (long a = 0; a < N; a++) {
+    phi[a] = phi[a] + dt * phid[a] + gm * dt * (phid_new[a] - phid[a]);
+    phid[a] = phid_new[a];
+  }
 }

@@ -139,6 +139,51 @@ int ox_ffd_morph(const double min_point[3], const int n[3], const double S[3], c
                  const double U[3], int n_disp, const int *disp_node, const double *disp_val, int npts,
                  const double *coords, double *out);
 
+/* ------------------------------------------- implicit advection-diffusion */
+/* femIncompressibleSolver's advection-diffusion stage (src/
+ * femIncompressibleSolver.cpp:129-283) and its generalized-alpha update
+ * (:471-485).  The matrix is kept as CSR over the reference's pattern (every
+ * node with itself and each node it shares an element with, ascending); the
+ * reference stores the same structurally symmetric matrix column-wise. */
+
+/* femSparseMatrix::femSparseMatrix (src/femMatrix.cpp:29-105).  Fills
+ * row_ptr [N+1]; col [nnz] when not NULL.  Returns nnz. */
+long ox_ins_pattern(int N, int E, int npe, const int *conn, int *row_ptr, int *col);
+
+/* femModel::prescribeNodeVels (src/femModel.cpp:4515-4546): type 0 the
+ * time-periodic vortex, 1 the steady cellular flow, 2 node_vel [N][3] (the
+ * reference reads elVelocity by node index).  vel [N][3].  Returns 0, or -1
+ * for another type or a missing node_vel. */
+int ox_ins_prescribe(int type, double t, int N, const double *coords, const double *node_vel, double *vel);
+
+/* formTransientAdvDiffLHS / formTransientAdvDiffRHS (src/femElement.cpp:
+ * 1471-1689) on one element: xel/vel [npe][3], phi/phid [npe], diff [3];
+ * elmat [npe][npe], elrhs [npe]. */
+void ox_ins_element(int npe, const double *xel, const double *vel, const double *phi, const double *phid,
+                    const double diff[3], double dt, double am, double af, double gm, double *elmat,
+                    double *elrhs);
+
+/* advanceAdvectionDiffusion assembly (:177-245): element matrices summed in
+ * element order into val [nnz], element vectors into rhs [N], then the
+ * Dirichlet rows (femSparseMatrix::applyDirichelet src/femMatrix.cpp:183-205,
+ * femDenseVector::applyDirichelet src/femVector.cpp:89-93): row zeroed, 1 on
+ * the diagonal, rhs = the first BC value.  vel [N][3]; el_diff [E][3];
+ * dir_val [n_dir][3].  Returns 0. */
+int ox_ins_assemble(const oxMesh *m, const int *row_ptr, const int *col, const double *vel, const double *phi,
+                    const double *phid, const double *el_diff, double dt, double am, double af, double gm,
+                    int n_dir, const int *dir_node, const double *dir_val, double *val, double *rhs);
+
+/* The linear solve (the AztecOO role of femSolver::solveLinearSystem,
+ * src/femSolver.cpp:146-177): restarted GMRES(restart), right Jacobi
+ * preconditioning, classical Gram-Schmidt applied twice, zero initial guess,
+ * stop when ||b - A x|| <= tol ||b|| or after max_iter iterations.
+ * Returns 1 when converged, 0 otherwise. */
+int ox_gmres(int N, const int *row_ptr, const int *col, const double *val, const double *b, double *x,
+             double tol, int max_iter, int restart, int *iters, double *relres);
+
+/* Generalized-alpha update of the advected quantity (:471-485). */
+void ox_ins_update(int N, double dt, double gm, double *phi, double *phid, const double *phid_new);
+
 #ifdef __cplusplus
 }
 #endif

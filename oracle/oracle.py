@@ -73,6 +73,14 @@ def lib():
         L.ox_shape.argtypes = [C.c_int, _dp, _dp]
         L.ox_global_deriv.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.ox_geom_matrix.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.ox_ins_pattern.restype = C.c_long
+        L.ox_ins_pattern.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip]
+        L.ox_ins_prescribe.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.ox_ins_element.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp] + [C.c_double] * 4 + [_dp, _dp]
+        L.ox_ins_assemble.argtypes = ([C.POINTER(OxMesh), _ip, _ip, _dp, _dp, _dp, _dp] + [C.c_double] * 4 +
+                                      [C.c_int, _ip, _dp, _dp, _dp])
+        L.ox_gmres.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]
+        L.ox_ins_update.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -270,3 +278,96 @@ def ref_rules_lib():
     L = C.CDLL(path)
     L.ref_rule.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
     return L
+
+
+def ins_pattern(mesh):
+    """CSR pattern of femSparseMatrix (row_ptr [N+1], col [nnz])."""
+    L = lib()
+    conn = np.ascontiguousarray(mesh.conn, np.int32)
+    rp = np.zeros(mesh.n_nodes + 1, np.int32)
+    nnz = L.ox_ins_pattern(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(conn, _ip), _p(rp, _ip), None)
+    col = np.zeros(nnz, np.int32)
+    L.ox_ins_pattern(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(conn, _ip), _p(rp, _ip), _p(col, _ip))
+    return rp, col
+
+
+def ins_prescribe(vel_type, t, coords, node_vel=None):
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    out = np.zeros_like(c)
+    nv = None if node_vel is None else np.ascontiguousarray(node_vel, np.float64).reshape(-1, 3)
+    r = lib().ox_ins_prescribe(int(vel_type), float(t), len(c), _p(c), None if nv is None else _p(nv), _p(out))
+    if r != 0:
+        raise ValueError("unknown velocity type or missing node velocities")
+    return out
+
+
+def ins_element(xel, vel, phi, phid, diff, dt, am, af, gm):
+    """Element matrix and vector of the transient advection-diffusion stage."""
+    xel = np.ascontiguousarray(xel, np.float64).reshape(-1, 3)
+    npe = len(xel)
+    vel = np.ascontiguousarray(vel, np.float64).reshape(npe, 3)
+    phi = np.ascontiguousarray(phi, np.float64)
+    phid = np.ascontiguousarray(phid, np.float64)
+    diff = np.ascontiguousarray(diff, np.float64)
+    emat = np.zeros((npe, npe))
+    erhs = np.zeros(npe)
+    lib().ox_ins_element(npe, _p(xel), _p(vel), _p(phi), _p(phid), _p(diff), dt, am, af, gm, _p(emat), _p(erhs))
+    return emat, erhs
+
+
+def gmres(row_ptr, col, val, b, tol=1e-5, max_iter=1000, restart=30):
+    """Returns (x, converged, iterations, relative residual)."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    cl = np.ascontiguousarray(col, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b)
+    it = np.zeros(1, np.int32)
+    rr = np.zeros(1)
+    ok = lib().ox_gmres(len(b), _p(rp, _ip), _p(cl, _ip), _p(v), _p(b), _p(x), tol, max_iter, restart,
+                        _p(it, _ip), _p(rr))
+    return x, bool(ok), int(it[0]), float(rr[0])
+
+
+class INSOracle:
+    """FP64 restatement of femIncompressibleSolver's advection-diffusion
+    stage: assemble, Dirichlet rows, GMRES, generalized-alpha update."""
+
+    def __init__(self, mesh, el_diff, dir_node, dir_val, dt, alpha_m, alpha_f, gamma, tol=1e-5, max_iter=1000,
+                 restart=30):
+        self.mesh = mesh
+        self.k = _Keep(mesh, __import__("femorph_amd.mesh", fromlist=["BoundaryConditions"]).BoundaryConditions())
+        self.rp, self.col = ins_pattern(mesh)
+        N = mesh.n_nodes
+        self.diff = np.ascontiguousarray(el_diff, np.float64).reshape(mesh.n_elems, 3)
+        self.dir_node = np.ascontiguousarray(dir_node, np.int32)
+        self.dir_val = np.ascontiguousarray(dir_val, np.float64).reshape(-1, 3)
+        self.dt, self.am, self.af, self.gm = float(dt), float(alpha_m), float(alpha_f), float(gamma)
+        self.tol, self.max_iter, self.restart = tol, max_iter, restart
+        self.phi = np.zeros(N)
+        self.phid = np.zeros(N)
+        self.vel = np.zeros((N, 3))
+        self.val = np.zeros(len(self.col))
+        self.rhs = np.zeros(N)
+        self.phid_new = None
+
+    def prescribe(self, vel_type, t, node_vel=None):
+        self.vel = ins_prescribe(vel_type, t, self.mesh.coords, node_vel)
+
+    def assemble(self):
+        r = lib().ox_ins_assemble(C.byref(self.k.m), _p(self.rp, _ip), _p(self.col, _ip), _p(self.vel),
+                                  _p(self.phi), _p(self.phid), _p(self.diff), self.dt, self.am, self.af, self.gm,
+                                  len(self.dir_node), _p(self.dir_node, _ip), _p(self.dir_val), _p(self.val),
+                                  _p(self.rhs))
+        if r != 0:
+            raise RuntimeError("pattern mismatch")
+        return self.val, self.rhs
+
+    def advdiff_solve(self):
+        self.assemble()
+        x, ok, it, rr = gmres(self.rp, self.col, self.val, self.rhs, self.tol, self.max_iter, self.restart)
+        self.phid_new = x
+        return ok, it, rr
+
+    def update(self):
+        lib().ox_ins_update(len(self.phi), self.dt, self.gm, _p(self.phi), _p(self.phid), _p(self.phid_new))

@@ -370,6 +370,105 @@ def workload_c2(args):
     print(json.dumps(out), flush=True)
 
 
+def make_ins_workload(n, seed):
+    """Implicit advection-diffusion (femIncompressibleSolver, STAGES 0 1,
+    PRESCRIBEDVEL 0): a Gaussian blob in the time-periodic vortex on a
+    jittered n^3 x 6 tet cube, zero Dirichlet walls, diffusivity 1e-3,
+    dt = 0.5 h, generalized-alpha with rho_inf = 0.5."""
+    m = meshgen.tet_cube(n, 0.15, seed)
+    c = m.coords
+    wall = np.where((c.min(axis=1) < 1e-12) | (c.max(axis=1) > 1 - 1e-12))[0].astype(np.int32)
+    phi = np.exp(-40.0 * ((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.3) ** 2 + (c[:, 2] - 0.5) ** 2))
+    phi[wall] = 0.0
+    rinf = 0.5
+    am = 0.5 * (3.0 - rinf) / (1.0 + rinf)
+    af = 1.0 / (1.0 + rinf)
+    params = dict(dt=0.5 / n, alpha_m=am, alpha_f=af, gamma=0.5 + am - af)
+    diff = np.full((m.n_elems, 3), 1.0e-3 / np.sqrt(3.0))
+    return m, wall, phi, diff, params
+
+
+def workload_ins(args):
+    """Implicit transient advection-diffusion stage on the GPU: per step the
+    prescribed vortex velocity, assembly, Dirichlet rows, GMRES to the
+    reference's 1e-5, generalized-alpha update."""
+    n = args.n
+    m, wall, phi0, diff, p = make_ins_workload(n, args.seed)
+    E, N = m.n_elems, m.n_nodes
+    dm = femx.DeviceMesh(m)
+    t0 = time.perf_counter()
+    g = femx.INSSolver(dm, el_diff=diff, dir_node=wall, dir_val=np.zeros((len(wall), 3)), **p)
+    setup_s = time.perf_counter() - t0
+    g.set_state(phi=phi0, phi_dot=np.zeros(N))
+    dt = p["dt"]
+
+    def run(k, t):
+        its = []
+        for _ in range(k):
+            g.prescribe_velocity(0, t)
+            its.append(g.advdiff_solve()["iterations"])
+            g.update()
+            t += dt
+        g.sync()
+        return t, its
+
+    t, _ = run(args.warmup, 0.0)
+    t0 = time.perf_counter()
+    t, its = run(args.steps, t)
+    secs = time.perf_counter() - t0
+    g.kernel_timing(True)
+    t, its2 = run(args.steps, t)
+    kt = g.kernel_times()
+    g.kernel_timing(False)
+    phi, _, _ = g.get_state()
+    _, col, _, _, _ = g.get_system()
+    nnz = len(col)
+    value = E * args.steps / secs
+    dom = max(kt, key=lambda k: kt[k]["total_ms"])
+    d = kt[dom]
+    mean_ms = d["total_ms"] / max(d["launches"], 1)
+    achieved = d["bytes_per_launch"] / (mean_ms * 1e-3) / 1e9
+    out = {
+        "metric": "element-steps/sec (implicit advection-diffusion, tet4)", "value": round(value, 1),
+        "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered Kuhn-split cube)",
+        "config": {"workload": f"implicit advection-diffusion tet4 {n}^3x6, vortex velocity, Gaussian blob, "
+                               f"GMRES(30) to 1e-5 ({E} tets, {N} nodes, {nnz} non-zeros)",
+                   "elements": E, "nodes": N, "nnz": nnz, **{k: round(v, 6) for k, v in p.items()}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "kernel_ms": round(mean_ms, 4), "kernel_bytes": d["bytes_per_launch"],
+                     "kernels": {k: {"launches": v["launches"], "mean_ms": round(v["total_ms"] / max(v["launches"], 1), 4),
+                                     "bytes": round(v["bytes_per_launch"]),
+                                     "gbs": round(v["bytes_per_launch"] / (v["total_ms"] / max(v["launches"], 1) * 1e-3) / 1e9, 1)
+                                     if v["launches"] else None,
+                                     "ms_per_step": round(v["total_ms"] / args.steps, 4)} for k, v in kt.items()}},
+        "gmres_iterations_per_step": round(float(np.mean(its + its2)), 2),
+        "setup_seconds": round(setup_s, 2),
+        "finite_state": bool(np.isfinite(phi).all()),
+    }
+    if not args.no_cpu:
+        from oracle.oracle import INSOracle  # test infrastructure, checker only
+        o = INSOracle(m, diff, wall, np.zeros((len(wall), 3)), p["dt"], p["alpha_m"], p["alpha_f"], p["gamma"])
+        o.phi = phi0.copy()
+        k, tt, tc = 0, 0.0, 0.0
+        while tt < args.cpu_budget and k < 10:
+            t0 = time.perf_counter()
+            o.prescribe(0, tc)
+            o.advdiff_solve()
+            o.update()
+            tt += time.perf_counter() - t0
+            tc += dt
+            k += 1
+        cv = E * k / tt
+        out["cpu_baseline"] = {"value": round(cv, 1), "unit": "element-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} full steps of the same mesh (assembly + GMRES(30) to 1e-5 + update), "
+                                         f"oracle (plain C, 1 thread), {tt:.1f} s"}
+        out["speedup_vs_cpu_1t"] = round(value / cv, 1)
+    print(json.dumps(out), flush=True)
+
+
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -455,7 +554,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins"], default="c3",
                     help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request stream")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
@@ -488,6 +587,9 @@ def main():
         return
     if args.workload == "serve":
         workload_serve(args)
+        return
+    if args.workload == "ins":
+        workload_ins(args)
         return
     if args.workload == "c2":
         workload_c2(args)

@@ -38,7 +38,7 @@ def hipcc() -> str:
 HOST = os.path.join(CSRC, "host")
 MODEL_LIB = os.path.join(HERE, "libfemx_model.so")
 DRIVER = os.path.join(HERE, "femx_explicit")
-HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp", "femPointGrid.cpp"]
+HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp", "femPointGrid.cpp", "femIncompressibleSolver.cpp"]
 GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-result"]
 
 

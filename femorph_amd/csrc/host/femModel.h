@@ -132,7 +132,7 @@ class femModel {
   void setNodeVelocity(double currTime, femDoubleMat &sol);
   void setDirichletBC(femDoubleMat &sol);
   // src/femModel.cpp:1294-1530 (ASCII VTK legacy, '%e')
-  void ExportToVTKLegacy(std::string fileName, const bool write_message);
+  void ExportToVTKLegacy(std::string fileName, const bool write_message = true);
   // src/femModel.cpp:2844-3185 (nodes, tri3/tet4/tet10 cells, point results)
   void ReadFromVTKLegacy(std::string fileName);
 };

@@ -7,6 +7,8 @@
 //   femx_explicit --dump-model out.txt [model.txt]   (read + dump the parsed fields)
 //   femx_explicit -j -f commandfile.txt [-o out.vtk] (free-form deformation, runFFD
 //                                                     src/main.cpp:742-781)
+//   femx_explicit -u [model.txt]                     (femIncompressibleSolver, the
+//                                                     implicit solver of `femorph -u`)
 //
 // Default: the SUPG solver (what solveExplicitNS runs) on ./model.txt in
 // FP64; --vms runs femVMSExplicitFluidSolver; --mixed selects FP32 element
@@ -16,6 +18,7 @@
 #include <string>
 
 #include "femExplicitFluidSolvers.h"
+#include "femIncompressibleSolver.h"
 #include "femPointGrid.h"
 #include "femx.h"
 
@@ -55,6 +58,14 @@ void dump_model(const femModel *m, const char *path) {
   fprintf(f, "problem %d\n", (int)m->problemType);
   fprintf(f, "output %s\n", m->sol_file_prefix.empty() ? "-" : m->sol_file_prefix.c_str());
   fprintf(f, "logfile %s\n", m->log_file.empty() ? "-" : m->log_file.c_str());
+  fprintf(f, "timeintegration %.17g %.17g %.17g\n", m->alphaM, m->alphaF, m->gamma);
+  fprintf(f, "stages");
+  for (int v : m->solStages) fprintf(f, " %d", v);
+  fprintf(f, "\nprescribedvel %d\n", m->usePrescribedVelocity ? m->prescribedVelType : -1);
+  for (size_t i = 0; i < m->iniNodeNumbers.size(); i++)
+    fprintf(f, "ini %d %d %.17g\n", m->iniNodeNumbers[i], m->iniDofNumber[i], m->iniDofValue[i]);
+  for (const auto &r : m->elVelocity) fprintf(f, "elvel %.17g %.17g %.17g\n", r[0], r[1], r[2]);
+  for (const auto &r : m->elDiffusivity) fprintf(f, "eldiff %.17g %.17g %.17g\n", r[0], r[1], r[2]);
   fclose(f);
 }
 
@@ -84,9 +95,13 @@ int run_ffd(const std::string &cmd, const std::string &out) {
 
 int main(int argc, char **argv) {
   std::string input = "model.txt", write_vtk, dump, ffd_cmd, output = "morphed.vtk";
-  bool vms = false, mixed = false, ffd = false;
+  bool vms = false, mixed = false, ffd = false, ins = false;
   for (int i = 1; i < argc; i++) {
     if (!strcmp(argv[i], "-y")) continue;
+    if (!strcmp(argv[i], "-u")) {  // solveIncompressibleNS (src/main.cpp:690-705)
+      ins = true;
+      continue;
+    }
     if (!strcmp(argv[i], "-j")) {
       ffd = true;
       continue;
@@ -110,8 +125,9 @@ int main(int argc, char **argv) {
     } else if (argv[i][0] == '-') {
       fprintf(stderr,
               "usage: %s [-y] [--vms] [--mixed] [model.txt]\n"
+              "       %s -u [model.txt]                 (implicit incompressible solver)\n"
               "       %s --write-vtk out.vtk | --dump-model out.txt [model.txt]\n",
-              argv[0], argv[0]);
+              argv[0], argv[0], argv[0]);
       return 2;
     } else {
       input = argv[i];
@@ -131,7 +147,10 @@ int main(int argc, char **argv) {
       delete model;
       return 0;
     }
-    if (vms) {
+    if (ins) {
+      femIncompressibleSolver s;
+      s.solve(model);
+    } else if (vms) {
       femVMSExplicitFluidSolver s;
       s.precision = mixed ? FEMX_MIXED : FEMX_FP64;
       s.solve(model);

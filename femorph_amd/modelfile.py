@@ -31,7 +31,8 @@ class ModelFormatError(ValueError):
 
 @dataclass
 class Model:
-    """The fields of femModel the explicit solvers consume (src/femModel.h:22-108)."""
+    """The fields of femModel the explicit and implicit solvers consume
+    (src/femModel.h:22-108)."""
 
     coords: np.ndarray
     elem_types: list
@@ -50,6 +51,15 @@ class Model:
     problem: int = 0
     output: str = ""
     log_file: str = ""
+    # implicit solver (femIncompressibleSolver) fields, src/femModel.h
+    alpha_m: float = 0.0
+    alpha_f: float = 0.0
+    gamma: float = 0.0
+    stages: list = field(default_factory=list)
+    prescribed_vel: int = -1  # PRESCRIBEDVEL type, -1 when absent
+    ini: list = field(default_factory=list)  # (node 0-based, dof, value)
+    el_velocity: list = field(default_factory=list)  # ELVELS rows in file order
+    el_diffusivity: list = field(default_factory=list)  # ELDIFF rows in file order
 
     def mesh(self) -> Mesh:
         """The element list as a single-type Mesh (tet4 or hexa8)."""
@@ -150,6 +160,18 @@ def read_model_txt(path: str) -> Model:
                 m["vmsprops"] += [_f(t, k, "VMSPROPS") for k in range(1, 6)]
             elif key == "EXOPTS":
                 m["exopts"] += [1.0 if _i(t, k, "EXOPTS") != 0 else 0.0 for k in range(1, 7)]
+            elif key == "TIMEINTEGRATION":
+                m["alpha_m"], m["alpha_f"], m["gamma"] = (_f(t, k, "TIMEINTEGRATION") for k in (1, 2, 3))
+            elif key == "STAGES":
+                m["stages"] = [_i(t, k, "STAGES") for k in range(1, len(t))]
+            elif key == "PRESCRIBEDVEL":
+                m["prescribed_vel"] = int(_f(t, 1, "PRESCRIBEDVEL"))
+            elif key == "INI":
+                m.setdefault("ini", []).append((_i(t, 1, "INI") - 1, _i(t, 2, "INI"), _f(t, 3, "INI")))
+            elif key == "ELVELS":
+                m.setdefault("el_velocity", []).append([_f(t, k, "Element Velocity") for k in (2, 3, 4)])
+            elif key == "ELDIFF":
+                m.setdefault("el_diffusivity", []).append([_f(t, k, "Element Diffusivity") for k in (2, 3, 4)])
     bc = BoundaryConditions.from_lines(nodevel, nodedir, nodepres)
     # one direction entry per NODEVEL line, indexed by unique node afterwards
     if nodevel:
@@ -160,7 +182,8 @@ def read_model_txt(path: str) -> Model:
 def write_model_txt(path: str, mesh: Mesh, bc: BoundaryConditions, *, problem="EXPLICITVMS",
                     time_step=0.01, total_steps=2, save_every=1, dual_step=None, tot_dual_steps=0,
                     vmsprops=(1.0, 0.01, 1.0, 4.0, 2.0), exopts=None, max_node_dofs=4, fixed_dofs="NONE",
-                    exit_condition=None, output=None, log_file=None) -> None:
+                    exit_condition=None, output=None, log_file=None, time_integration=None, stages=None,
+                    prescribed_vel=None, ini=(), el_velocity=None, el_diffusivity=None) -> None:
     """Write a model file.  Values are written with 17 significant digits, so
     both readers recover the arrays exactly.  NODEVEL lines are grouped per
     node; the reference indexes their direction codes by line, so a model
@@ -198,3 +221,15 @@ def write_model_txt(path: str, mesh: Mesh, bc: BoundaryConditions, *, problem="E
             f.write(f"OUTPUT {output}\n")
         if log_file:
             f.write(f"LOGFILE {log_file}\n")
+        if time_integration is not None:
+            f.write("TIMEINTEGRATION " + " ".join(g % v for v in time_integration) + "\n")
+        if stages is not None:
+            f.write("STAGES " + " ".join(str(int(v)) for v in stages) + "\n")
+        if prescribed_vel is not None:
+            f.write(f"PRESCRIBEDVEL {int(prescribed_vel)}\n")
+        for n, dof, v in ini:
+            f.write(f"INI {int(n) + 1} {int(dof)} {g % v}\n")
+        for key, rows in (("ELVELS", el_velocity), ("ELDIFF", el_diffusivity)):
+            if rows is not None:
+                for e, r in enumerate(np.asarray(rows, np.float64).reshape(-1, 3)):
+                    f.write(f"{key} {e + 1} {g % r[0]} {g % r[1]} {g % r[2]}\n")

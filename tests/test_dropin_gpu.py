@@ -18,7 +18,7 @@ import pytest
 
 from femorph_amd import meshgen, modelfile
 from femorph_amd.build import DRIVER
-from oracle.oracle import SUPGOracle, VMSOracle
+from oracle.oracle import INSOracle, SUPGOracle, VMSOracle
 from tests.vtkio import read_vtk_legacy
 
 pytestmark = pytest.mark.gpu
@@ -138,3 +138,78 @@ def test_supg_exit_condition_exits(gpu, tmp_path):
     assert "SOLUTION TERMINATED." in r.stdout
     log = open(os.path.join(str(tmp_path), "solver.log")).read().splitlines()
     assert log[-1] == "SOLUTION TERMINATED." and log[1].split()[0] == "1"
+
+
+def _ins_model(tmp_path, mesh, stages, prescribed, total_steps, save_every, dt):
+    rng = np.random.default_rng(9)
+    c = mesh.coords
+    wall = np.where((c.min(axis=1) < 1e-12) | (c.max(axis=1) > 1 - 1e-12))[0]
+    from femorph_amd.mesh import BoundaryConditions
+    bc = BoundaryConditions(dir_node=wall.astype(np.int32), dir_val=np.zeros((len(wall), 3)))
+    blob = np.exp(-30.0 * ((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.3) ** 2 + (c[:, 2] - 0.5) ** 2))
+    ini = [(n, 4, float(blob[n])) for n in range(mesh.n_nodes) if n not in set(wall.tolist())]
+    diff = rng.uniform(0.0, 0.01, (mesh.n_elems, 3))
+    modelfile.write_model_txt(os.path.join(str(tmp_path), "model.txt"), mesh, bc, problem="POISSON",
+                              time_step=dt, total_steps=total_steps, save_every=save_every, max_node_dofs=5,
+                              time_integration=(0.8, 0.6, 0.7), stages=stages, prescribed_vel=prescribed, ini=ini,
+                              el_diffusivity=diff)
+    return bc, ini, diff
+
+
+def test_incompressible_solve_files_and_prints(gpu, tmp_path):
+    """femIncompressibleSolver::solve (src/femIncompressibleSolver.cpp:
+    289-496) through `femx_explicit -u`: velocities prescribed by the
+    rotating vortex at every step (stage 0), one advection-diffusion stage
+    (stage 1), generalized-alpha update, exports at the save cadence."""
+    mesh = meshgen.tet_cube(5, 0.1, 3)
+    dt = 0.05
+    bc, ini, diff = _ins_model(tmp_path, mesh, (0, 1), 0, total_steps=5, save_every=2, dt=dt)
+    r = _run(tmp_path, "-u")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Program Completed." in r.stdout
+    # totalSteps + 1 passes; saves after passes 2, 4, 6 (out_Step_<loopTime+1>)
+    files = sorted(f for f in os.listdir(str(tmp_path)) if f.endswith(".vtk"))
+    assert files == ["out_Step_0.vtk", "out_Step_2.vtk", "out_Step_4.vtk", "out_Step_6.vtk"]
+    steps = re.findall(r"Advection-Diffusion Step (\d+), Current Time: ([0-9.]+)", r.stdout)
+    assert [int(s) for s, _ in steps] == list(range(1, 7))
+    assert [float(t) for _, t in steps] == [round(max(k, 0) * dt, 6) for k in range(6)]
+    assert r.stdout.count("Update Velocities...Done.") == 6
+    assert r.stdout.count("Solver performed") == 6
+    # the oracle runs the same loop
+    o = INSOracle(mesh, diff, bc.dir_node, bc.dir_val, dt, 0.8, 0.6, 0.7, tol=1e-5)
+    for n, dof, v in ini:
+        o.phi[n] = v
+    o.prescribe(0, 0.0)
+    expect = {0: (o.phi.copy(), o.phid.copy(), o.vel.copy())}
+    t, counter = 0.0, 0
+    for loop in range(6):
+        if loop > 0:
+            t += dt
+        counter += 1
+        o.prescribe(0, t)
+        ok, it, rr = o.advdiff_solve()
+        assert ok
+        o.update()
+        if counter == 2:
+            counter = 0
+            expect[loop + 1] = (o.phi.copy(), o.phid.copy(), o.vel.copy())
+    for k, (phi, phid, vel) in expect.items():
+        v = read_vtk_legacy(os.path.join(str(tmp_path), f"out_Step_{k}.vtk"))["point_data"]
+        _close(v["velocity"], vel)
+        # both solves stop at ||r|| <= 1e-5 ||b||, each by its own Krylov
+        # path: the states agree to the solver tolerance, not to rounding
+        for name, ref in (("INSSolution", phi), ("INSSolution_Dot", phid)):
+            got = v[name].reshape(-1)
+            assert np.abs(got - ref).max() <= 1e-4 * max(np.abs(ref).max(), 1e-30), (k, name)
+
+
+def test_incompressible_ns_stage_not_implemented(gpu, tmp_path):
+    """Stage 0 without PRESCRIBEDVEL is the reference's Navier-Stokes stage,
+    which cannot run (empty RHS, no solve; its sparse block assembly throws
+    "Not Implemented.", src/femMatrix.cpp:307-309)."""
+    mesh = meshgen.tet_cube(2, 0.0, 0)
+    _ins_model(tmp_path, mesh, (0,), None, total_steps=2, save_every=1, dt=0.1)
+    r = _run(tmp_path, "-u")
+    assert r.returncode == 255
+    assert "Incompressible NS Step 1, Current Time: 0.000000" in r.stdout
+    assert "Not Implemented." in r.stdout and "Program Terminated." in r.stdout

@@ -32,7 +32,7 @@ def _dump(model_path, tmp_path):
     out = os.path.join(str(tmp_path), "dump.txt")
     r = _driver("--dump-model", out, model_path, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
-    d = dict(node=[], element=[], vel=[], dir=[], pres=[])
+    d = dict(node=[], element=[], vel=[], dir=[], pres=[], ini=[], elvel=[], eldiff=[])
     for ln in open(out):
         t = ln.split()
         if t[0] in d:
@@ -71,6 +71,12 @@ def _compare(d, m: modelfile.Model):
     assert float(d["exitcond"][0]) == m.exit_condition
     assert int(d["problem"][0]) == m.problem
     assert d["output"][0] == (m.output or "-") and d["logfile"][0] == (m.log_file or "-")
+    assert [float(v) for v in d["timeintegration"]] == [m.alpha_m, m.alpha_f, m.gamma]
+    assert [int(v) for v in d["stages"]] == m.stages
+    assert int(d["prescribedvel"][0]) == m.prescribed_vel
+    assert [(int(a), int(b), float(c)) for a, b, c in d["ini"]] == [tuple(r) for r in m.ini]
+    assert [[float(v) for v in r] for r in d["elvel"]] == m.el_velocity
+    assert [[float(v) for v in r] for r in d["eldiff"]] == m.el_diffusivity
 
 
 def test_reference_example_model_both_readers(tmp_path):
@@ -136,6 +142,27 @@ def test_grammar_details(tmp_path):
     assert m.bc.vel_dir.tolist() == [6, 1]
     assert m.bc.dir_val.tolist() == [[0.25, 0, 0], [1, 2, 3]]
     assert m.fixed_dofs == "XY" and m.max_node_dofs == 4 and m.exit_condition == 7.0
+    assert m.ini == [(0, 0, 3.0)]
+    _compare(_dump(path, tmp_path), m)
+
+
+def test_implicit_solver_keywords(tmp_path):
+    """TIMEINTEGRATION, STAGES, PRESCRIBEDVEL, INI, ELVELS and ELDIFF
+    (src/femModel.cpp:3892-3925, 4089-4133), written and read back by both
+    readers."""
+    mesh = meshgen.tet_cube(2, 0.1, 1)
+    bc = meshgen.lid_cavity_bc(mesh)
+    rng = np.random.default_rng(3)
+    ev = rng.normal(size=(mesh.n_elems, 3))
+    ed = rng.uniform(0, 0.1, (mesh.n_elems, 3))
+    path = os.path.join(str(tmp_path), "model.txt")
+    modelfile.write_model_txt(path, mesh, bc, max_node_dofs=5, time_integration=(0.8, 0.6, 0.7), stages=(0, 1),
+                              prescribed_vel=2, ini=[(3, 4, 1.25), (0, 1, -0.5)], el_velocity=ev, el_diffusivity=ed)
+    m = modelfile.read_model_txt(path)
+    assert (m.alpha_m, m.alpha_f, m.gamma) == (0.8, 0.6, 0.7) and m.stages == [0, 1] and m.prescribed_vel == 2
+    assert m.ini == [(3, 4, 1.25), (0, 1, -0.5)]
+    np.testing.assert_array_equal(np.array(m.el_velocity), ev)
+    np.testing.assert_array_equal(np.array(m.el_diffusivity), ed)
     _compare(_dump(path, tmp_path), m)
 
 

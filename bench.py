@@ -370,12 +370,12 @@ def workload_c2(args):
     print(json.dumps(out), flush=True)
 
 
-def make_ins_workload(n, seed):
+def make_ins_workload(n, seed, hexa=False):
     """Implicit advection-diffusion (femIncompressibleSolver, STAGES 0 1,
     PRESCRIBEDVEL 0): a Gaussian blob in the time-periodic vortex on a
-    jittered n^3 x 6 tet cube, zero Dirichlet walls, diffusivity 1e-3,
-    dt = 0.5 h, generalized-alpha with rho_inf = 0.5."""
-    m = meshgen.tet_cube(n, 0.15, seed)
+    jittered n^3 x 6 tet cube (or n^3 hexa cube), zero Dirichlet walls,
+    diffusivity 1e-3, dt = 0.5 h, generalized-alpha with rho_inf = 0.5."""
+    m = meshgen.hex_cube(n, 0.2, seed) if hexa else meshgen.tet_cube(n, 0.15, seed)
     c = m.coords
     wall = np.where((c.min(axis=1) < 1e-12) | (c.max(axis=1) > 1 - 1e-12))[0].astype(np.int32)
     phi = np.exp(-40.0 * ((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.3) ** 2 + (c[:, 2] - 0.5) ** 2))
@@ -393,7 +393,9 @@ def workload_ins(args):
     prescribed vortex velocity, assembly, Dirichlet rows, GMRES to the
     reference's 1e-5, generalized-alpha update."""
     n = args.n
-    m, wall, phi0, diff, p = make_ins_workload(n, args.seed)
+    hexa = args.ins_elem == "hexa8"
+    m, wall, phi0, diff, p = make_ins_workload(n, args.seed, hexa)
+    shape = f"hexa8 {n}^3" if hexa else f"tet4 {n}^3x6"
     E, N = m.n_elems, m.n_nodes
     dm = femx.DeviceMesh(m)
     t0 = time.perf_counter()
@@ -429,12 +431,12 @@ def workload_ins(args):
     mean_ms = d["total_ms"] / max(d["launches"], 1)
     achieved = d["bytes_per_launch"] / (mean_ms * 1e-3) / 1e9
     out = {
-        "metric": "element-steps/sec (implicit advection-diffusion, tet4)", "value": round(value, 1),
+        "metric": f"element-steps/sec (implicit advection-diffusion, {shape.split()[0]})", "value": round(value, 1),
         "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered Kuhn-split cube)",
-        "config": {"workload": f"implicit advection-diffusion tet4 {n}^3x6, vortex velocity, Gaussian blob, "
-                               f"GMRES(30) to 1e-5 ({E} tets, {N} nodes, {nnz} non-zeros)",
+        "config": {"workload": f"implicit advection-diffusion {shape}, vortex velocity, Gaussian blob, "
+                               f"GMRES(30) to 1e-5 ({E} elements, {N} nodes, {nnz} non-zeros)",
                    "elements": E, "nodes": N, "nnz": nnz, **{k: round(v, 6) for k, v in p.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
@@ -557,6 +559,7 @@ def main():
     ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins"], default="c3",
                     help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request stream")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--ins-elem", choices=["tet4", "hexa8"], default="tet4", help="ins: element type")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
     ap.add_argument("--requests", type=int, default=40, help="requests per rate (serve)")
     ap.add_argument("--budget-gb", type=int, default=64)

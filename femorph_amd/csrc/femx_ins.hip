@@ -5,19 +5,21 @@
 // One stage per femx_ins_advdiff_solve call, all on the handle's stream:
 //   ins_element     one thread per element: the element matrix and vector of
 //                   formTransientAdvDiffLHS / formTransientAdvDiffRHS
-//                   (src/femElement.cpp:1471-1689), FP64, into element-major
-//                   scratch (tet4 128 + 32 B, hexa8 512 + 64 B per element)
-//   ins_gather_mat  one thread per CSR entry: its element values summed in
-//                   ascending element order (the order femSparseMatrix::assemble
-//                   adds them, src/femMatrix.cpp:129-160), then the Dirichlet
-//                   row (applyDirichelet :183-205) and the Jacobi inverse
-//   ins_gather_rhs  one thread per node: element vector entries in ascending
-//                   element order (femDenseVector::assemble), Dirichlet value,
-//                   and the start of the Krylov solve (x = 0, r = b, ||b||^2)
+//                   (src/femElement.cpp:1471-1689) in FP64, scattered straight
+//                   into the order the gathers read (cpos / rpos maps)
+//   ins_gather_mat  one thread per CSR entry: its contiguous run of element
+//                   values, ascending element (the order femSparseMatrix::
+//                   assemble adds them, src/femMatrix.cpp:129-160), then the
+//                   Dirichlet row (applyDirichelet :183-205), Jacobi inverse
+//   ins_gather_rhs  one thread per node: the same for the element vectors
+//                   (femDenseVector::assemble), Dirichlet value, and the start
+//                   of the Krylov solve (x = 0, r = b, z = D^-1 b, ||b||^2)
 //   GMRES           restarted, right Jacobi preconditioning, classical
-//                   Gram-Schmidt applied twice; the SpMV carries the first
-//                   projection's dot products; every reduction is a fixed
-//                   tree (deterministic).  Plays AztecOO's role
+//                   Gram-Schmidt applied twice, zero initial guess; per
+//                   iteration a vector-CSR SpMV of z = D^-1 v, three
+//                   Gram-Schmidt passes sized to the basis in use, and
+//                   one-block finishers of their reductions (fixed order:
+//                   deterministic).  Plays AztecOO's role
 //                   (femSolver::solveLinearSystem src/femSolver.cpp:146-177).
 //   ins_update      phi += dt phid + gamma dt (phid_new - phid); phid = phid_new.
 //
@@ -68,11 +70,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block sums of acc[0..cnt) -> part[i * nb + blockIdx.x] (RT threads).
-__device__ __forceinline__ void block_store(const double (&acc)[MAXM], int cnt, double *part, int nb) {
-  __shared__ double red[RT / 64][MAXM];
+template <int J>
+__device__ __forceinline__ void block_store(const double (&acc)[J], int cnt, double *part, int nb) {
+  __shared__ double red[RT / 64][J];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < MAXM; i++)
+  for (int i = 0; i < J; i++)
     if (i < cnt) {
       const double v = wave_sum(acc[i]);
       if (lane == 0) red[wv][i] = v;
@@ -134,11 +137,15 @@ __global__ __launch_bounds__(256) void ins_pack(int N, const double *phi, const 
 }
 
 // ------------------------------------------------------------- element
+// Element t of the processing order (the mesh's Morton block order; every
+// per-element input is stored in that order).  Each XCD takes a contiguous
+// range of workgroups, so neighbouring elements write their scattered
+// values through the same L2.
 template <int NPE>
 __global__ __launch_bounds__(256) void ins_element(int E, const int *conn, const double4 *xyz, const double4 *vphi,
-                                                   const double *phid, const double *iso, InsConst k, double *emat,
-                                                   double *erhs) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   const double *phid, const double *iso, InsConst k,
+                                                   const int *cpos, const int *rpos, double *emat, double *erhs) {
+  const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int id[NPE];
 #pragma unroll
@@ -166,7 +173,9 @@ __global__ __launch_bounds__(256) void ins_element(int E, const int *conn, const
 #pragma unroll
     for (int c = 0; c < NPE; c++) M[b][c] = 0.0;
   }
-#pragma unroll
+  // hexa8: one Gauss point at a time keeps the 8x8 accumulator in registers
+  constexpr int GP_UNROLL = NPE == 4 ? 4 : 1;
+#pragma unroll GP_UNROLL
   for (int g = 0; g < Shape<NPE>::NGP; g++) {
     double DN[NPE][3], detJ, iJ[3][3];
     kinematics<NPE>(g, dx, DN, detJ, iJ);
@@ -221,58 +230,29 @@ __global__ __launch_bounds__(256) void ins_element(int E, const int *conn, const
       for (int c = 0; c < NPE; c++) M[b][c] += P[b] * Q[c];
     }
   }
-  double *me = emat + (long)e * NPE * NPE;
+  // scatter into the gathers' order: contiguous per matrix entry / per node
+  const int *cp = cpos + (long)e * NPE * NPE;
 #pragma unroll
   for (int b = 0; b < NPE; b++)
 #pragma unroll
-    for (int c = 0; c < NPE; c++) me[b * NPE + c] = M[b][c];
+    for (int c = 0; c < NPE; c++) emat[cp[b * NPE + c]] = M[b][c];
+  const int *rq = rpos + (long)e * NPE;
 #pragma unroll
-  for (int b = 0; b < NPE; b++) erhs[(long)e * NPE + b] = R[b];
+  for (int b = 0; b < NPE; b++) erhs[rq[b]] = R[b];
 }
 
-// -------------------------------------------------------------- gathers
-__global__ __launch_bounds__(256) void ins_gather_mat(long nnz, const int *c_ptr, const int *c_idx,
-                                                      const double *emat, const int *nz_row, const int *col,
-                                                      const int *dir_flag, double *val, double *dinv) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nnz) return;
-  const int r = nz_row[q], c = col[q];
-  double s = 0.0;
-  for (int j = c_ptr[q]; j < c_ptr[q + 1]; j++) s += emat[c_idx[j]];
-  if (dir_flag[r]) s = (c == r) ? 1.0 : 0.0;
-  val[q] = s;
-  if (c == r) dinv[r] = (s != 0.0) ? 1.0 / s : 1.0;
-}
+// ------------------------------------------------- reduction finishers
+// One small launch after each reduction pass reads its nb partial sums.
 
-// b, and the start of the solve: x = 0, r = b, partial ||b||^2.
-__global__ __launch_bounds__(RT) void ins_gather_rhs(int N, const int *g_off, const int *g_slot, const double *erhs,
-                                                     const int *dir_flag, const double *dir_v, double *b, double *x,
-                                                     double *rvec, double *part, int nb) {
-  double acc[MAXM];
-#pragma unroll
-  for (int i = 0; i < MAXM; i++) acc[i] = 0.0;
-  for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += nb * RT) {
-    double s = 0.0;
-    for (int j = g_off[r]; j < g_off[r + 1]; j++) s += erhs[g_slot[j]];
-    if (dir_flag[r]) s = dir_v[r];
-    b[r] = s;
-    x[r] = 0.0;
-    rvec[r] = s;
-    acc[0] += s * s;
-  }
-  block_store(acc, 1, part, nb);
-}
-
-// --------------------------------------------------------------- GMRES
-// Start of a cycle from r (first call: r0 := ||r||).
-__global__ __launch_bounds__(64) void gm_start(const double *part, int nb, GmresState *st, int first) {
+// Start of a cycle from r = b - A x: beta = ||r|| (first: also r0).
+__device__ __forceinline__ void fin_start(GmresState *st, const double *part, int nb, bool first) {
+  if (threadIdx.x >= 64) return;
   const double s = wave_total(part, 0, nb);
   if (threadIdx.x != 0) return;
   const double beta = sqrt(s);
   if (first) {
     st->r0 = beta;
     st->iters = 0;
-    st->conv = 0;
   }
   st->beta = beta;
   st->relres = st->r0 > 0.0 ? beta / st->r0 : 0.0;
@@ -286,40 +266,10 @@ __global__ __launch_bounds__(64) void gm_start(const double *part, int nb, Gmres
   st->vscale = beta > 0.0 ? 1.0 / beta : 0.0;
 }
 
-// v_j = src * vscale; w = A D^-1 v_j; partial dots w . v_i (i <= j).
-__global__ __launch_bounds__(RT) void gm_spmv(int N, const int *rp, const int *col, const double *val,
-                                              const double *dinv, const double *src, double *V, long ldv, double *w,
-                                              int j, const GmresState *st, double *part, int nb) {
-  if (st->cyc_stop) return;
-  const double sc = st->vscale;
-  double acc[MAXM];
-#pragma unroll
-  for (int i = 0; i < MAXM; i++) acc[i] = 0.0;
-  double *vj = V + (long)j * ldv;
-  for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += nb * RT) {
-    double s = 0.0;
-    for (int q = rp[r]; q < rp[r + 1]; q++) {
-      const int c = col[q];
-      s += val[q] * (dinv[c] * src[c]);
-    }
-    const double wr = s * sc, vr = src[r] * sc;
-    vj[r] = vr;
-    w[r] = wr;
-#pragma unroll
-    for (int i = 0; i < MAXM; i++) {
-      if (i < j) acc[i] += wr * V[(long)i * ldv + r];
-      else if (i == j) acc[i] += wr * vr;
-    }
-  }
-  block_store(acc, j + 1, part, nb);
-}
-
 // h = the reduced dots; pass 0 sets column j of H, pass 1 adds to it.
-__global__ __launch_bounds__(256) void gm_hfinal(const double *part, int nb, int cnt, GmresState *st, int j,
-                                                 int pass) {
-  if (st->cyc_stop) return;
+__device__ __forceinline__ void fin_h(GmresState *st, const double *part, int nb, int cnt, int j, int pass) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i = wv; i < cnt; i += 4) {
+  for (int i = wv; i < cnt; i += RT / 64) {
     const double s = wave_total(part, i, nb);
     if (lane == 0) {
       st->h[i] = s;
@@ -328,40 +278,9 @@ __global__ __launch_bounds__(256) void gm_hfinal(const double *part, int nb, int
   }
 }
 
-// w -= sum_i h_i v_i; pass 0: partial dots w . v_i, pass 1: partial ||w||^2.
-__global__ __launch_bounds__(RT) void gm_orth(int N, double *w, const double *V, long ldv, int j,
-                                              const GmresState *st, double *part, int nb, int pass) {
-  if (st->cyc_stop) return;
-  double h[MAXM], acc[MAXM];
-#pragma unroll
-  for (int i = 0; i < MAXM; i++) {
-    h[i] = i <= j ? st->h[i] : 0.0;
-    acc[i] = 0.0;
-  }
-  for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += nb * RT) {
-    double v[MAXM];
-    double s = w[r];
-#pragma unroll
-    for (int i = 0; i < MAXM; i++)
-      if (i <= j) {
-        v[i] = V[(long)i * ldv + r];
-        s -= h[i] * v[i];
-      }
-    w[r] = s;
-    if (pass == 0) {
-#pragma unroll
-      for (int i = 0; i < MAXM; i++)
-        if (i <= j) acc[i] += s * v[i];
-    } else {
-      acc[0] += s * s;
-    }
-  }
-  block_store(acc, pass == 0 ? j + 1 : 1, part, nb);
-}
-
 // H[j+1][j] = ||w||, Givens rotations, residual estimate, cycle stop test.
-__global__ __launch_bounds__(64) void gm_givens(const double *part, int nb, GmresState *st, int j) {
-  if (st->cyc_stop) return;
+__device__ __forceinline__ void fin_givens(GmresState *st, const double *part, int nb, int j) {
+  if (threadIdx.x >= 64) return;
   const double s = wave_total(part, 0, nb);
   if (threadIdx.x != 0) return;
   double *H = st->H;
@@ -387,6 +306,125 @@ __global__ __launch_bounds__(64) void gm_givens(const double *part, int nb, Gmre
   st->vscale = hn != 0.0 ? 1.0 / hn : 0.0;
 }
 
+// -------------------------------------------------------------- gathers
+constexpr int CAP = 4096;  // doubles of element values staged in LDS per pass
+
+// Sum of src[beg .. end) in ascending order for each thread of the block,
+// where the threads' ranges are consecutive and together span [lo, hi):
+// the values pass through LDS in coalesced chunks of CAP (a node's run of
+// element values is long, ~24 for tet4, so lane-private reads would stride).
+__device__ __forceinline__ double staged_run_sum(const double *src, long lo, long hi, long beg, long end) {
+  __shared__ double buf[CAP];
+  double s = 0.0;
+  for (long c0 = lo; c0 < hi; c0 += CAP) {
+    const long c1 = c0 + CAP < hi ? c0 + CAP : hi;
+    __syncthreads();
+    for (long k = c0 + threadIdx.x; k < c1; k += blockDim.x) buf[k - c0] = src[k];
+    __syncthreads();
+    const long a = beg > c0 ? beg : c0, b = end < c1 ? end : c1;
+    for (long k = a; k < b; k++) s += buf[k - c0];
+  }
+  return s;
+}
+
+// The element values of entry q are emat[c_ptr[q] .. c_ptr[q+1]), ascending
+// element (the order of femSparseMatrix::assemble).  The runs are short
+// (a few values), so consecutive threads already read consecutive memory.
+__global__ __launch_bounds__(RT) void ins_gather_mat(long nnz, const int *c_ptr, const double *emat,
+                                                     const int *nz_row, const int *col, const int *dir_flag,
+                                                     double *val, double *dinv) {
+  const long q = (long)blockIdx.x * RT + threadIdx.x;
+  if (q >= nnz) return;
+  const int r = nz_row[q], c = col[q];
+  double s = 0.0;
+  for (int j = c_ptr[q]; j < c_ptr[q + 1]; j++) s += emat[j];
+  if (dir_flag[r]) s = (c == r) ? 1.0 : 0.0;
+  val[q] = s;
+  if (c == r) dinv[r] = (s != 0.0) ? 1.0 / s : 1.0;
+}
+
+// b (element vectors erhs[g_off[r] .. g_off[r+1]), ascending element, as
+// femDenseVector::assemble adds them), and the start of the solve: x = 0,
+// r = b, z = D^-1 r, per-block ||b||^2 into part[blockIdx.x].
+__global__ __launch_bounds__(RT) void ins_gather_rhs(int N, const int *g_off, const double *erhs,
+                                                     const int *dir_flag, const double *dir_v, const double *dinv,
+                                                     double *b, double *x, double *rvec, double *z, double *part) {
+  const int r0 = blockIdx.x * RT, r = r0 + threadIdx.x;
+  const int re = r0 + RT < N ? r0 + RT : N;
+  const long lo = g_off[r0], hi = g_off[re];
+  const long beg = r < N ? g_off[r] : hi, end = r < N ? g_off[r + 1] : hi;
+  double s = staged_run_sum(erhs, lo, hi, beg, end);
+  double acc[1] = {0.0};
+  if (r < N) {
+    if (dir_flag[r]) s = dir_v[r];
+    b[r] = s;
+    x[r] = 0.0;
+    rvec[r] = s;
+    z[r] = dinv[r] * s;
+    acc[0] = s * s;
+  }
+  block_store<1>(acc, 1, part, gridDim.x);
+}
+
+// --------------------------------------------------------------- GMRES
+// v_j = src * vscale and w = A D^-1 v_j = vscale * A z, z = D^-1 src;
+// LPR lanes per row (vector CSR), lane 0 of each row group stores.
+template <int LPR>
+__global__ __launch_bounds__(RT) void gm_spmv(int N, const int *rp, const int *col, const double *val,
+                                              const double *z, const double *src, double *V, long ldv, double *w,
+                                              int j, const GmresState *st) {
+  if (st->cyc_stop) return;
+  const double sc = st->vscale;
+  const int lane = threadIdx.x & (LPR - 1);
+  const int r = (blockIdx.x * RT + threadIdx.x) / LPR;
+  if (r >= N) return;
+  double s = 0.0;
+  for (int q = rp[r] + lane; q < rp[r + 1]; q += LPR) s += val[q] * z[col[q]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) {
+    w[r] = s * sc;
+    V[(long)j * ldv + r] = src[r] * sc;
+  }
+}
+
+// Gram-Schmidt passes over w and v_0..v_j (j < J), partial sums into part:
+//   mode 0: dots w . v_i                            (then H[:, j]  = h)
+//   mode 1: w -= sum_i h_i v_i, dots w . v_i        (then H[:, j] += h)
+//   mode 2: w -= sum_i h_i v_i, ||w||^2, z = D^-1 w (then the Givens step)
+template <int J>
+__global__ __launch_bounds__(RT) void gm_orth(int N, double *w, const double *V, long ldv, int j,
+                                              const GmresState *st, double *part, int nb, int mode,
+                                              const double *dinv, double *z) {
+  if (st->cyc_stop) return;
+  double h[J], acc[J];
+#pragma unroll
+  for (int i = 0; i < J; i++) {
+    h[i] = (mode != 0 && i <= j) ? st->h[i] : 0.0;
+    acc[i] = 0.0;
+  }
+  for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += nb * RT) {
+    double v[J];
+    double s = w[r];
+#pragma unroll
+    for (int i = 0; i < J; i++)
+      if (i <= j) {
+        v[i] = V[(long)i * ldv + r];
+        s -= h[i] * v[i];
+      }
+    if (mode != 0) w[r] = s;
+    if (mode != 2) {
+#pragma unroll
+      for (int i = 0; i < J; i++)
+        if (i <= j) acc[i] += s * v[i];
+    } else {
+      acc[0] += s * s;
+      z[r] = dinv[r] * s;
+    }
+  }
+  block_store<J>(acc, mode == 2 ? 1 : j + 1, part, nb);
+}
+
 // y = H^-1 g on the cycle's upper triangle.
 __global__ __launch_bounds__(64) void gm_backsub(GmresState *st) {
   if (threadIdx.x != 0 || st->done) return;
@@ -399,55 +437,59 @@ __global__ __launch_bounds__(64) void gm_backsub(GmresState *st) {
 }
 
 // x += D^-1 V y.
+template <int J>
 __global__ __launch_bounds__(RT) void gm_xupdate(int N, double *x, const double *dinv, const double *V, long ldv,
                                                  const GmresState *st) {
   if (st->done) return;
   const int je = st->jend;
-  double y[MAXM];
+  double y[J];
 #pragma unroll
-  for (int i = 0; i < MAXM; i++) y[i] = i <= je ? st->y[i] : 0.0;
+  for (int i = 0; i < J; i++) y[i] = i <= je ? st->y[i] : 0.0;
   for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += gridDim.x * RT) {
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < MAXM; i++)
+    for (int i = 0; i < J; i++)
       if (i <= je) s += y[i] * V[(long)i * ldv + r];
     x[r] += dinv[r] * s;
   }
 }
 
-// r = b - A x, partial ||r||^2.
+// r = b - A x, z = D^-1 r, ||r||, then the restart test; LPR lanes per row.
+template <int LPR>
 __global__ __launch_bounds__(RT) void gm_resid(int N, const int *rp, const int *col, const double *val,
-                                               const double *x, const double *b, double *rvec, const GmresState *st,
-                                               double *part, int nb) {
+                                               const double *x, const double *b, const double *dinv, double *rvec,
+                                               double *z, const GmresState *st, double *part, int nb) {
   if (st->done) return;
-  double acc[MAXM];
-#pragma unroll
-  for (int i = 0; i < MAXM; i++) acc[i] = 0.0;
-  for (int r = blockIdx.x * RT + threadIdx.x; r < N; r += nb * RT) {
+  double acc[1] = {0.0};
+  const int lane = threadIdx.x & (LPR - 1);
+  for (int r = (blockIdx.x * RT + threadIdx.x) / LPR; r < N; r += nb * (RT / LPR)) {
     double s = 0.0;
-    for (int q = rp[r]; q < rp[r + 1]; q++) s += val[q] * x[col[q]];
-    const double rv = b[r] - s;
-    rvec[r] = rv;
-    acc[0] += rv * rv;
+    for (int q = rp[r] + lane; q < rp[r + 1]; q += LPR) s += val[q] * x[col[q]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) {
+      const double rv = b[r] - s;
+      rvec[r] = rv;
+      z[r] = dinv[r] * rv;
+      acc[0] += rv * rv;
+    }
   }
-  block_store(acc, 1, part, nb);
+  block_store<1>(acc, 1, part, nb);
 }
 
-__global__ __launch_bounds__(64) void gm_restart(const double *part, int nb, GmresState *st) {
-  if (st->done) return;
-  const double s = wave_total(part, 0, nb);
-  if (threadIdx.x != 0) return;
-  const double beta = sqrt(s);
-  st->beta = beta;
-  st->relres = st->r0 > 0.0 ? beta / st->r0 : 0.0;
-  const bool conv = beta <= st->tol * st->r0;
-  st->conv = conv ? 1 : 0;
-  st->done = (conv || st->iters >= st->max_iter) ? 1 : 0;
-  st->cyc_stop = st->done;
-  st->jend = -1;
-  for (int i = 0; i <= MAXM; i++) st->g[i] = 0.0;
-  st->g[0] = beta;
-  st->vscale = beta > 0.0 ? 1.0 / beta : 0.0;
+// Finisher launches of the reductions above.
+__global__ __launch_bounds__(64) void gm_start(const double *part, int nb, GmresState *st, int first) {
+  if (!first && st->done) return;
+  fin_start(st, part, nb, first != 0);
+}
+__global__ __launch_bounds__(RT) void gm_hfinal(const double *part, int nb, int cnt, GmresState *st, int j,
+                                                int pass) {
+  if (st->cyc_stop) return;
+  fin_h(st, part, nb, cnt, j, pass);
+}
+__global__ __launch_bounds__(64) void gm_givens(const double *part, int nb, GmresState *st, int j) {
+  if (st->cyc_stop) return;
+  fin_givens(st, part, nb, j);
 }
 
 // ------------------------------------------------------------- update
@@ -477,11 +519,14 @@ struct femx_ins {
   long ldv = 0;     // leading dimension of V
   std::vector<int> h_row_ptr, h_col;
   DevBuf<double4> vphi;
-  DevBuf<double> phid, iso, emat, erhs, val, b, dinv, x, rvec, w[2], V, part, dir_v, stage;
-  DevBuf<int> row_ptr, col, nz_row, c_ptr, c_idx, dir_flag;
+  int lpr = 4;      // lanes per row of the CSR kernels
+  DevBuf<double> phid, iso, emat, erhs, val, b, dinv, x, rvec, z, w[2], V, part, dir_v, stage;
+  DevBuf<int> row_ptr, col, nz_row, c_ptr, cpos, rpos, dir_flag;
+  DevBuf<int> econn;  // connectivity in the element kernel's processing order
   DevBuf<GmresState> gst;
   GmresCtl *ctl = nullptr;  // pinned host copy of the control words
   bool have_solution = false;
+  int last_iters = 0;  // Krylov iterations of the previous solve
   KernelTimer timer;
 };
 
@@ -598,15 +643,54 @@ InsConst ins_const(const femx_ins_params &p) {
 }
 
 double element_bytes(const Mesh &m) {
-  // conn + coordinates, velocity/phi and phid of each corner + |d| + element matrix and vector
+  // per element: conn, |d|, the scatter positions and the element matrix and
+  // vector; per node (read once): coordinates, velocity/phi, phid
   const double npe = m.npe;
-  return m.E * (4.0 * npe + 8.0 + 8.0 * npe * npe + 8.0 * npe) + m.N * (32.0 + 32.0 + 8.0);
+  return m.E * (4.0 * npe + 8.0 + 12.0 * npe * npe + 12.0 * npe) + m.N * (32.0 + 32.0 + 8.0);
 }
 
 femx_status read_ctl(femx_ins *s) {
   FEMX_HIP(hipMemcpyAsync(s->ctl, s->gst.p, sizeof(GmresCtl), hipMemcpyDeviceToHost, s->st));
   FEMX_HIP(hipStreamSynchronize(s->st));
   return FEMX_OK;
+}
+
+template <int LPR>
+void launch_spmv(femx_ins *s, const double *src, double *w, int j) {
+  const int N = s->mesh->N;
+  gm_spmv<LPR><<<grid_for((long)N * LPR, RT), RT, 0, s->st>>>(N, s->row_ptr.p, s->col.p, s->val.p, s->z.p, src,
+                                                                s->V.p, s->ldv, w, j, s->gst.p);
+}
+
+template <int LPR>
+void launch_resid(femx_ins *s) {
+  const int N = s->mesh->N;
+  gm_resid<LPR><<<s->nb, RT, 0, s->st>>>(N, s->row_ptr.p, s->col.p, s->val.p, s->x.p, s->b.p, s->dinv.p,
+                                         s->rvec.p, s->z.p, s->gst.p, s->part.p, s->nb);
+}
+
+// Krylov kernels sized for the basis in use: j < J.
+template <int J>
+void orth_j(femx_ins *s, double *w, int j, int mode) {
+  gm_orth<J><<<s->nb, RT, 0, s->st>>>(s->mesh->N, w, s->V.p, s->ldv, j, s->gst.p, s->part.p, s->nb, mode,
+                                       s->dinv.p, s->z.p);
+}
+void launch_orth(femx_ins *s, double *w, int j, int mode) {
+  if (j < 4) orth_j<4>(s, w, j, mode);
+  else if (j < 8) orth_j<8>(s, w, j, mode);
+  else if (j < 16) orth_j<16>(s, w, j, mode);
+  else orth_j<MAXM>(s, w, j, mode);
+}
+template <int J>
+void xupdate_j(femx_ins *s) {
+  gm_xupdate<J><<<s->nb, RT, 0, s->st>>>(s->mesh->N, s->x.p, s->dinv.p, s->V.p, s->ldv, s->gst.p);
+}
+// jmax: the last iteration enqueued in the cycle (the basis used is <= jmax + 1)
+void launch_xupdate(femx_ins *s, int jmax) {
+  if (jmax < 4) xupdate_j<4>(s);
+  else if (jmax < 8) xupdate_j<8>(s);
+  else if (jmax < 16) xupdate_j<16>(s);
+  else xupdate_j<MAXM>(s);
 }
 
 femx_status ins_solve(femx_ins *s, femx_ins_stats *stats) {
@@ -616,58 +700,71 @@ femx_status ins_solve(femx_ins *s, femx_ins_stats *stats) {
   {
     TimedLaunch tl(s->timer, s->timer.kind("ins_element", element_bytes(m)), s->st);
     if (m.npe == 8)
-      ins_element<8><<<grid_for(m.E, 256), 256, 0, s->st>>>(m.E, m.conn.p, m.xyz.p, s->vphi.p, s->phid.p,
-                                                            s->iso.p, k, s->emat.p, s->erhs.p);
+      ins_element<8><<<grid_for(m.E, 256), 256, 0, s->st>>>(m.E, s->econn.p, m.xyz.p, s->vphi.p, s->phid.p,
+                                                            s->iso.p, k, s->cpos.p, s->rpos.p, s->emat.p, s->erhs.p);
     else
-      ins_element<4><<<grid_for(m.E, 256), 256, 0, s->st>>>(m.E, m.conn.p, m.xyz.p, s->vphi.p, s->phid.p,
-                                                            s->iso.p, k, s->emat.p, s->erhs.p);
+      ins_element<4><<<grid_for(m.E, 256), 256, 0, s->st>>>(m.E, s->econn.p, m.xyz.p, s->vphi.p, s->phid.p,
+                                                            s->iso.p, k, s->cpos.p, s->rpos.p, s->emat.p, s->erhs.p);
   }
   {
-    const double bytes = s->nnz * (8.0 + 4.0 + 4.0 + 4.0 + 8.0) + (double)m.E * m.npe * m.npe * 12.0 + N * 12.0;
+    const double bytes = s->nnz * (4.0 + 4.0 + 4.0 + 8.0) + (double)m.E * m.npe * m.npe * 8.0 + N * 12.0;
     TimedLaunch tl(s->timer, s->timer.kind("ins_gather_mat", bytes), s->st);
-    ins_gather_mat<<<grid_for(s->nnz, 256), 256, 0, s->st>>>(s->nnz, s->c_ptr.p, s->c_idx.p, s->emat.p,
-                                                             s->nz_row.p, s->col.p, s->dir_flag.p, s->val.p,
-                                                             s->dinv.p);
+    ins_gather_mat<<<grid_for(s->nnz, RT), RT, 0, s->st>>>(s->nnz, s->c_ptr.p, s->emat.p, s->nz_row.p, s->col.p,
+                                                           s->dir_flag.p, s->val.p, s->dinv.p);
   }
   {
-    const double bytes = (double)m.E * m.npe * 12.0 + N * (4.0 + 4.0 + 8.0 + 8.0 * 3);
+    const double bytes = (double)m.E * m.npe * 8.0 + N * (4.0 + 4.0 + 8.0 + 8.0 + 8.0 * 4);
     TimedLaunch tl(s->timer, s->timer.kind("ins_gather_rhs", bytes), s->st);
-    ins_gather_rhs<<<nb, RT, 0, s->st>>>(N, m.g_off.p, m.g_slot.p, s->erhs.p, s->dir_flag.p, s->dir_v.p, s->b.p,
-                                         s->x.p, s->rvec.p, s->part.p, nb);
+    ins_gather_rhs<<<grid_for(N, RT), RT, 0, s->st>>>(N, m.g_off.p, s->erhs.p, s->dir_flag.p, s->dir_v.p, s->dinv.p,
+                                                      s->b.p, s->x.p, s->rvec.p, s->z.p, s->part.p);
   }
-  gm_start<<<1, 64, 0, s->st>>>(s->part.p, nb, s->gst.p, 1);
+  gm_start<<<1, 64, 0, s->st>>>(s->part.p, grid_for(N, RT), s->gst.p, 1);
   femx_status r;
-  if ((r = read_ctl(s)) != FEMX_OK) return r;
   const double spmv_bytes = s->nnz * 12.0 + N * (4.0 + 8.0 * 4);
-  while (!s->ctl->done) {
+  // host checks of the cycle flag: first where the previous solve stopped,
+  // then at the next iteration, then every CHECK iterations.  Work enqueued
+  // past the stop is skipped by the kernels themselves (cyc_stop / done).
+  int first_check = s->last_iters > 0 ? s->last_iters : CHECK;
+  do {
+    int next_check = std::min(first_check, M);
+    int jmax = 0;
     for (int j = 0; j < M; j++) {
       const double *src = j == 0 ? s->rvec.p : s->w[(j - 1) & 1].p;
       double *w = s->w[j & 1].p;
       {
-        TimedLaunch tl(s->timer, s->timer.kind("gm_spmv", spmv_bytes + N * 8.0 * j), s->st);
-        gm_spmv<<<nb, RT, 0, s->st>>>(N, s->row_ptr.p, s->col.p, s->val.p, s->dinv.p, src, s->V.p, s->ldv, w, j,
-                                      s->gst.p, s->part.p, nb);
+        TimedLaunch tl(s->timer, s->timer.kind("gm_spmv", spmv_bytes), s->st);
+        if (s->lpr == 8) launch_spmv<8>(s, src, w, j);
+        else launch_spmv<4>(s, src, w, j);
       }
-      for (int pass = 0; pass < 2; pass++) {
-        gm_hfinal<<<1, 256, 0, s->st>>>(s->part.p, nb, j + 1, s->gst.p, j, pass);
-        TimedLaunch tl(s->timer, s->timer.kind("gm_orth", N * 8.0 * (j + 3)), s->st);
-        gm_orth<<<nb, RT, 0, s->st>>>(N, w, s->V.p, s->ldv, j, s->gst.p, s->part.p, nb, pass);
+      {
+        TimedLaunch tl(s->timer, s->timer.kind("gm_orth", N * 8.0 * (j + 2)), s->st);
+        launch_orth(s, w, j, 0);
+      }
+      for (int mode = 1; mode <= 2; mode++) {
+        gm_hfinal<<<1, RT, 0, s->st>>>(s->part.p, nb, j + 1, s->gst.p, j, mode - 1);
+        TimedLaunch tl(s->timer, s->timer.kind("gm_orth", N * 8.0 * (j + 3) + (mode == 2 ? N * 16.0 : 0.0)),
+                       s->st);
+        launch_orth(s, w, j, mode);
       }
       gm_givens<<<1, 64, 0, s->st>>>(s->part.p, nb, s->gst.p, j);
-      if ((j + 1) % CHECK == 0 && j + 1 < M) {
+      jmax = j;
+      if (j + 1 == next_check && j + 1 < M) {
         if ((r = read_ctl(s)) != FEMX_OK) return r;
         if (s->ctl->cyc_stop) break;
+        next_check = j + 1 == first_check ? j + 2 : j + 1 + CHECK;
       }
     }
     gm_backsub<<<1, 64, 0, s->st>>>(s->gst.p);
-    gm_xupdate<<<nb, RT, 0, s->st>>>(N, s->x.p, s->dinv.p, s->V.p, s->ldv, s->gst.p);
-    gm_resid<<<nb, RT, 0, s->st>>>(N, s->row_ptr.p, s->col.p, s->val.p, s->x.p, s->b.p, s->rvec.p, s->gst.p,
-                                   s->part.p, nb);
-    gm_restart<<<1, 64, 0, s->st>>>(s->part.p, nb, s->gst.p);
+    launch_xupdate(s, jmax);
+    if (s->lpr == 8) launch_resid<8>(s);
+    else launch_resid<4>(s);
+    gm_start<<<1, 64, 0, s->st>>>(s->part.p, nb, s->gst.p, 0);
     if ((r = read_ctl(s)) != FEMX_OK) return r;
-  }
+    first_check = CHECK;
+  } while (!s->ctl->done);
   FEMX_HIP(hipGetLastError());
   s->have_solution = true;
+  s->last_iters = s->ctl->iters;
   if (stats) {
     stats->iterations = s->ctl->iters;
     stats->converged = s->ctl->conv;
@@ -730,12 +827,32 @@ extern "C" femx_status femx_ins_create(femx_mesh *mesh, const double *el_diff, i
   std::vector<int> nz_row, c_ptr, c_idx;
   host_pattern(m, n_off, n_slot, s->h_row_ptr, s->h_col, nz_row, c_ptr, c_idx);
   s->nnz = s->h_row_ptr[N];
+  // where each element value goes: matrix entry contributions and node
+  // contributions in the gathers' (ascending element) order
+  std::vector<int> cpos(c_idx.size()), rpos((size_t)E * npe);
+  for (size_t q = 0; q < c_idx.size(); q++) cpos[c_idx[q]] = (int)q;
+  for (size_t j = 0; j < n_slot.size(); j++) rpos[n_slot[j]] = (int)j;
+  c_idx.clear();
+  c_idx.shrink_to_fit();
+  s->lpr = (double)s->nnz / N > 20.0 ? 8 : 4;
+  // the element kernel's inputs in processing order: the Morton block order
+  // of the mesh (element t is original element ord[t])
+  const std::vector<int> &ord = m.blk.perm;
+  if ((long)ord.size() != E) return ins_fail(s, (set_error("femx_ins_create: mesh has no element order"), FEMX_ERR_ARG));
   std::vector<double> iso(E, 0.0);
-  if (el_diff)
-    for (int e = 0; e < E; e++) {
+  std::vector<int> econn((size_t)E * npe), cpos_o(cpos.size()), rpos_o(rpos.size());
+  for (int t = 0; t < E; t++) {
+    const int e = ord[t];
+    if (el_diff) {
       const double *d = el_diff + 3L * e;
-      iso[e] = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      iso[t] = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     }
+    for (int a = 0; a < npe; a++) {
+      econn[(size_t)t * npe + a] = m.h_conn[(size_t)e * npe + a];
+      rpos_o[(size_t)t * npe + a] = rpos[(size_t)e * npe + a];
+    }
+    for (int q = 0; q < npe * npe; q++) cpos_o[(size_t)t * npe * npe + q] = cpos[(size_t)e * npe * npe + q];
+  }
   std::vector<int> dflag(N, 0);
   std::vector<double> dv(N, 0.0);
   for (int i = 0; i < n_dir; i++) {
@@ -749,7 +866,9 @@ extern "C" femx_status femx_ins_create(femx_mesh *mesh, const double *el_diff, i
       (r = s->col.upload(s->h_col.data(), s->nnz, s->st)) != FEMX_OK ||
       (r = s->nz_row.upload(nz_row.data(), s->nnz, s->st)) != FEMX_OK ||
       (r = s->c_ptr.upload(c_ptr.data(), c_ptr.size(), s->st)) != FEMX_OK ||
-      (r = s->c_idx.upload(c_idx.data(), c_idx.size(), s->st)) != FEMX_OK ||
+      (r = s->cpos.upload(cpos_o.data(), cpos_o.size(), s->st)) != FEMX_OK ||
+      (r = s->rpos.upload(rpos_o.data(), rpos_o.size(), s->st)) != FEMX_OK ||
+      (r = s->econn.upload(econn.data(), econn.size(), s->st)) != FEMX_OK || (r = s->z.alloc(N)) != FEMX_OK ||
       (r = s->iso.upload(iso.data(), E, s->st)) != FEMX_OK ||
       (r = s->dir_flag.upload(dflag.data(), N, s->st)) != FEMX_OK ||
       (r = s->dir_v.upload(dv.data(), N, s->st)) != FEMX_OK || (r = s->vphi.alloc(N)) != FEMX_OK ||
@@ -758,7 +877,7 @@ extern "C" femx_status femx_ins_create(femx_mesh *mesh, const double *el_diff, i
       (r = s->b.alloc(N)) != FEMX_OK || (r = s->dinv.alloc(N)) != FEMX_OK || (r = s->x.alloc(N)) != FEMX_OK ||
       (r = s->rvec.alloc(N)) != FEMX_OK || (r = s->w[0].alloc(N)) != FEMX_OK || (r = s->w[1].alloc(N)) != FEMX_OK ||
       (r = s->V.alloc((size_t)s->ldv * (M + 1))) != FEMX_OK ||
-      (r = s->part.alloc((size_t)MAXM * s->nb)) != FEMX_OK || (r = s->stage.alloc((size_t)N * 3)) != FEMX_OK ||
+      (r = s->part.alloc((size_t)MAXM * std::max(s->nb, grid_for(N, RT)))) != FEMX_OK || (r = s->stage.alloc((size_t)N * 3)) != FEMX_OK ||
       (r = s->gst.alloc(1)) != FEMX_OK)
     return ins_fail(s, r);
   GmresState h0;

@@ -7,7 +7,7 @@
 //
 // The reference's Navier-Stokes stage (stage 0 without PRESCRIBEDVEL) has
 // no right-hand side (formNS_RHS is empty, src/femElement.cpp:1839-1849)
-// and no solve (commented out, :117).  Its sparse-matrix assembly throws
+// and no solve (commented out, src/femIncompressibleSolver.cpp:121).  Its sparse-matrix assembly throws
 // "Not Implemented." (src/femMatrix.cpp:307-309).  Here that stage throws
 // the same femException after the reference's printed lines.
 #pragma once

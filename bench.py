@@ -296,9 +296,15 @@ def workload_serve(args):
         span = max(s_["t_done"] for s_ in st) - min(s_["t_submit"] for s_ in st)
         done = [s_ for s_ in st if s_["status"] in (0, femx.EXIT_CONDITION)]
         es = sum(s_["elements"] * s_["steps_done"] for s_ in done)
+        # steady state: completions between the 10th and 90th percentile of
+        # the completion times, over that window (no ramp-up, no drain)
+        td = np.sort([s_["t_done"] for s_ in st])
+        lo, hi = td[len(td) // 10], td[(9 * len(td)) // 10]
+        steady = ((td > lo) & (td <= hi)).sum() / (hi - lo) if hi > lo else float("nan")
         table[str(int(rate))] = {"requests": len(st), "ok": len(done), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
                                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                                 "req_per_s": round(len(st) / span, 2), "element_steps_per_s": round(es / span, 1),
+                                 "req_per_s": round(len(st) / span, 2), "steady_req_per_s": round(float(steady), 2),
+                                 "element_steps_per_s": round(es / span, 1),
                                  "offered_req_per_s": round(len(st) / float(arr[-1]), 2),
                                  "mean_ms": {"queue": round(float(wait.mean()) * 1e3, 2),
                                              "prep": round(float(prep.mean()) * 1e3, 2),
@@ -561,7 +567,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ins-elem", choices=["tet4", "hexa8"], default="tet4", help="ins: element type")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
-    ap.add_argument("--requests", type=int, default=40, help="requests per rate (serve)")
+    ap.add_argument("--requests", type=int, default=120, help="requests per rate (serve)")
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--max-active", type=int, default=64)
     ap.add_argument("--prep-threads", type=int, default=8)

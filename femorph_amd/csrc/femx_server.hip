@@ -63,9 +63,13 @@ struct Request {
   femx_vms *vh = nullptr;
   femx_supg *sh = nullptr;
   hipStream_t st = nullptr;
-  hipEvent_t ev = nullptr;
-  int done = 0, last_k = 0;
-  bool inflight = false;
+  // chunks in flight on the request's stream, oldest first (a ring of two):
+  // VMS keeps two queued so the GPU never waits for the next chunk's
+  // launches; SUPG keeps one (its exit check synchronises the stream)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int chunk_k[2] = {0, 0};
+  int head = 0, n_inflight = 0;
+  int done = 0, issued = 0;
   femx_request_stats stats{};
   std::vector<double> result;
 
@@ -85,8 +89,10 @@ struct Request {
     return b;
   }
   void release() {
-    if (ev) (void)hipEventDestroy(ev);
-    ev = nullptr;
+    for (auto &e : ev) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
     femx_vms_destroy(vh);
     femx_supg_destroy(sh);
     femx_mesh_free(mesh);
@@ -181,7 +187,8 @@ void femx_server::prep_loop() {
         }
       }
     }
-    if (st == FEMX_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) st = FEMX_ERR_HIP;
+    for (auto &e : r->ev)
+      if (st == FEMX_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) st = FEMX_ERR_HIP;
     if (st != FEMX_OK) {
       fail(r, st);
       continue;
@@ -218,33 +225,44 @@ void femx_server::gpu_loop(int t) {
       auto &r = active[i];
       femx_status st = FEMX_OK;
       bool over = false;
-      if (r->inflight) {
-        if (hipEventQuery(r->ev) == hipErrorNotReady) {
-          i++;
-          continue;
-        }
-        r->inflight = false;
+      // retire finished chunks, oldest first
+      while (r->n_inflight > 0 && hipEventQuery(r->ev[r->head]) != hipErrorNotReady) {
+        const int k = r->chunk_k[r->head];
+        r->head ^= 1;
+        r->n_inflight--;
+        changed = true;
         if (r->kind == FEMX_REQ_SUPG) {
           int d = 0;
           st = femx_supg_finish(r->sh, &d);  // the exit condition of the chunk
           r->done += d;
-          over = st != FEMX_OK;
+          if (st != FEMX_OK) {
+            over = true;
+            break;
+          }
         } else {
-          r->done += r->last_k;
+          r->done += k;
         }
       }
-      if (!over && r->done < r->steps) {
-        const int k = std::min(chunk, r->steps - r->done);
+      const int depth = r->kind == FEMX_REQ_VMS ? 2 : 1;
+      while (!over && r->n_inflight < depth && r->issued < r->steps) {
+        const int k = std::min(chunk, r->steps - r->issued);
         st = r->kind == FEMX_REQ_VMS ? femx_vms_step(r->vh, k) : femx_supg_step_async(r->sh, k);
-        if (st == FEMX_OK && hipEventRecord(r->ev, r->st) == hipSuccess) {
-          r->last_k = k;
-          r->inflight = true;
-          changed = true;
-          i++;
-          continue;
+        const int slot = (r->head + r->n_inflight) & 1;
+        if (st == FEMX_OK && hipEventRecord(r->ev[slot], r->st) != hipSuccess) st = FEMX_ERR_HIP;
+        if (st != FEMX_OK) {
+          over = true;
+          break;
         }
-        if (st == FEMX_OK) st = FEMX_ERR_HIP;
+        r->chunk_k[slot] = k;
+        r->n_inflight++;
+        r->issued += k;
+        changed = true;
       }
+      if (!over && (r->n_inflight > 0 || r->done < r->steps)) {
+        i++;
+        continue;
+      }
+      if (over && st == FEMX_OK) st = FEMX_ERR_HIP;
       // finished (or failed): read the final state back, free the handles and the budget
       const femx_status end_st = st;
       if (st == FEMX_OK || st == FEMX_EXIT_CONDITION) {

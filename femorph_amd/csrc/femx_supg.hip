@@ -457,6 +457,7 @@ __global__ __launch_bounds__(256) void supg_node_diag(int N, const double4 *U, d
 // registers, into one partial row; one wave folds the partial rows with all
 // of its loads in flight together.
 constexpr int DIAG_PARTS = 32;
+constexpr int DIAG_ONE_PASS = 2048;  // up to this many rows, one workgroup and one launch
 __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int ne_blk, const double *nblk,
                                                          int nn_blk, int with_elem, double *part, const int *halt) {
   if (halted(halt)) return;
@@ -901,9 +902,15 @@ femx_status supg_enqueue(femx_supg *s, int nsteps) {
     s->press_pending = true;
     const int nnb = grid_for(s->mesh->N, 256);
     supg_node_diag<<<nnb, 256, 0, s->st>>>(s->mesh->N, s->U[s->cur].p, s->nblk.p, s->halt.p);
-    supg_diag_partial<<<DIAG_PARTS, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb,
-                                                     s->diag_enabled ? 1 : 0, s->dpart.p, s->halt.p);
-    supg_diag_final<<<1, 64, 0, s->st>>>(s->dpart.p, DIAG_PARTS, row, s->halt.p);
+    if (nnb + s->mesh->blk.nb <= DIAG_ONE_PASS) {
+      // few rows: one workgroup folds them straight into the log row
+      supg_diag_partial<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0,
+                                              row, s->halt.p);
+    } else {
+      supg_diag_partial<<<DIAG_PARTS, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb,
+                                                       s->diag_enabled ? 1 : 0, s->dpart.p, s->halt.p);
+      supg_diag_final<<<1, 64, 0, s->st>>>(s->dpart.p, DIAG_PARTS, row, s->halt.p);
+    }
     FEMX_HIP(hipGetLastError());
     s->pend_n++;
   }

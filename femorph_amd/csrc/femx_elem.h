@@ -18,6 +18,7 @@ namespace {
 __constant__ double cN8[8][8], cdN8[8][8][3];
 __constant__ double cN4[4][4], cdN4[4][4][3];
 __constant__ double cW8, cW4;
+__constant__ double cGP8[8][3];  // hexa8 Gauss point coordinates (xi, eta, zeta)
 
 template <int NPE>
 struct Shape;
@@ -116,6 +117,7 @@ femx_status upload_tables(int dev) {
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cdN4), dN4, sizeof(dN4)));
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW8), &w8, sizeof(w8)));
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW4), &w4, sizeof(w4)));
+  FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cGP8), gp8, sizeof(gp8)));
   if (dev >= 0 && dev < 64) g_tables_ready[dev] = true;
   return FEMX_OK;
 }

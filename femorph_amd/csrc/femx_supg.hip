@@ -206,33 +206,103 @@ __global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) 
   __syncthreads();
   if (has_e) {
     const SupgConst &k = a.k;
-    double dx[NPE][3];
-    double U[NPE][3];
+    // Element kinematics in local coordinates: the physical derivatives
+    // DN = invJ dN/dxi are never formed.  Every term the reference builds
+    // from DN (velocity gradient, divergence, B-hat strain, the K2/K4 node
+    // forces, u.grad N) is the same arithmetic regrouped through the local
+    // derivatives of the table and 3x3 products per Gauss point.
+    double U0[3], dU[NPE][3];  // node velocities relative to node 0 (dU[0] = 0)
+    U0[0] = su[id[0]];
+    U0[1] = sv[id[0]];
+    U0[2] = sw[id[0]];
 #pragma unroll
     for (int n = 0; n < NPE; n++) {
-      dx[n][0] = sx[id[n]] - sx[id[0]];
-      dx[n][1] = sy[id[n]] - sy[id[0]];
-      dx[n][2] = sz[id[n]] - sz[id[0]];
-      U[n][0] = su[id[n]];
-      U[n][1] = sv[id[n]];
-      U[n][2] = sw[id[n]];
+      dU[n][0] = su[id[n]] - U0[0];
+      dU[n][1] = sv[id[n]] - U0[1];
+      dU[n][2] = sw[id[n]] - U0[2];
     }
-    // eval_avg_shapeDeriv (:660-707): volume average of DN (hexa only)
-    double avg[NPE][3];
+    // hexa8: the trilinear map's Jacobian rows are J_a = E_a + (xi terms),
+    // from 7 coefficient vectors (femHexa8 node signs, src/femHexa8.cpp:15-17);
+    // tet4 (femTetra4, N3 = 1 - sum xi): constant, J_a = x_a - x_3.
+    double E1[3], E2[3], E3[3], E12[3], E13[3], E23[3], E123[3], Jc[3][3];
+    {
+      double dx[NPE][3];
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        dx[n][0] = sx[id[n]] - sx[id[0]];
+        dx[n][1] = sy[id[n]] - sy[id[0]];
+        dx[n][2] = sz[id[n]] - sz[id[0]];
+      }
+      if constexpr (NPE == 8) {
+        constexpr double s1[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, s2[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                         s3[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          double e1 = 0.0, e2 = 0.0, e3 = 0.0, e12 = 0.0, e13 = 0.0, e23 = 0.0, e123 = 0.0;
+#pragma unroll
+          for (int n = 1; n < 8; n++) {
+            e1 += s1[n] * dx[n][b];
+            e2 += s2[n] * dx[n][b];
+            e3 += s3[n] * dx[n][b];
+            e12 += s1[n] * s2[n] * dx[n][b];
+            e13 += s1[n] * s3[n] * dx[n][b];
+            e23 += s2[n] * s3[n] * dx[n][b];
+            e123 += s1[n] * s2[n] * s3[n] * dx[n][b];
+          }
+          E1[b] = 0.125 * e1;
+          E2[b] = 0.125 * e2;
+          E3[b] = 0.125 * e3;
+          E12[b] = 0.125 * e12;
+          E13[b] = 0.125 * e13;
+          E23[b] = 0.125 * e23;
+          E123[b] = 0.125 * e123;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) Jc[r][b] = dx[r][b] - dx[3][b];
+      }
+    }
+    auto jacobian = [&](int g, double J[3][3]) {
+      if constexpr (NPE == 8) {
+        const double xi = cGP8[g][0], et = cGP8[g][1], ze = cGP8[g][2];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          J[0][b] = E1[b] + et * E12[b] + ze * E13[b] + (et * ze) * E123[b];
+          J[1][b] = E2[b] + xi * E12[b] + ze * E23[b] + (xi * ze) * E123[b];
+          J[2][b] = E3[b] + xi * E13[b] + et * E23[b] + (xi * et) * E123[b];
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) J[r][b] = Jc[r][b];
+      }
+    };
+    // eval_avg_shapeDeriv (:660-707): volume average of DN (hexa only);
+    // with it the B-hat strain needs P_k = sum_n avg_nk u_nk per element
+    double avg[NPE][3], P[3] = {0.0, 0.0, 0.0};
     if (BHAT) {
 #pragma unroll
       for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = 0.0;
       double den = 0.0;
       for (int g = 0; g < NGP; g++) {
-        double DN[NPE][3], detJ, iJ[3][3];
-        kinematics<NPE>(g, dx, DN, detJ, iJ);
+        double J[3][3], iJ[3][3], detJ;
+        jacobian(g, J);
+        inv3(J, iJ, detJ);
         const double dv = detJ * Shape<NPE>::W();
+        double Q[3][3];
 #pragma unroll
-        for (int n = 0; n < NPE; n++) {
-          avg[n][0] += DN[n][0] * dv;
-          avg[n][1] += DN[n][1] * dv;
-          avg[n][2] += DN[n][2] * dv;
-        }
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) Q[b][c] = iJ[b][c] * dv;
+#pragma unroll
+        for (int n = 0; n < NPE; n++)
+#pragma unroll
+          for (int b = 0; b < 3; b++)
+            avg[n][b] += Q[b][0] * Shape<NPE>::dN(g, n, 0) + Q[b][1] * Shape<NPE>::dN(g, n, 1) +
+                         Q[b][2] * Shape<NPE>::dN(g, n, 2);
         den += dv;
       }
       const double iden = 1.0 / den;
@@ -241,6 +311,9 @@ __global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) 
         avg[n][0] *= iden;
         avg[n][1] *= iden;
         avg[n][2] *= iden;
+        P[0] += avg[n][0] * dU[n][0];  // sum_n avg_n = 0: relative velocities suffice
+        P[1] += avg[n][1] * dU[n][1];
+        P[2] += avg[n][2] * dU[n][2];
       }
     }
     T out[NPE][3];
@@ -248,28 +321,68 @@ __global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) 
     for (int n = 0; n < NPE; n++) out[n][0] = out[n][1] = out[n][2] = (T)0;
     // per-thread diagnostic accumulators, columns D_TAU .. D_NU+1
     double vol = 0.0, lam_int = 0.0, reg = 0.0, pnum = 0.0;
+    double Aacc[3] = {0.0, 0.0, 0.0};  // sum over Gauss points of f2 alpha (the B-hat avg term)
     const double nu_k = k.mu / k.rho;
+    // tet4: the Jacobian, its inverse and the metric are the same at every Gauss point
+    double iJc[3][3], detJc = 0.0;
+    if constexpr (NPE == 4) inv3(Jc, iJc, detJc);
     for (int g = 0; g < NGP; g++) {
-      double DN[NPE][3], detJ, iJ[3][3];
-      kinematics<NPE>(g, dx, DN, detJ, iJ);
+      double iJ[3][3], detJ;
+      if constexpr (NPE == 8) {
+        double J[3][3];
+        jacobian(g, J);
+        inv3(J, iJ, detJ);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) iJ[r][c] = iJc[r][c];
+        detJ = detJc;
+      }
       const double dv = detJ * Shape<NPE>::W();
       vol += dv;
-      double ug[3] = {0.0, 0.0, 0.0}, div = 0.0;
+      // u at the Gauss point and the velocity gradient gU[b][i] = du_i/dx_b
+      double ug[3] = {U0[0], U0[1], U0[2]}, L[3][3];
 #pragma unroll
-      for (int n = 0; n < NPE; n++) {
+      for (int c = 0; c < 3; c++) L[c][0] = L[c][1] = L[c][2] = 0.0;
+#pragma unroll
+      for (int n = 1; n < NPE; n++) {
         const double Nn = Shape<NPE>::N(g, n);
-        ug[0] += Nn * U[n][0];
-        ug[1] += Nn * U[n][1];
-        ug[2] += Nn * U[n][2];
-        div += DN[n][0] * (U[n][0] - U[0][0]) + DN[n][1] * (U[n][1] - U[0][1]) + DN[n][2] * (U[n][2] - U[0][2]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) ug[i] += Nn * dU[n][i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double d = Shape<NPE>::dN(g, n, c);
+          L[c][0] += d * dU[n][0];
+          L[c][1] += d * dU[n][1];
+          L[c][2] += d * dU[n][2];
+        }
       }
+      double gU[3][3];
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) gU[b][i] = iJ[b][0] * L[0][i] + iJ[b][1] * L[1][i] + iJ[b][2] * L[2][i];
+      const double div = gU[0][0] + gU[1][1] + gU[2][2];
       if (PRESS) {
         if (PSEP) {
+          double Lp[3][3];
+#pragma unroll
+          for (int c = 0; c < 3; c++) Lp[c][0] = Lp[c][1] = Lp[c][2] = 0.0;
+#pragma unroll
+          for (int n = 1; n < NPE; n++) {
+            const double q0 = pu[id[n]] - pu[id[0]], q1 = pv[id[n]] - pv[id[0]], q2 = pw[id[n]] - pw[id[0]];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              const double d = Shape<NPE>::dN(g, n, c);
+              Lp[c][0] += d * q0;
+              Lp[c][1] += d * q1;
+              Lp[c][2] += d * q2;
+            }
+          }
           double dp = 0.0;
 #pragma unroll
-          for (int n = 0; n < NPE; n++)
-            dp += DN[n][0] * (pu[id[n]] - pu[id[0]]) + DN[n][1] * (pv[id[n]] - pv[id[0]]) +
-                  DN[n][2] * (pw[id[n]] - pw[id[0]]);
+          for (int b = 0; b < 3; b++) dp += iJ[b][0] * Lp[0][b] + iJ[b][1] * Lp[1][b] + iJ[b][2] * Lp[2][b];
           pnum += dp * dv;
         } else {
           pnum += div * dv;
@@ -318,36 +431,28 @@ __global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) 
         dg[2] = fmin(dg[2], lam);
         dg[3] = fmax(dg[3], lam);
       }
-      // matrix-free K u contributions at this Gauss point
-      double c[3] = {0.0, 0.0, 0.0};  // (u_g . grad) u
+      // (u_g . grad) u
+      double c[3];
 #pragma unroll
-      for (int n = 0; n < NPE; n++) {
-        const double ud = ug[0] * DN[n][0] + ug[1] * DN[n][1] + ug[2] * DN[n][2];
-        c[0] += ud * (U[n][0] - U[0][0]);
-        c[1] += ud * (U[n][1] - U[0][1]);
-        c[2] += ud * (U[n][2] - U[0][2]);
+      for (int i = 0; i < 3; i++) c[i] = ug[0] * gU[0][i] + ug[1] * gU[1][i] + ug[2] * gU[2][i];
+      // strain B u (:222-296): sum_n DN_nb u_ni = gU[b][i]; with B-hat the
+      // diagonal uses b_n = (avg_n - DN_n)/3, whose sums are B_k = (P_k - D_k)/3
+      double eps[6];
+      if (BHAT) {
+        const double B0 = (P[0] - gU[0][0]) * (1.0 / 3.0), B1 = (P[1] - gU[1][1]) * (1.0 / 3.0),
+                     B2 = (P[2] - gU[2][2]) * (1.0 / 3.0);
+        const double Bs = B0 + B1 + B2;
+        eps[0] = gU[0][0] - 2.0 * B0 + Bs;
+        eps[1] = gU[1][1] - 2.0 * B1 + Bs;
+        eps[2] = gU[2][2] - 2.0 * B2 + Bs;
+      } else {
+        eps[0] = gU[0][0];
+        eps[1] = gU[1][1];
+        eps[2] = gU[2][2];
       }
-      double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int n = 0; n < NPE; n++) {
-        const double u0 = U[n][0], u1 = U[n][1], u2 = U[n][2];
-        const double d0 = DN[n][0], d1 = DN[n][1], d2 = DN[n][2];
-        if (BHAT) {
-          const double b0 = (1.0 / 3.0) * (avg[n][0] - d0), b1 = (1.0 / 3.0) * (avg[n][1] - d1),
-                       b2 = (1.0 / 3.0) * (avg[n][2] - d2);
-          const double sn = b0 * u0 + b1 * u1 + b2 * u2;
-          eps[0] += (d0 - 2.0 * b0) * u0 + sn;
-          eps[1] += (d1 - 2.0 * b1) * u1 + sn;
-          eps[2] += (d2 - 2.0 * b2) * u2 + sn;
-        } else {
-          eps[0] += d0 * u0;
-          eps[1] += d1 * u1;
-          eps[2] += d2 * u2;
-        }
-        eps[3] += d1 * u0 + d0 * u1;
-        eps[4] += d2 * u1 + d1 * u2;
-        eps[5] += d2 * u0 + d0 * u2;
-      }
+      eps[3] = gU[1][0] + gU[0][1];
+      eps[4] = gU[2][1] + gU[1][2];
+      eps[5] = gU[2][0] + gU[0][2];
       const double trE = eps[0] + eps[1] + eps[2];
       const double s0 = 2.0 * k.mu * eps[0] + lam * trE, s1 = 2.0 * k.mu * eps[1] + lam * trE,
                    s2 = 2.0 * k.mu * eps[2] + lam * trE;
@@ -355,30 +460,42 @@ __global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) 
       const double ssum = s0 + s1 + s2;
       const double f1 = k.k1 ? k.rho * dv : 0.0, f3 = k.k3 ? k.rho * tau * dv : 0.0;
       const double f4 = k.k4 ? k.rho * nuC * div * dv : 0.0, f2 = k.k2 ? dv : 0.0;
+      // node forces (K2 u)_n + f4 DN_n = (f2 S + f4 I) DN_n (+ f2 avg_n o alpha
+      // with B-hat), with S the stress matrix (B-hat: diagonal (5 s_k - ssum)/3,
+      // alpha_k = (ssum - 2 s_k)/3); DN_n = invJ dN_n gives Mg = (f2 S + f4 I) invJ
+      const double sd0 = BHAT ? (5.0 * s0 - ssum) * (1.0 / 3.0) : s0;
+      const double sd1 = BHAT ? (5.0 * s1 - ssum) * (1.0 / 3.0) : s1;
+      const double sd2 = BHAT ? (5.0 * s2 - ssum) * (1.0 / 3.0) : s2;
+      if (BHAT) {
+        Aacc[0] += f2 * ((ssum - 2.0 * s0) * (1.0 / 3.0));
+        Aacc[1] += f2 * ((ssum - 2.0 * s1) * (1.0 / 3.0));
+        Aacc[2] += f2 * ((ssum - 2.0 * s2) * (1.0 / 3.0));
+      }
+      const double A[3][3] = {{f2 * sd0 + f4, f2 * s3, f2 * s5},
+                              {f2 * s3, f2 * sd1 + f4, f2 * s4},
+                              {f2 * s5, f2 * s4, f2 * sd2 + f4}};
+      double Mg[3][3], w3[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) Mg[i][cc] = A[i][0] * iJ[0][cc] + A[i][1] * iJ[1][cc] + A[i][2] * iJ[2][cc];
+        w3[i] = iJ[0][i] * ug[0] + iJ[1][i] * ug[1] + iJ[2][i] * ug[2];  // u.grad N_n = w3 . dN_n
+      }
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
-        const double d0 = DN[n][0], d1 = DN[n][1], d2 = DN[n][2];
-        const double Nn = Shape<NPE>::N(g, n);
-        const double ud = ug[0] * d0 + ug[1] * d1 + ug[2] * d2;
-        double k2x, k2y, k2z;
-        if (BHAT) {
-          const double b0 = (1.0 / 3.0) * (avg[n][0] - d0), b1 = (1.0 / 3.0) * (avg[n][1] - d1),
-                       b2 = (1.0 / 3.0) * (avg[n][2] - d2);
-          k2x = b0 * ssum + (d0 - 2.0 * b0) * s0;
-          k2y = b1 * ssum + (d1 - 2.0 * b1) * s1;
-          k2z = b2 * ssum + (d2 - 2.0 * b2) * s2;
-        } else {
-          k2x = d0 * s0;
-          k2y = d1 * s1;
-          k2z = d2 * s2;
-        }
-        k2x += d1 * s3 + d2 * s5;
-        k2y += d0 * s3 + d2 * s4;
-        k2z += d1 * s4 + d0 * s5;
-        const double cv = f1 * Nn + f3 * ud;
-        out[n][0] += (T)(cv * c[0] + f2 * k2x + f4 * d0);
-        out[n][1] += (T)(cv * c[1] + f2 * k2y + f4 * d1);
-        out[n][2] += (T)(cv * c[2] + f2 * k2z + f4 * d2);
+        const double d0 = Shape<NPE>::dN(g, n, 0), d1 = Shape<NPE>::dN(g, n, 1), d2 = Shape<NPE>::dN(g, n, 2);
+        const double cv = f1 * Shape<NPE>::N(g, n) + f3 * (w3[0] * d0 + w3[1] * d1 + w3[2] * d2);
+        out[n][0] += (T)(cv * c[0] + (Mg[0][0] * d0 + Mg[0][1] * d1 + Mg[0][2] * d2));
+        out[n][1] += (T)(cv * c[1] + (Mg[1][0] * d0 + Mg[1][1] * d1 + Mg[1][2] * d2));
+        out[n][2] += (T)(cv * c[2] + (Mg[2][0] * d0 + Mg[2][1] * d1 + Mg[2][2] * d2));
+      }
+    }
+    if (BHAT && DYN) {
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        out[n][0] += (T)(avg[n][0] * Aacc[0]);
+        out[n][1] += (T)(avg[n][1] * Aacc[1]);
+        out[n][2] += (T)(avg[n][2] * Aacc[2]);
       }
     }
     if (PRESS) a.pres[e] = -lam_prev * (pnum / vol);  // compute_pressure of the previous step (:498)

@@ -159,8 +159,11 @@ __device__ __forceinline__ void supg_node(const double F[3], double4 fprev, doub
   fstore = make_double4(F[0], F[1], F[2], 0.0);
 }
 
+// Two waves per SIMD: the hexa8 variants then spill ~140-240 B per lane to
+// scratch, which costs less than running one wave per SIMD (0.58 vs 0.77 ms
+// per launch on the 2.1M-hexa cavity).
 template <typename T, int NPE, bool BHAT, bool DYN, bool PRESS, bool PSEP, bool DIAG>
-__global__ __launch_bounds__(threads_for<NPE>()) void supg_block(SupgArgs<T> a) {
+__global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> a) {
   constexpr int NGP = Shape<NPE>::NGP;
   constexpr int SB = threads_for<NPE>();
   __shared__ double sx[256], sy[256], sz[256], sm[256];

@@ -313,11 +313,16 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   }
 }
 
+constexpr int BC_BIT = 0x8000;
+
 template <typename T>
 struct BlockArgs {
   int nblk;
   const int *elem_off, *node_off, *loc_node, *loc_pslot, *lcsr_off;
-  const ushort4 *lpos;  // per element: node-sorted position of each corner's contribution
+  // per element: node-sorted position of each corner's contribution (bits
+  // 0..14) and BC_BIT when the corner's node has a BC record, so its pre-BC
+  // and post-BC states may differ
+  const ushort4 *lpos;
   const uchar4 *lconn;
   const double4 *xyzm;  // x y z, lumped nodal mass
   const double4 *Q, *P; // post-BC / pre-BC state of the previous iteration
@@ -416,7 +421,6 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   __shared__ double2 snd[F32 ? 1 : VN][5];
   __shared__ float4 sfa[F32 ? VN : 1], sfb[F32 ? VN : 1];
   __shared__ float2 sfc[F32 ? VN : 1];
-  __shared__ int sbc[VN];  // 1 for nodes with a BC record (pre-BC and post-BC states may differ)
   // element-corner contributions (u v w p) in the block's node-sorted order,
   // one T4 record per corner at a bank-swizzled position
   __shared__ T4 sres[RES ? 4 * VB : 1];
@@ -494,7 +498,6 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       snd[j][3] = make_double2(q_own.x, q_own.y);
       snd[j][4] = make_double2(q_own.z, q_own.w);
     }
-    sbc[j] = rec_own >= 0;
   }
   double q4[4];
   if (PROJ) {
@@ -540,7 +543,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
           uq[n][2] = rc.x;
           p[n] = rc.y;
         }
-        const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
+        const bool reuse = PROJ && !ACC && !((pos.x | pos.y | pos.z | pos.w) & BC_BIT);
         if (!reuse) {
           const float qref[3] = {(float)qr.x, (float)qr.y, (float)qr.z};
           elem_vel_rel(DN, qref, uq, ev);
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
         }
         // without BC nodes the post-BC velocities equal the pre-BC ones the
         // projection just used: reuse their gradient
-        const bool reuse = PROJ && !ACC && !(sbc[id[0]] | sbc[id[1]] | sbc[id[2]] | sbc[id[3]]);
+        const bool reuse = PROJ && !ACC && !((pos.x | pos.y | pos.z | pos.w) & BC_BIT);
         if (!reuse) elem_vel<T>(DN, uq, ev);
         if (ACC) {
 #pragma unroll
@@ -619,7 +622,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
           a.diag_sdu[eo * 3 + i] = sm;
         }
       }
-      const int ps[4] = {pos.x, pos.y, pos.z, pos.w};
+      const int ps[4] = {pos.x & ~BC_BIT, pos.y & ~BC_BIT, pos.z & ~BC_BIT, pos.w & ~BC_BIT};
 #pragma unroll
       for (int b = 0; b < 4; b++) {
         T4 v;
@@ -882,6 +885,7 @@ struct femx_vms {
   int n_bmax = 0;
   DevBuf<double> scratch;  // staging of reference-layout arrays
   DevBuf<int> rec_node;
+  DevBuf<uint16_t> lpos_bc;  // the mesh's corner positions with BC_BIT set per the BC records
   int cur = 0;
   bool proj_pending = false;
   long iter = 0;
@@ -951,7 +955,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.loc_node = b.d_loc_node.p;
   a.loc_pslot = b.d_loc_pslot.p;
   a.lcsr_off = b.d_lcsr_off.p;
-  a.lpos = reinterpret_cast<const ushort4 *>(b.d_lcsr.p);
+  a.lpos = reinterpret_cast<const ushort4 *>(s->lpos_bc.p);
   a.lconn = reinterpret_cast<const uchar4 *>(b.d_lconn.p);
   a.xyzm = s->xyzm.p;
   a.Q = s->Q[s->cur].p;
@@ -1084,6 +1088,22 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(FEMX_ERR_HIP);
   if ((r = s->bc.build(bc, m.N, true, s->st)) != FEMX_OK) return fail(r);
   if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
+  {
+    // the positions need 10 bits (< 4 x 256 corners per block): bit 15 carries
+    // whether the corner's node has a BC record
+    std::vector<uint16_t> lp(m.blk.lcsr);
+    std::vector<char> has_rec(m.N, 0);
+    for (int nd : s->bc.h_rec_node) has_rec[nd] = 1;
+    for (int blk = 0; blk < m.blk.nb; blk++)
+      for (int e = m.blk.elem_off[blk]; e < m.blk.elem_off[blk + 1]; e++)
+        for (int a = 0; a < 4; a++) {
+          uint16_t &v = lp[4L * e + a];
+          if (v & BC_BIT) return fail((set_error("block positions overflow 15 bits"), FEMX_ERR_ARG));
+          const int g = m.blk.loc_node[m.blk.node_off[blk] + m.blk.lconn[4L * e + a]];
+          if (has_rec[g]) v |= BC_BIT;
+        }
+    if ((r = s->lpos_bc.upload(lp.data(), lp.size(), s->st)) != FEMX_OK) return fail(r);
+  }
   if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->mass.alloc(m.N)) != FEMX_OK) return fail(r);
   for (int k = 0; k < 2; k++)
     if ((r = s->P[k].alloc(m.N)) != FEMX_OK || (r = s->Q[k].alloc(m.N)) != FEMX_OK) return fail(r);

@@ -635,28 +635,42 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     }
   }
   if (!RES) return;
+  // FP32 element math re-reads the owned node's slot and run bounds (from
+  // L2): issued before the barrier, their latency overlaps its wait
+  int ps_own = ps_keep, s0 = s0_keep, s1 = s1_keep;
+  if (F32 && has_n) {
+    ps_own = a.loc_pslot[n0 + tid];
+    s0 = a.lcsr_off[n0 + tid];
+    s1 = a.lcsr_off[n0 + tid + 1];
+  }
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
   double m2 = 0.0;
   if (has_n) {
     const int g = g_own;
-    const int ps_own = F32 ? a.loc_pslot[n0 + tid] : ps_keep;
-    double R[4] = {0.0, 0.0, 0.0, 0.0};
-    const int s0 = (F32 ? a.lcsr_off[n0 + tid] : s0_keep) - e0 * 4;
-    const int s1 = (F32 ? a.lcsr_off[n0 + tid + 1] : s1_keep) - e0 * 4;
-#pragma unroll 4
-    for (int s = s0; s < s1; s++) {
-      const T4 v = sres[swz<T4>(s)];
-      R[0] += (double)v.x;
-      R[1] += (double)v.y;
-      R[2] += (double)v.z;
-      R[3] += (double)v.w;
+    // base of update_dual_solution (post-BC at the first dual iteration,
+    // pre-BC after) and 1/m of an owned node, loaded ahead of the sum
+    double4 base = base_keep;
+    double im = im_keep;
+    if (F32 && ps_own < 0) {
+      base = !ACC ? a.Q[g] : a.P[g];
+      im = a.xyzm[g].w;
     }
+    // the node's contributions summed in T (FP32 element math: the sum's
+    // rounding is of the order of the FP32 contributions' own), then FP64
+    T Rt[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll 4
+    for (int s = s0 - e0 * 4; s < s1 - e0 * 4; s++) {
+      const T4 v = sres[swz<T4>(s)];
+      Rt[0] += v.x;
+      Rt[1] += v.y;
+      Rt[2] += v.z;
+      Rt[3] += v.w;
+    }
+    double R[4] = {(double)Rt[0], (double)Rt[1], (double)Rt[2], (double)Rt[3]};
     if (ps_own < 0) {
-      // base of update_dual_solution: post-BC at the first dual iteration, pre-BC after
-      const double4 base = F32 ? (!ACC ? a.Q[g] : a.P[g]) : base_keep;
       double4 pre, post;
-      m2 = node_update(base, F32 ? a.xyzm[g].w : im_keep, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, im, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -688,19 +702,39 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double m2 = 0.0;
   if (s < a.n_sh) {
+    // two rounds of independent loads: what s indexes, then the base state
+    // and the first FIX_BATCH partial slots (most nodes have at most 4),
+    // loaded unconditionally at indexes clamped into [0, k1 - 1] (the slot
+    // array holds at least one record; nodes of no element have none)
+    constexpr int FIX_BATCH = 4;
     const int g = a.sh_node[s];
+    const int k0 = a.sh_off[s], k1 = a.sh_off[s + 1];
+    const double im = a.sh_im[s];
+    const int rec = a.sh_rec[s];
+    const double4 *B = a.first_dual ? a.Q : a.P;
+    const double4 base = B[g];
+    double4 p[FIX_BATCH];
+#pragma unroll
+    for (int j = 0; j < FIX_BATCH; j++) p[j] = a.partials[max(min(k0 + j, k1 - 1), 0)];
     double R[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = a.sh_off[s]; k < a.sh_off[s + 1]; k++) {
-      const double4 p = a.partials[k];
-      R[0] += p.x;
-      R[1] += p.y;
-      R[2] += p.z;
-      R[3] += p.w;
+#pragma unroll
+    for (int j = 0; j < FIX_BATCH; j++)
+      if (k0 + j < k1) {
+        R[0] += p[j].x;
+        R[1] += p[j].y;
+        R[2] += p[j].z;
+        R[3] += p[j].w;
+      }
+    for (int k = k0 + FIX_BATCH; k < k1; k++) {
+      const double4 q = a.partials[k];
+      R[0] += q.x;
+      R[1] += q.y;
+      R[2] += q.z;
+      R[3] += q.w;
     }
-    const double4 base = a.first_dual ? a.Q[g] : a.P[g];
     const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
-    m2 = node_update(base, a.sh_im[s], R, k, pre, post, a.sh_rec[s], a.rec, a.row_off, a.rows, a.err);
+    m2 = node_update(base, im, R, k, pre, post, rec, a.rec, a.row_off, a.rows, a.err);
     a.Pn[g] = pre;
     a.Qn[g] = post;
   }

@@ -313,6 +313,17 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   }
 }
 
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
 constexpr int BC_BIT = 0x8000;
 
 template <typename T>
@@ -327,7 +338,9 @@ struct BlockArgs {
   const double4 *xyzm;  // x y z, lumped nodal mass
   const double4 *Q, *P; // post-BC / pre-BC state of the previous iteration
   double4 *Qn, *Pn;     // outputs of this iteration
-  double4 *partials;    // FP64 partial residual sums of block-boundary nodes
+  // partial residual sums of block-boundary nodes, in T: with FP32 element
+  // math the in-block sums are FP32 values, so float4 records hold them exactly
+  typename Vec4<T>::type *partials;
   T *sdus;              // [12][E] (block order), index (g*3+i)*E + e
   int E;
   int first_dual;       // base of the update: Q at the first dual iteration, P later
@@ -383,17 +396,6 @@ __device__ __forceinline__ double node_update(const double4 base, double im, con
   post = make_double4(u, v, w, p);
   return m2;
 }
-
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<double> {
-  using type = double4;
-};
-template <>
-struct Vec4<float> {
-  using type = float4;
-};
 
 // XOR swizzle of a record index inside each 256-byte LDS row group, so that
 // lanes reading records 16 (float4) or 8 (double4) apart hit different banks.
@@ -674,7 +676,12 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
-      a.partials[ps_own] = make_double4(R[0], R[1], R[2], R[3]);
+      T4 v;
+      v.x = Rt[0];
+      v.y = Rt[1];
+      v.z = Rt[2];
+      v.w = Rt[3];
+      a.partials[ps_own] = v;
     }
   }
   block_max_store<VB>(m2, a.bmax);
@@ -685,7 +692,8 @@ struct FixArgs {
   const int *sh_node, *sh_off;
   const double *sh_im;  // 1 / nd_mass of every shared node, contiguous
   const int *sh_rec;    // its BC record or -1, contiguous
-  const double4 *partials, *Q, *P;
+  const void *partials;  // T4 records of the block kernel
+  const double4 *Q, *P;
   double4 *Qn, *Pn;
   int first_dual;
   const BCRecord *rec;
@@ -698,7 +706,10 @@ struct FixArgs {
   VmsConst k;
 };
 
+template <typename T>
 __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
+  using T4 = typename Vec4<T>::type;
+  const T4 *partials = static_cast<const T4 *>(a.partials);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double m2 = 0.0;
   if (s < a.n_sh) {
@@ -713,24 +724,24 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     const int rec = a.sh_rec[s];
     const double4 *B = a.first_dual ? a.Q : a.P;
     const double4 base = B[g];
-    double4 p[FIX_BATCH];
+    T4 p[FIX_BATCH];
 #pragma unroll
-    for (int j = 0; j < FIX_BATCH; j++) p[j] = a.partials[max(min(k0 + j, k1 - 1), 0)];
+    for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[max(min(k0 + j, k1 - 1), 0)];
     double R[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < FIX_BATCH; j++)
       if (k0 + j < k1) {
-        R[0] += p[j].x;
-        R[1] += p[j].y;
-        R[2] += p[j].z;
-        R[3] += p[j].w;
+        R[0] += (double)p[j].x;
+        R[1] += (double)p[j].y;
+        R[2] += (double)p[j].z;
+        R[3] += (double)p[j].w;
       }
     for (int k = k0 + FIX_BATCH; k < k1; k++) {
-      const double4 q = a.partials[k];
-      R[0] += q.x;
-      R[1] += q.y;
-      R[2] += q.z;
-      R[3] += q.w;
+      const T4 q = partials[k];
+      R[0] += (double)q.x;
+      R[1] += (double)q.y;
+      R[2] += (double)q.z;
+      R[3] += (double)q.w;
     }
     const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
@@ -996,7 +1007,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.P = s->P[s->cur].p;
   a.Qn = s->Q[s->cur ^ 1].p;
   a.Pn = s->P[s->cur ^ 1].p;
-  a.partials = s->partials.p;
+  a.partials = reinterpret_cast<typename Vec4<T>::type *>(s->partials.p);
   a.sdus = sdus;
   a.E = m.E;
   a.first_dual = first_dual ? 1 : 0;
@@ -1043,7 +1054,9 @@ femx_status launch_block(femx_vms *s, bool proj, bool res, bool acc, bool first_
 femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   Blocks &b = s->mesh->blk;
   if (b.n_sh == 0) return FEMX_OK;
-  const double bytes = b.sh_off[b.n_sh] * 32.0 * 2 + b.n_sh * (4.0 + 4.0 + 32.0 + 32.0 + 64.0);
+  const bool mixed = s->p.precision == FEMX_MIXED;
+  const double rec_bytes = mixed ? 16.0 : 32.0;  // one partial slot record
+  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 32.0 + 32.0 + 64.0);
   TimedLaunch tl(s->timer, s->timer.kind("vms_fixup", bytes), s->st);
   FixArgs a;
   a.n_sh = b.n_sh;
@@ -1065,7 +1078,8 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.node_mesh = s->batch ? s->node_mesh.p : nullptr;
   a.kt = s->batch ? s->kt.p : nullptr;
   a.k = vms_const(s, t);
-  vms_fixup<<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  if (mixed) vms_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  else vms_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
 }

@@ -155,26 +155,30 @@ constexpr double PR_SUM = (4.0 * GB * GB + 2.0 * GB * GD) / GS;  // coefficient 
 constexpr double M_DIAG = GA * GA + 3.0 * GB * GB;            // sum_g lN[g][b]^2
 constexpr double M_OFF = 2.0 * GA * GB + 2.0 * GB * GB;       // sum_g lN[g][b] lN[g][c], b != c
 
-// Nodal velocities of one element in T with their sum and the (constant)
+// The nodal velocities of one element enter the projection and the residual
+// only through their Gauss-point values sum_b lN[g][b] u_b = GD u_g + GB
+// sum_b u_b (the identity above), kept as a0[g], and through the (constant)
 // velocity gradient grad[i][j] = d u_i / d x_j (:259-268, :499-506).
 template <typename T>
 struct ElemVel {
-  T u[4][3], su[3], grad[3][3];
+  T a0[4][3], grad[3][3];
 };
 template <typename T>
 __device__ __forceinline__ void elem_vel(const T DN[3][4], const double uv[4][3], ElemVel<T> &ev) {
-  T du[3][3];
+  T u[4][3], du[3][3];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) ev.u[a][i] = (T)uv[a][i];
+    for (int i = 0; i < 3; i++) u[a][i] = (T)uv[a][i];
 #pragma unroll
   for (int a = 1; a < 4; a++)
 #pragma unroll
     for (int i = 0; i < 3; i++) du[a - 1][i] = (T)(uv[a][i] - uv[0][i]);  // sum_a DN[j][a] = 0
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    ev.su[i] = (ev.u[0][i] + ev.u[1][i]) + (ev.u[2][i] + ev.u[3][i]);
+    const T gbs = (T)GB * ((u[0][i] + u[1][i]) + (u[2][i] + u[3][i]));
+#pragma unroll
+    for (int g = 0; g < 4; g++) ev.a0[g][i] = (T)GD * u[g][i] + gbs;
 #pragma unroll
     for (int j = 0; j < 3; j++) ev.grad[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
   }
@@ -182,21 +186,20 @@ __device__ __forceinline__ void elem_vel(const T DN[3][4], const double uv[4][3]
 
 // The same from FP32 velocities relative to the block's reference velocity
 // ref (u = ref + rel): the differences u_a - u_0 are differences of the
-// small relative values, so they keep their significant digits.
+// small relative values, so they keep their significant digits, and
+// a0[g] = GD rel_g + (GB sum_b rel_b + GS ref), since GD + 4 GB = GS.
 __device__ __forceinline__ void elem_vel_rel(const float DN[3][4], const float ref[3], const float rel[4][3],
                                              ElemVel<float> &ev) {
   float du[3][3];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int i = 0; i < 3; i++) ev.u[a][i] = ref[i] + rel[a][i];
 #pragma unroll
   for (int a = 1; a < 4; a++)
 #pragma unroll
     for (int i = 0; i < 3; i++) du[a - 1][i] = rel[a][i] - rel[0][i];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    ev.su[i] = (ev.u[0][i] + ev.u[1][i]) + (ev.u[2][i] + ev.u[3][i]);
+    const float c = (float)GB * ((rel[0][i] + rel[1][i]) + (rel[2][i] + rel[3][i])) + (float)GS * ref[i];
+#pragma unroll
+    for (int g = 0; g < 4; g++) ev.a0[g][i] = (float)GD * rel[g][i] + c;
 #pragma unroll
     for (int j = 0; j < 3; j++) ev.grad[i][j] = DN[j][1] * du[0][i] + DN[j][2] * du[1][i] + DN[j][3] * du[2][i];
   }
@@ -215,16 +218,13 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const Ele
   T gradP[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) gradP[i] = DN[i][1] * dq[0] + DN[i][2] * dq[1] + DN[i][3] * dq[2];
-  const T(&u)[4][3] = ev.u;
-  const T(&su)[3] = ev.su;
   const T(&gradDu)[3][3] = ev.grad;
-  const T gb = (T)GB, gd = (T)GD;
   T maxA2 = (T)0, r[4][3], rs[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     T hah[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) hah[i] = gd * u[g][i] + (gb * su[i] + sdu[g][i]);  // convective velocity :529-535
+    for (int i = 0; i < 3; i++) hah[i] = ev.a0[g][i] + sdu[g][i];  // convective velocity :529-535
     const T m2 = hah[0] * hah[0] + hah[1] * hah[1] + hah[2] * hah[2];
     maxA2 = m2 > maxA2 ? m2 : maxA2;
 #pragma unroll
@@ -237,11 +237,15 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const Ele
   const T inv_dt = (T)k.inv_dt;
   const T itu = ((T)k.c1 * (T)k.nu * ih + (T)k.c2 * sqrt_t(maxA2)) * ih;
   const T tau1 = rcp_t(inv_dt + itu);
-  const T ps = (T)PR_SELF, pm = (T)PR_SUM;
+  // sdu <- (sdu / dt + PR_SELF r_g + PR_SUM sum_h r_h) tau1  (:548 + :630)
+  const T ka = inv_dt * tau1, kb = (T)PR_SELF * tau1, kc = (T)PR_SUM * tau1;
+  T cs[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) cs[i] = kc * rs[i];
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) sdu[g][i] = (sdu[g][i] * inv_dt + ps * r[g][i] + pm * rs[i]) * tau1;  // :548 + :630
+    for (int i = 0; i < 3; i++) sdu[g][i] = ka * sdu[g][i] + (kb * r[g][i] + cs[i]);
 }
 
 // Element residual lRes[4][4] of assemble_RHS (:93-399) with sgps = 0 and
@@ -253,8 +257,6 @@ template <typename T, bool ACC>
 __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVel<T> &ev, const T p[4],
                                              const T dd[4][3], const T sdu[4][3], T res[4][4],
                                              const VmsConst &k) {
-  const T(&u)[4][3] = ev.u;
-  const T(&su)[3] = ev.su;
   const T(&gradU)[3][3] = ev.grad;
   T ss[3];
   const T psum = (p[0] + p[1]) + (p[2] + p[3]);
@@ -274,7 +276,7 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   for (int g = 0; g < 4; g++) {
     T ag[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) ag[i] = gd * u[g][i] + (gb * su[i] + sdu[g][i]);
+    for (int i = 0; i < 3; i++) ag[i] = ev.a0[g][i] + sdu[g][i];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       agu[g][i] = gradU[i][0] * ag[0] + gradU[i][1] * ag[1] + gradU[i][2] * ag[2];

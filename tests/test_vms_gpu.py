@@ -64,6 +64,25 @@ def test_fp64_per_step_parity(gpu, kind, n, jitter):
         assert t == o.get("scalars")[0]
 
 
+@pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5)])
+def test_nonzero_pressure_and_offset_velocity(gpu, precision, tol):
+    """Every node starts with a non-integer pressure, so the pressure
+    gradient the projection takes from global nodes 0..3 (:489) is nonzero,
+    and the velocity carries a non-integer mean, so the block reference
+    values of FP32 element math are not small integers."""
+    m, bc, params, ic = _case("channel", 6, 0.15, seed=9)
+    rng = np.random.default_rng(9)
+    ic[:, 3] = rng.uniform(0.2, 0.8, m.n_nodes)
+    ic[:, :3] += np.array([0.37, -0.21, 0.13])
+    assert np.ptp(ic[:4, 3]) > 0.01
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=precision)
+    for step in range(6):
+        o.step()
+        g.step()
+    sol, _, _ = g.get_state()
+    assert rel_l2(sol, o.get("sol_n")) <= tol, rel_l2(sol, o.get("sol_n"))
+
+
 def test_fp64_fused_run_matches(gpu):
     """Continuous stepping (projection fused into the next residual launch)."""
     m, bc, params, ic = _case("channel", 6, 0.15, seed=5)

@@ -184,6 +184,18 @@ __device__ inline void apply_bc(const BCRecord &r, const int *row_off, const dou
   }
 }
 
+// Four-component vector of T (node records, partial sums).
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
 // gfx950 deals workgroups round-robin over its 8 XCDs (each with its own L2);
 // give every XCD a contiguous range of the Morton-ordered element blocks so
 // neighbouring blocks share node data in one L2.  Bijective on [0, nblk).

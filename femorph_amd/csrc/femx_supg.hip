@@ -96,7 +96,9 @@ struct SupgArgs {
   const double4 *Upress; // post-BC state before the fixed-DOF zeroing (pressure of the previous step)
   double4 *Un, *Unp;     // outputs: next sol_p, next pre-zeroing state
   double4 *Fprev;        // assembled force of the previous step (AB2)
-  double4 *partials;
+  // partial force sums of block-boundary nodes, in T: with FP32 element math
+  // the in-block sums are FP32 values, so float4 records hold them exactly
+  typename Vec4<T>::type *partials;
   double *lam, *regime, *pres;  // [E] block order
   const int *node_rec;
   const BCRecord *rec;
@@ -525,14 +527,17 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
   __syncthreads();
   const int base_s = e0 * NPE;
   for (int j = tid; j < nn; j += SB) {
-    double F[3] = {0.0, 0.0, 0.0};
+    // the node's contributions summed in T (FP32 element math: the sum's
+    // rounding is of the order of the FP32 contributions' own), then FP64
+    T Ft[3] = {(T)0, (T)0, (T)0};
     const int s0 = a.lcsr_off[n0 + j] - base_s, s1 = a.lcsr_off[n0 + j + 1] - base_s;
 #pragma unroll 4
     for (int s = s0; s < s1; s++) {
-      F[0] += (double)sres[0 * (NPE * SB) + s];
-      F[1] += (double)sres[1 * (NPE * SB) + s];
-      F[2] += (double)sres[2 * (NPE * SB) + s];
+      Ft[0] += sres[0 * (NPE * SB) + s];
+      Ft[1] += sres[1 * (NPE * SB) + s];
+      Ft[2] += sres[2 * (NPE * SB) + s];
     }
+    const double F[3] = {(double)Ft[0], (double)Ft[1], (double)Ft[2]};
     const int psl = a.loc_pslot[n0 + j];
     const int g = a.loc_node[n0 + j];
     if (psl < 0) {
@@ -544,7 +549,12 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       if (a.Unp) a.Unp[g] = pst;
       a.Fprev[g] = fst;
     } else {
-      a.partials[psl] = make_double4(F[0], F[1], F[2], 0.0);
+      typename Vec4<T>::type v;
+      v.x = Ft[0];
+      v.y = Ft[1];
+      v.z = Ft[2];
+      v.w = (T)0;
+      a.partials[psl] = v;
     }
   }
 }
@@ -637,7 +647,8 @@ __global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np
 struct SupgFix {
   int n_sh;
   const int *sh_node, *sh_off;
-  const double4 *partials, *xyzm, *U;
+  const void *partials;  // T4 records of the block kernel
+  const double4 *xyzm, *U;
   double4 *Un, *Unp, *Fprev;
   const int *node_rec;
   const BCRecord *rec;
@@ -648,21 +659,43 @@ struct SupgFix {
   SupgConst k;
 };
 
+template <typename T>
 __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
+  using T4 = typename Vec4<T>::type;
   if (halted(a.halt)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sh) return;
+  const T4 *partials = static_cast<const T4 *>(a.partials);
+  // two rounds of independent loads: what s indexes, then the node's state
+  // and its first FIX_BATCH partial slots (most nodes have at most 4),
+  // loaded unconditionally at indexes clamped into [0, q1 - 1] (the slot
+  // array holds at least one record; nodes of no element have none)
+  constexpr int FIX_BATCH = 4;
   const int g = a.sh_node[s];
+  const int q0 = a.sh_off[s], q1 = a.sh_off[s + 1];
+  const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
+  const double im = a.xyzm[g].w;
+  const double4 u = a.U[g];
+  const int rec = a.node_rec[g];
+  T4 p[FIX_BATCH];
+#pragma unroll
+  for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[max(min(q0 + j, q1 - 1), 0)];
   double F[3] = {0.0, 0.0, 0.0};
-  for (int q = a.sh_off[s]; q < a.sh_off[s + 1]; q++) {
-    const double4 p = a.partials[q];
-    F[0] += p.x;
-    F[1] += p.y;
-    F[2] += p.z;
+#pragma unroll
+  for (int j = 0; j < FIX_BATCH; j++)
+    if (q0 + j < q1) {
+      F[0] += (double)p[j].x;
+      F[1] += (double)p[j].y;
+      F[2] += (double)p[j].z;
+    }
+  for (int q = q0 + FIX_BATCH; q < q1; q++) {
+    const T4 r = partials[q];
+    F[0] += (double)r.x;
+    F[1] += (double)r.y;
+    F[2] += (double)r.z;
   }
   double4 next, pst, fst;
-  const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
-  supg_node(F, fp, a.xyzm[g].w, a.U[g], a.k, a.node_rec[g], a.rec, a.row_off, a.rows, a.err, next, pst, fst);
+  supg_node(F, fp, im, u, a.k, rec, a.rec, a.row_off, a.rows, a.err, next, pst, fst);
   a.Un[g] = next;
   if (a.Unp) a.Unp[g] = pst;
   a.Fprev[g] = fst;
@@ -792,7 +825,7 @@ SupgArgs<T> supg_args(femx_supg *s, bool euler_step, double t) {
   a.Un = s->U[s->cur ^ 1].p;
   a.Unp = s->p.fixed_dofs ? s->Up[s->cur ^ 1].p : nullptr;
   a.Fprev = s->Fprev.p;
-  a.partials = s->partials.p;
+  a.partials = reinterpret_cast<typename Vec4<T>::type *>(s->partials.p);
   a.lam = s->lam.p;
   a.regime = s->regime.p;
   a.pres = s->pres.p;
@@ -873,7 +906,9 @@ femx_status launch_supg(femx_supg *s, bool dyn, bool press, bool euler_step, dou
 femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   Blocks &b = s->mesh->blk;
   if (b.n_sh == 0) return FEMX_OK;
-  TimedLaunch tl(s->timer, s->timer.kind("supg_fixup", b.sh_off[b.n_sh] * 64.0 + b.n_sh * 200.0), s->st);
+  const bool mixed = s->p.precision == FEMX_MIXED;
+  const double rec_bytes = mixed ? 16.0 : 32.0;  // one partial slot record, written and read
+  TimedLaunch tl(s->timer, s->timer.kind("supg_fixup", b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * 200.0), s->st);
   SupgFix a;
   a.n_sh = b.n_sh;
   a.sh_node = b.d_sh_node.p;
@@ -891,7 +926,8 @@ femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   a.err = s->bc.err.p;
   a.halt = s->halt.p;
   a.k = supg_const(s, t, euler_step);
-  supg_fixup<<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  if (mixed) supg_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  else supg_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
 }

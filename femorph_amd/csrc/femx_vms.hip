@@ -315,17 +315,6 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   }
 }
 
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<double> {
-  using type = double4;
-};
-template <>
-struct Vec4<float> {
-  using type = float4;
-};
-
 constexpr int BC_BIT = 0x8000;
 
 template <typename T>

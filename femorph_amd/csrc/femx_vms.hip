@@ -388,20 +388,26 @@ __device__ __forceinline__ double node_update(const double4 base, double im, con
   return m2;
 }
 
-// XOR swizzle of a record index inside each 256-byte LDS row group, so that
-// lanes reading records 16 (float4) or 8 (double4) apart hit different banks.
-template <typename R>
-__device__ __forceinline__ int swz(int p) {
-  constexpr int W = 256 / (int)sizeof(R);
-  return p ^ ((p / W) & (W - 1));
-}
-
 // Row `row_elems` of an SoA array, addressed as a wave-uniform base plus a
 // zero-extended 32-bit byte offset (SGPR base + VGPR offset addressing).
 template <typename T>
 __device__ __forceinline__ T *row_ptr(T *base, size_t row_elems, unsigned byte_off) {
   return reinterpret_cast<T *>(reinterpret_cast<char *>(base + row_elems) + (size_t)byte_off);
 }
+
+// A value every lane of the wave holds, moved to SGPRs bit for bit
+// (readfirstlane moves 32-bit integers: wider or floating values go through
+// their bit patterns, never a numeric conversion).  The kernel also writes
+// through pointers of the types it reads, so without this the compiler keeps
+// per-block values in VGPRs.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)uni((int)(unsigned)(b & 0xffffffffLL));
+  const unsigned hi = (unsigned)uni((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double4 uni(double4 v) { return make_double4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
 
 template <typename T, bool PROJ, bool RES, bool ACC>
 __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 4 : 2)) void vms_block(BlockArgs<T> a) {
@@ -415,19 +421,26 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   __shared__ float4 sfa[F32 ? VN : 1], sfb[F32 ? VN : 1];
   __shared__ float2 sfc[F32 ? VN : 1];
   // element-corner contributions (u v w p) in the block's node-sorted order,
-  // one T4 record per corner at a bank-swizzled position
+  // one T4 record per corner
   __shared__ T4 sres[RES ? 4 * VB : 1];
   const int tid = threadIdx.x;
   const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
   VmsConst k = a.k;
   int node0 = 0;
   if (a.kt) {
-    const int mid = a.blk_mesh[blk];
-    k = a.kt[mid];
-    node0 = a.mesh_node0[mid];
+    const int mid = uni(a.blk_mesh[blk]);
+    const VmsConst &m = a.kt[mid];
+    k.nu = uni(m.nu);
+    k.inv_eps = uni(m.inv_eps);
+    k.c1 = uni(m.c1);
+    k.c2 = uni(m.c2);
+    k.inv_dt = uni(m.inv_dt);
+    k.dual_step = uni(m.dual_step);
+    k.t = uni(m.t);
+    node0 = uni(a.mesh_node0[mid]);
   }
-  const int n0 = a.node_off[blk], nn = a.node_off[blk + 1] - n0;
-  const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
+  const int n0 = uni(a.node_off[blk]), nn = uni(a.node_off[blk + 1]) - n0;
+  const int e0 = uni(a.elem_off[blk]), ne = uni(a.elem_off[blk + 1]) - e0;
   // 0. issue this thread's element loads before staging (they overlap)
   const bool has_e = tid < ne;
   const unsigned e = (unsigned)(e0 + tid);
@@ -461,10 +474,10 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   // reference node of the FP32 relative values: the block's first local node
   double4 xr = make_double4(0, 0, 0, 0), pr = make_double4(0, 0, 0, 0), qr = make_double4(0, 0, 0, 0);
   if (F32) {
-    const int gr = a.loc_node[n0];
-    xr = a.xyzm[gr];
-    if (PROJ || ACC) pr = a.P[gr];
-    if (RES) qr = a.Q[gr];
+    const int gr = uni(a.loc_node[n0]);
+    xr = uni(a.xyzm[gr]);
+    if (PROJ || ACC) pr = uni(a.P[gr]);
+    if (RES) qr = uni(a.Q[gr]);
   }
   // 1. stage the block's nodes
   double4 base_keep = make_double4(0, 0, 0, 0);
@@ -496,7 +509,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   if (PROJ) {
     // p[i] = sol_n[i][3]: pressures of GLOBAL nodes 0..3 (:489)
 #pragma unroll
-    for (int n = 0; n < 4; n++) q4[n] = a.P[node0 + n].w;
+    for (int n = 0; n < 4; n++) q4[n] = uni(a.P[node0 + n].w);
   }
   __syncthreads();
   // 2. one element per thread
@@ -623,7 +636,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
         v.y = res[b][1];
         v.z = res[b][2];
         v.w = res[b][3];
-        sres[swz<T4>(ps[b])] = v;
+        sres[ps[b]] = v;
       }
     }
   }
@@ -654,7 +667,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     T Rt[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll 4
     for (int s = s0 - e0 * 4; s < s1 - e0 * 4; s++) {
-      const T4 v = sres[swz<T4>(s)];
+      const T4 v = sres[s];
       Rt[0] += v.x;
       Rt[1] += v.y;
       Rt[2] += v.z;

@@ -30,6 +30,9 @@ sys.path.insert(0, ROOT)
 from femorph_amd import femx, meshgen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+F32SUM_NOTE = ("opt-in FEMX_MIXED_F32SUM: FP32 element math with FP32 in-block node sums and FP32 block "
+               "partials (sum over blocks, state and update FP64); not the headline, which keeps the residual "
+               "accumulation in FP64")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vms_block_c3_traffic.json")
 PROFILED_KERNEL = {"mixed": "vms_block<float, true, true, false>", "fp64": "vms_block<double, true, true, false>"}
 
@@ -335,7 +338,7 @@ def workload_c2(args):
     ic = meshgen.random_velocity(m, 1e-3, args.seed)[:, :3]
     E, N = m.n_elems, m.n_nodes
     res = {}
-    for name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64)):
+    for name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64), ("mixed_f32sum", femx.FEMX_MIXED_F32SUM)):
         g = femx.SUPGSolver(dm, bc, precision=prec, **p)
         g.set_state(ic)
         g.step(args.warmup)
@@ -372,7 +375,10 @@ def workload_c2(args):
                         "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4))
                                     for k, v in kt.items()}},
            other: {"value": round(E * args.steps / res[other][0], 1),
-                   "ms_per_step": round(res[other][0] / args.steps * 1e3, 4)}}
+                   "ms_per_step": round(res[other][0] / args.steps * 1e3, 4)},
+           "mixed_f32sum": {"value": round(E * args.steps / res["mixed_f32sum"][0], 1),
+                            "ms_per_step": round(res["mixed_f32sum"][0] / args.steps * 1e3, 4),
+                            "note": F32SUM_NOTE}}
     print(json.dumps(out), flush=True)
 
 
@@ -501,7 +507,8 @@ def workload_accuracy(args):
     o.set_ic(ic)
     dm = femx.DeviceMesh(m)
     res = {}
-    for prec in (femx.FEMX_MIXED, femx.FEMX_FP64):
+    names = {femx.FEMX_MIXED: "mixed", femx.FEMX_FP64: "fp64", femx.FEMX_MIXED_F32SUM: "mixed_f32sum"}
+    for prec in names:
         g = femx.VMSSolver(dm, bc, precision=prec, **p)
         g.set_state(ic)
         res[prec] = g
@@ -516,7 +523,7 @@ def workload_accuracy(args):
     c1 = {}
     for prec, g in res.items():
         sol = g.get_state()[0]
-        c1["mixed" if prec == femx.FEMX_MIXED else "fp64"] = {
+        c1[names[prec]] = {
             "rel_l2_velocity": _rel(sol[:, :3], ref[:, :3]), "rel_l2_pressure": _rel(sol[:, 3], ref[:, 3])}
     out["configs"]["C1 VMS tet4 20^3x6 cavity Re100 (vs FP64 oracle)"] = c1
     del res, dm, o
@@ -525,14 +532,16 @@ def workload_accuracy(args):
     dm = femx.DeviceMesh(m)
     gm = femx.VMSSolver(dm, bc, precision=femx.FEMX_MIXED, **params)
     gd = femx.VMSSolver(dm, bc, precision=femx.FEMX_FP64, **params)
-    for g in (gm, gd):
+    gs = femx.VMSSolver(dm, bc, precision=femx.FEMX_MIXED_F32SUM, **params)
+    for g in (gm, gd, gs):
         g.set_state(ic)
         g.step(steps)
-    a, b = gm.get_state()[0], gd.get_state()[0]
+    a, b, c = gm.get_state()[0], gd.get_state()[0], gs.get_state()[0]
     out["configs"]["C3 VMS tet4 96^3x6 channel Re200 (vs FP64 GPU path)"] = {
-        "rel_l2_velocity": _rel(a[:, :3], b[:, :3]), "rel_l2_pressure": _rel(a[:, 3], b[:, 3])}
+        "rel_l2_velocity": _rel(a[:, :3], b[:, :3]), "rel_l2_pressure": _rel(a[:, 3], b[:, 3]),
+        "mixed_f32sum": {"rel_l2_velocity": _rel(c[:, :3], b[:, :3]), "rel_l2_pressure": _rel(c[:, 3], b[:, 3])}}
     note("C3 done")
-    del gm, gd, dm
+    del gm, gd, gs, dm
     # C2: SUPG hexa8 128^3 jittered lid cavity Re 400, AB2 + B-hat + K1..K4
     mh = meshgen.hex_cube(128, 0.2, args.seed)
     bch = meshgen.lid_cavity_bc(mh)
@@ -541,12 +550,14 @@ def workload_accuracy(args):
     ich = meshgen.random_velocity(mh, 1e-3, args.seed)[:, :3]
     sm = femx.SUPGSolver(dmh, bch, precision=femx.FEMX_MIXED, **ph)
     sd = femx.SUPGSolver(dmh, bch, precision=femx.FEMX_FP64, **ph)
-    for g in (sm, sd):
+    ss = femx.SUPGSolver(dmh, bch, precision=femx.FEMX_MIXED_F32SUM, **ph)
+    for g in (sm, sd, ss):
         g.set_state(ich)
         g.step(steps)
-    a, b = sm.get_state(), sd.get_state()
+    a, b, c = sm.get_state(), sd.get_state(), ss.get_state()
     out["configs"]["C2 SUPG hexa8 128^3 jitter0.2 cavity Re400 (vs FP64 GPU path)"] = {
-        "rel_l2_velocity": _rel(a["sol"], b["sol"]), "rel_l2_pressure": _rel(a["pres"], b["pres"])}
+        "rel_l2_velocity": _rel(a["sol"], b["sol"]), "rel_l2_pressure": _rel(a["pres"], b["pres"]),
+        "mixed_f32sum": {"rel_l2_velocity": _rel(c["sol"], b["sol"]), "rel_l2_pressure": _rel(c["pres"], b["pres"])}}
     note("C2 done")
     print(json.dumps(out), flush=True)
 
@@ -614,7 +625,8 @@ def main():
     r = N / E
     dm = femx.DeviceMesh(m)
     results = {}
-    for prec_name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64)):
+    for prec_name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64),
+                            ("mixed_f32sum", femx.FEMX_MIXED_F32SUM)):
         secs, kt, finite = time_solver(dm, bc, params, ic, prec, args.steps, args.warmup)
         results[prec_name] = dict(secs=secs, kt=kt, finite=finite)
     main_res = results[args.precision]
@@ -653,6 +665,9 @@ def main():
         "roofline": roof,
         other: {"value": round(E * args.steps / results[other]["secs"], 1),
                 "ms_per_step": round(results[other]["secs"] / args.steps * 1e3, 4)},
+        "mixed_f32sum": {"value": round(E * args.steps / results["mixed_f32sum"]["secs"], 1),
+                         "ms_per_step": round(results["mixed_f32sum"]["secs"] / args.steps * 1e3, 4),
+                         "note": F32SUM_NOTE},
         "finite_state": all(v["finite"] for v in results.values()),
         "event_step_ms": round(step_ms_events, 4),
     }

@@ -75,20 +75,27 @@ struct DevBuf {
 //   loc_node  int[L]        global id of each (block, local node), ascending
 //   loc_pslot int[L]        -1 if every element of the node is in the block
 //                           ("complete": updated inside the block), else the
-//                           node's partial-sum slot (finished by the fix-up)
+//                           node's partial-sum slot (finished by the fix-up);
+//                           slots are numbered block by block in local node
+//                           order, so a block writes its partials as one run
 //   lcsr_off  int[L+1]      per local node: its range of contributions,
 //                           block contributions sorted by (node, element)
 //   lcsr      u16[npe*E]    per (element, corner): its position in that
 //                           node-sorted order, relative to the block start
 //   lconn     u8[E*npe]     local node index of each corner (block order)
 //   sh_node   int[S]        nodes shared between blocks
-//   sh_off    int[S+1]      their partial slots, in block order
+//   sh_off    int[S+1]      range of each one's slots in sh_slot
+//   sh_slot   int[slots]    its partial slots, in block order
+//   sh_first4 int4[S]       its first 4 slots (the last one repeated when it
+//                           has fewer; 0 when it has none)
 struct Blocks {
   int B = 0, nb = 0, max_local = 0, n_sh = 0, total_local = 0;
-  std::vector<int> perm, elem_off, node_off, loc_node, loc_pslot, lcsr_off, sh_node, sh_off;
+  std::vector<int> perm, elem_off, node_off, loc_node, loc_pslot, lcsr_off, sh_node, sh_off, sh_slot;
+  std::vector<int4> sh_first4;
   std::vector<uint16_t> lcsr;
   std::vector<uint8_t> lconn;
-  DevBuf<int> d_perm, d_elem_off, d_node_off, d_loc_node, d_loc_pslot, d_lcsr_off, d_sh_node, d_sh_off;
+  DevBuf<int> d_perm, d_elem_off, d_node_off, d_loc_node, d_loc_pslot, d_lcsr_off, d_sh_node, d_sh_off, d_sh_slot;
+  DevBuf<int4> d_sh_first4;
   DevBuf<uint16_t> d_lcsr;
   DevBuf<uint8_t> d_lconn;
 };
@@ -182,6 +189,14 @@ __device__ inline void apply_bc(const BCRecord &r, const int *row_off, const dou
     v = r.dval[1];
     w = r.dval[2];
   }
+}
+
+// Precision modes: FP32 element math (FEMX_MIXED, FEMX_MIXED_F32SUM), and
+// FP32 in-block node sums with FP32 block partials (FEMX_MIXED_F32SUM only).
+inline bool f32_elem(int precision) { return precision == FEMX_MIXED || precision == FEMX_MIXED_F32SUM; }
+inline bool f32_sum(int precision) { return precision == FEMX_MIXED_F32SUM; }
+inline bool valid_precision(int precision) {
+  return precision == FEMX_FP64 || precision == FEMX_MIXED || precision == FEMX_MIXED_F32SUM;
 }
 
 // Four-component vector of T (node records, partial sums).

@@ -225,12 +225,24 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
       b.sh_off.push_back(b.sh_off.back() + nparts[n]);
     }
   b.n_sh = (int)b.sh_node.size();
+  // slots numbered block by block in local node order (each block writes its
+  // partials as one contiguous run); per shared node its slots in block order
   std::vector<int> taken(b.n_sh, 0);
+  b.sh_slot.assign(b.sh_off[b.n_sh], 0);
+  int slot = 0;
   for (long l = 0; l < L; l++)
     if (b.loc_pslot[l] == -2) {
       const int si = sh_index[b.loc_node[l]];
-      b.loc_pslot[l] = b.sh_off[si] + taken[si]++;
+      b.sh_slot[b.sh_off[si] + taken[si]++] = slot;
+      b.loc_pslot[l] = slot++;
     }
+  b.sh_first4.resize(b.n_sh);
+  for (int si = 0; si < b.n_sh; si++) {
+    const int k0 = b.sh_off[si], c = b.sh_off[si + 1] - k0;
+    int f[4];
+    for (int j = 0; j < 4; j++) f[j] = c == 0 ? 0 : b.sh_slot[k0 + std::min(j, c - 1)];
+    b.sh_first4[si] = make_int4(f[0], f[1], f[2], f[3]);
+  }
 }
 
 femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
@@ -251,6 +263,9 @@ femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const st
   if (b.n_sh > 0) {
     if ((s = b.d_sh_node.upload(b.sh_node.data(), b.n_sh, st)) != FEMX_OK) return s;
     if ((s = b.d_sh_off.upload(b.sh_off.data(), b.n_sh + 1, st)) != FEMX_OK) return s;
+    if ((s = b.d_sh_first4.upload(b.sh_first4.data(), b.n_sh, st)) != FEMX_OK) return s;
+    if (!b.sh_slot.empty() && (s = b.d_sh_slot.upload(b.sh_slot.data(), b.sh_slot.size(), st)) != FEMX_OK)
+      return s;
   }
   return FEMX_OK;
 }

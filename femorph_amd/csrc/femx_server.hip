@@ -344,7 +344,7 @@ extern "C" femx_status femx_server_submit(femx_server *s, const femx_request *q,
   r->supg = q->supg;
   const bool vms = q->kind == FEMX_REQ_VMS;
   r->bytes = footprint(d.n_nodes, d.n_elems, d.npe, vms,
-                       (vms ? q->vms.precision : q->supg.precision) == FEMX_MIXED);
+                       f32_elem(vms ? q->vms.precision : q->supg.precision));
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;

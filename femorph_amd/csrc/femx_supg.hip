@@ -96,9 +96,10 @@ struct SupgArgs {
   const double4 *Upress; // post-BC state before the fixed-DOF zeroing (pressure of the previous step)
   double4 *Un, *Unp;     // outputs: next sol_p, next pre-zeroing state
   double4 *Fprev;        // assembled force of the previous step (AB2)
-  // partial force sums of block-boundary nodes, in T: with FP32 element math
-  // the in-block sums are FP32 values, so float4 records hold them exactly
-  typename Vec4<T>::type *partials;
+  // partial force sums of block-boundary nodes: double4 records, or float4
+  // with f32sum (FEMX_MIXED_F32SUM: FP32 in-block sums)
+  void *partials;
+  int f32sum;
   double *lam, *regime, *pres;  // [E] block order
   const int *node_rec;
   const BCRecord *rec;
@@ -527,17 +528,28 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
   __syncthreads();
   const int base_s = e0 * NPE;
   for (int j = tid; j < nn; j += SB) {
-    // the node's contributions summed in T (FP32 element math: the sum's
-    // rounding is of the order of the FP32 contributions' own), then FP64
-    T Ft[3] = {(T)0, (T)0, (T)0};
+    // the node's contributions summed in FP64, or in FP32 with f32sum
+    double F[3] = {0.0, 0.0, 0.0};
     const int s0 = a.lcsr_off[n0 + j] - base_s, s1 = a.lcsr_off[n0 + j + 1] - base_s;
+    if (a.f32sum) {
+      float Ft[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 4
-    for (int s = s0; s < s1; s++) {
-      Ft[0] += sres[0 * (NPE * SB) + s];
-      Ft[1] += sres[1 * (NPE * SB) + s];
-      Ft[2] += sres[2 * (NPE * SB) + s];
+      for (int s = s0; s < s1; s++) {
+        Ft[0] += (float)sres[0 * (NPE * SB) + s];
+        Ft[1] += (float)sres[1 * (NPE * SB) + s];
+        Ft[2] += (float)sres[2 * (NPE * SB) + s];
+      }
+      F[0] = Ft[0];
+      F[1] = Ft[1];
+      F[2] = Ft[2];
+    } else {
+#pragma unroll 4
+      for (int s = s0; s < s1; s++) {
+        F[0] += (double)sres[0 * (NPE * SB) + s];
+        F[1] += (double)sres[1 * (NPE * SB) + s];
+        F[2] += (double)sres[2 * (NPE * SB) + s];
+      }
     }
-    const double F[3] = {(double)Ft[0], (double)Ft[1], (double)Ft[2]};
     const int psl = a.loc_pslot[n0 + j];
     const int g = a.loc_node[n0 + j];
     if (psl < 0) {
@@ -549,12 +561,10 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       if (a.Unp) a.Unp[g] = pst;
       a.Fprev[g] = fst;
     } else {
-      typename Vec4<T>::type v;
-      v.x = Ft[0];
-      v.y = Ft[1];
-      v.z = Ft[2];
-      v.w = (T)0;
-      a.partials[psl] = v;
+      if (a.f32sum)
+        static_cast<float4 *>(a.partials)[psl] = make_float4((float)F[0], (float)F[1], (float)F[2], 0.0f);
+      else
+        static_cast<double4 *>(a.partials)[psl] = make_double4(F[0], F[1], F[2], 0.0);
     }
   }
 }
@@ -646,7 +656,8 @@ __global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np
 
 struct SupgFix {
   int n_sh;
-  const int *sh_node, *sh_off;
+  const int *sh_node, *sh_off, *sh_slot;
+  const int4 *sh_first4;  // each shared node's first 4 slots
   const void *partials;  // T4 records of the block kernel
   const double4 *xyzm, *U;
   double4 *Un, *Unp, *Fprev;
@@ -666,20 +677,21 @@ __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sh) return;
   const T4 *partials = static_cast<const T4 *>(a.partials);
-  // two rounds of independent loads: what s indexes, then the node's state
-  // and its first FIX_BATCH partial slots (most nodes have at most 4),
-  // loaded unconditionally at indexes clamped into [0, q1 - 1] (the slot
-  // array holds at least one record; nodes of no element have none)
+  // two rounds of independent loads: what s indexes (incl. its first 4
+  // slots, the last repeated when it has fewer), then the node's state and
+  // those slots' partials (most nodes have at most 4)
   constexpr int FIX_BATCH = 4;
   const int g = a.sh_node[s];
   const int q0 = a.sh_off[s], q1 = a.sh_off[s + 1];
+  const int4 f4 = a.sh_first4[s];
+  const int fs[FIX_BATCH] = {f4.x, f4.y, f4.z, f4.w};
   const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
   const double im = a.xyzm[g].w;
   const double4 u = a.U[g];
   const int rec = a.node_rec[g];
   T4 p[FIX_BATCH];
 #pragma unroll
-  for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[max(min(q0 + j, q1 - 1), 0)];
+  for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[fs[j]];
   double F[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < FIX_BATCH; j++)
@@ -689,7 +701,7 @@ __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
       F[2] += (double)p[j].z;
     }
   for (int q = q0 + FIX_BATCH; q < q1; q++) {
-    const T4 r = partials[q];
+    const T4 r = partials[a.sh_slot[q]];
     F[0] += (double)r.x;
     F[1] += (double)r.y;
     F[2] += (double)r.z;
@@ -825,7 +837,8 @@ SupgArgs<T> supg_args(femx_supg *s, bool euler_step, double t) {
   a.Un = s->U[s->cur ^ 1].p;
   a.Unp = s->p.fixed_dofs ? s->Up[s->cur ^ 1].p : nullptr;
   a.Fprev = s->Fprev.p;
-  a.partials = reinterpret_cast<typename Vec4<T>::type *>(s->partials.p);
+  a.partials = s->partials.p;
+  a.f32sum = f32_sum(s->p.precision) ? 1 : 0;
   a.lam = s->lam.p;
   a.regime = s->regime.p;
   a.pres = s->pres.p;
@@ -888,7 +901,7 @@ femx_status launch_supg(femx_supg *s, bool dyn, bool press, bool euler_step, dou
                         const double *prev_row) {
   const char *name = dyn ? (press ? "supg_block_press" : "supg_block") : "supg_press";
   TimedLaunch tl(s->timer, s->timer.kind(name, supg_bytes(s)), s->st);
-  if (s->p.precision == FEMX_MIXED) {
+  if (f32_elem(s->p.precision)) {
     SupgArgs<float> a = supg_args<float>(s, euler_step, t);
     a.prev_row = prev_row;
     a.step_id = (int)step_id;
@@ -906,13 +919,15 @@ femx_status launch_supg(femx_supg *s, bool dyn, bool press, bool euler_step, dou
 femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   Blocks &b = s->mesh->blk;
   if (b.n_sh == 0) return FEMX_OK;
-  const bool mixed = s->p.precision == FEMX_MIXED;
-  const double rec_bytes = mixed ? 16.0 : 32.0;  // one partial slot record, written and read
+  const bool f32p = f32_sum(s->p.precision);  // float4 partial records
+  const double rec_bytes = f32p ? 16.0 : 32.0;  // one partial slot record, written and read
   TimedLaunch tl(s->timer, s->timer.kind("supg_fixup", b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * 200.0), s->st);
   SupgFix a;
   a.n_sh = b.n_sh;
   a.sh_node = b.d_sh_node.p;
   a.sh_off = b.d_sh_off.p;
+  a.sh_slot = b.d_sh_slot.p;
+  a.sh_first4 = b.d_sh_first4.p;
   a.partials = s->partials.p;
   a.xyzm = s->xyzm.p;
   a.U = s->U[s->cur].p;
@@ -926,7 +941,7 @@ femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   a.err = s->bc.err.p;
   a.halt = s->halt.p;
   a.k = supg_const(s, t, euler_step);
-  if (mixed) supg_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  if (f32p) supg_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   else supg_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
@@ -957,7 +972,7 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
   if (m.npe == 4 && p->use_bhat)
     return set_error("B-hat needs hexa8 (eval_avg_shapeDeriv reads 8 nodes)"), FEMX_ERR_UNSUPPORTED;
   if (p->fixed_dofs < 0 || p->fixed_dofs > 6) return set_error("ERROR: Invalid global fixed DOFs string."), FEMX_ERR_ARG;
-  if (p->precision != FEMX_FP64 && p->precision != FEMX_MIXED) return set_error("precision"), FEMX_ERR_ARG;
+  if (!valid_precision(p->precision)) return set_error("precision"), FEMX_ERR_ARG;
   if (!(p->dt > 0.0) || p->rho == 0.0) return set_error("parameters"), FEMX_ERR_ARG;
   const int threads = m.npe == 4 ? 256 : 128;
   if (m.blk.B > threads || m.blk.max_local > 256)

@@ -33,6 +33,8 @@
 // subscales in FP32; nodal state, residual accumulation and the lumped-mass
 // update in FP64).  Coordinate and velocity differences are formed in FP64
 // before the narrowing, so FP32 gradients keep their significant digits.
+// S is the type of a node's in-block sum and of the block partials: double,
+// except for the opt-in FEMX_MIXED_F32SUM (float).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -329,9 +331,8 @@ struct BlockArgs {
   const double4 *xyzm;  // x y z, lumped nodal mass
   const double4 *Q, *P; // post-BC / pre-BC state of the previous iteration
   double4 *Qn, *Pn;     // outputs of this iteration
-  // partial residual sums of block-boundary nodes, in T: with FP32 element
-  // math the in-block sums are FP32 values, so float4 records hold them exactly
-  typename Vec4<T>::type *partials;
+  // partial residual sums of block-boundary nodes: one record of S per slot
+  void *partials;
   T *sdus;              // [12][E] (block order), index (g*3+i)*E + e
   int E;
   int first_dual;       // base of the update: Q at the first dual iteration, P later
@@ -409,7 +410,7 @@ __device__ __forceinline__ double uni(double v) {
 }
 __device__ __forceinline__ double4 uni(double4 v) { return make_double4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
 
-template <typename T, bool PROJ, bool RES, bool ACC>
+template <typename T, typename S, bool PROJ, bool RES, bool ACC>
 __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 4 : 2)) void vms_block(BlockArgs<T> a) {
   using T4 = typename Vec4<T>::type;
   // FP64: staged node records as 16-byte pairs {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}.
@@ -662,16 +663,15 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       base = !ACC ? a.Q[g] : a.P[g];
       im = a.xyzm[g].w;
     }
-    // the node's contributions summed in T (FP32 element math: the sum's
-    // rounding is of the order of the FP32 contributions' own), then FP64
-    T Rt[4] = {(T)0, (T)0, (T)0, (T)0};
+    // the node's contributions summed in S (FP64 unless FEMX_MIXED_F32SUM)
+    S Rt[4] = {(S)0, (S)0, (S)0, (S)0};
 #pragma unroll 4
     for (int s = s0 - e0 * 4; s < s1 - e0 * 4; s++) {
       const T4 v = sres[s];
-      Rt[0] += v.x;
-      Rt[1] += v.y;
-      Rt[2] += v.z;
-      Rt[3] += v.w;
+      Rt[0] += (S)v.x;
+      Rt[1] += (S)v.y;
+      Rt[2] += (S)v.z;
+      Rt[3] += (S)v.w;
     }
     double R[4] = {(double)Rt[0], (double)Rt[1], (double)Rt[2], (double)Rt[3]};
     if (ps_own < 0) {
@@ -680,12 +680,12 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
-      T4 v;
+      typename Vec4<S>::type v;
       v.x = Rt[0];
       v.y = Rt[1];
       v.z = Rt[2];
       v.w = Rt[3];
-      a.partials[ps_own] = v;
+      static_cast<typename Vec4<S>::type *>(a.partials)[ps_own] = v;
     }
   }
   block_max_store<VB>(m2, a.bmax);
@@ -693,7 +693,8 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 
 struct FixArgs {
   int n_sh;
-  const int *sh_node, *sh_off;
+  const int *sh_node, *sh_off, *sh_slot;
+  const int4 *sh_first4;  // each shared node's first 4 slots
   const double *sh_im;  // 1 / nd_mass of every shared node, contiguous
   const int *sh_rec;    // its BC record or -1, contiguous
   const void *partials;  // T4 records of the block kernel
@@ -717,20 +718,21 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double m2 = 0.0;
   if (s < a.n_sh) {
-    // two rounds of independent loads: what s indexes, then the base state
-    // and the first FIX_BATCH partial slots (most nodes have at most 4),
-    // loaded unconditionally at indexes clamped into [0, k1 - 1] (the slot
-    // array holds at least one record; nodes of no element have none)
+    // two rounds of independent loads: what s indexes (incl. its first 4
+    // slots, the last repeated when it has fewer), then the base state and
+    // those slots' partials (most nodes have at most 4)
     constexpr int FIX_BATCH = 4;
     const int g = a.sh_node[s];
     const int k0 = a.sh_off[s], k1 = a.sh_off[s + 1];
+    const int4 f4 = a.sh_first4[s];
     const double im = a.sh_im[s];
     const int rec = a.sh_rec[s];
     const double4 *B = a.first_dual ? a.Q : a.P;
     const double4 base = B[g];
+    const int fs[FIX_BATCH] = {f4.x, f4.y, f4.z, f4.w};
     T4 p[FIX_BATCH];
 #pragma unroll
-    for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[max(min(k0 + j, k1 - 1), 0)];
+    for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[fs[j]];
     double R[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < FIX_BATCH; j++)
@@ -741,7 +743,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
         R[3] += (double)p[j].w;
       }
     for (int k = k0 + FIX_BATCH; k < k1; k++) {
-      const T4 q = partials[k];
+      const T4 q = partials[a.sh_slot[k]];
       R[0] += (double)q.x;
       R[1] += (double)q.y;
       R[2] += (double)q.z;
@@ -986,7 +988,7 @@ VmsConst vms_const(const femx_vms *s, double t) {
 // arrays) once per node.
 double block_bytes(const femx_vms *s, bool proj, bool res) {
   const double E = s->mesh->E, N = s->mesh->N;
-  const double tb = s->p.precision == FEMX_MIXED ? 4.0 : 8.0;
+  const double tb = f32_elem(s->p.precision) ? 4.0 : 8.0;
   double b = E * 4.0 + E * 12 * tb + N * 32.0;
   if (proj) b += E * 12 * tb + N * 32.0;
   if (res) b += E * 8.0 + N * (32.0 + 64.0);
@@ -1011,7 +1013,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.P = s->P[s->cur].p;
   a.Qn = s->Q[s->cur ^ 1].p;
   a.Pn = s->P[s->cur ^ 1].p;
-  a.partials = reinterpret_cast<typename Vec4<T>::type *>(s->partials.p);
+  a.partials = s->partials.p;
   a.sdus = sdus;
   a.E = m.E;
   a.first_dual = first_dual ? 1 : 0;
@@ -1033,24 +1035,26 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   return a;
 }
 
-template <typename T>
-void launch_block_t(femx_vms *s, bool proj, bool res, bool acc, bool first_dual, double t, T *sdus) {
+template <typename T, typename S>
+void launch_block_ts(femx_vms *s, bool proj, bool res, bool acc, bool first_dual, double t, T *sdus) {
   BlockArgs<T> a = block_args<T>(s, first_dual, t, sdus);
   const int grd = s->mesh->blk.nb;
-  if (proj && res && acc) vms_block<T, true, true, true><<<grd, VB, 0, s->st>>>(a);
-  else if (proj && res) vms_block<T, true, true, false><<<grd, VB, 0, s->st>>>(a);
-  else if (res && acc) vms_block<T, false, true, true><<<grd, VB, 0, s->st>>>(a);
-  else if (res) vms_block<T, false, true, false><<<grd, VB, 0, s->st>>>(a);
-  else vms_block<T, true, false, false><<<grd, VB, 0, s->st>>>(a);
+  if (proj && res && acc) vms_block<T, S, true, true, true><<<grd, VB, 0, s->st>>>(a);
+  else if (proj && res) vms_block<T, S, true, true, false><<<grd, VB, 0, s->st>>>(a);
+  else if (res && acc) vms_block<T, S, false, true, true><<<grd, VB, 0, s->st>>>(a);
+  else if (res) vms_block<T, S, false, true, false><<<grd, VB, 0, s->st>>>(a);
+  else vms_block<T, S, true, false, false><<<grd, VB, 0, s->st>>>(a);
 }
 
 femx_status launch_block(femx_vms *s, bool proj, bool res, bool acc, bool first_dual, double t) {
   const char *name = res ? (proj ? "vms_block_proj_res" : "vms_block_res") : "vms_block_proj";
   TimedLaunch tl(s->timer, s->timer.kind(name, block_bytes(s, proj, res)), s->st);
-  if (s->p.precision == FEMX_MIXED)
-    launch_block_t<float>(s, proj, res, acc, first_dual, t, s->sdus32.p);
+  if (f32_sum(s->p.precision))
+    launch_block_ts<float, float>(s, proj, res, acc, first_dual, t, s->sdus32.p);
+  else if (f32_elem(s->p.precision))
+    launch_block_ts<float, double>(s, proj, res, acc, first_dual, t, s->sdus32.p);
   else
-    launch_block_t<double>(s, proj, res, acc, first_dual, t, s->sdus64.p);
+    launch_block_ts<double, double>(s, proj, res, acc, first_dual, t, s->sdus64.p);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
 }
@@ -1058,14 +1062,16 @@ femx_status launch_block(femx_vms *s, bool proj, bool res, bool acc, bool first_
 femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   Blocks &b = s->mesh->blk;
   if (b.n_sh == 0) return FEMX_OK;
-  const bool mixed = s->p.precision == FEMX_MIXED;
-  const double rec_bytes = mixed ? 16.0 : 32.0;  // one partial slot record
-  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 32.0 + 32.0 + 64.0);
+  const bool f32p = f32_sum(s->p.precision);
+  const double rec_bytes = f32p ? 16.0 : 32.0;  // one partial slot record
+  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 16.0 + 32.0 + 32.0 + 64.0);
   TimedLaunch tl(s->timer, s->timer.kind("vms_fixup", bytes), s->st);
   FixArgs a;
   a.n_sh = b.n_sh;
   a.sh_node = b.d_sh_node.p;
   a.sh_off = b.d_sh_off.p;
+  a.sh_slot = b.d_sh_slot.p;
+  a.sh_first4 = b.d_sh_first4.p;
   a.sh_im = s->sh_im.p;
   a.sh_rec = s->sh_rec.p;
   a.partials = s->partials.p;
@@ -1082,7 +1088,7 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.node_mesh = s->batch ? s->node_mesh.p : nullptr;
   a.kt = s->batch ? s->kt.p : nullptr;
   a.k = vms_const(s, t);
-  if (mixed) vms_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
+  if (f32p) vms_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   else vms_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
@@ -1127,7 +1133,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   if (m.npe != 4) return set_error("the VMS solver runs on tet4 meshes only"), FEMX_ERR_UNSUPPORTED;
   if (m.N < 4) return set_error("the VMS projection reads global nodes 0..3: need >= 4 nodes"), FEMX_ERR_ARG;
   if (m.blk.B > VB || m.blk.max_local > VN) return set_error("element blocks do not fit the VMS workgroup"), FEMX_ERR_ARG;
-  if (p->precision != FEMX_FP64 && p->precision != FEMX_MIXED) return set_error("precision"), FEMX_ERR_ARG;
+  if (!valid_precision(p->precision)) return set_error("precision"), FEMX_ERR_ARG;
   if (!(p->dt > 0.0) || p->rho == 0.0 || p->tot_dual_steps < 0) return set_error("parameters"), FEMX_ERR_ARG;
   femx_vms *s = new femx_vms();
   s->mesh = &m;
@@ -1161,7 +1167,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
     if ((r = s->P[k].alloc(m.N)) != FEMX_OK || (r = s->Q[k].alloc(m.N)) != FEMX_OK) return fail(r);
   const int nparts = m.blk.n_sh ? m.blk.sh_off[m.blk.n_sh] : 0;
   if ((r = s->partials.alloc(std::max(nparts, 1))) != FEMX_OK) return fail(r);
-  const bool mixed = p->precision == FEMX_MIXED;
+  const bool mixed = f32_elem(p->precision);
   if ((r = mixed ? s->sdus32.alloc(12L * m.E) : s->sdus64.alloc(12L * m.E)) != FEMX_OK) return fail(r);
   s->n_bmax = m.blk.nb + grid_for(std::max(m.blk.n_sh, 1), 256);
   if ((r = s->bmax.alloc(s->n_bmax)) != FEMX_OK) return fail(r);
@@ -1226,7 +1232,7 @@ extern "C" femx_status femx_vms_set_state(femx_vms *s, const double *sol, const 
   // sol_prev = sol_n = sol_tau (:893-898)
   FEMX_HIP(hipMemcpyAsync(s->P[0].p, s->Q[0].p, m.N * sizeof(double4), hipMemcpyDeviceToDevice, s->st));
   const long ns = 12L * m.E;
-  const bool mixed = s->p.precision == FEMX_MIXED;
+  const bool mixed = f32_elem(s->p.precision);
   if (sdus) {
     femx_status r;
     if ((r = s->scratch.alloc(ns)) != FEMX_OK) return r;
@@ -1271,7 +1277,7 @@ extern "C" femx_status femx_vms_get_state(femx_vms *s, double *sol, double *sdus
   if (sdus) {
     const long ns = 12L * m.E;
     if ((r = s->scratch.alloc(ns)) != FEMX_OK) return r;
-    if (s->p.precision == FEMX_MIXED)
+    if (f32_elem(s->p.precision))
       sdus_from_blocks<float><<<grid_for(ns, 256), 256, 0, s->st>>>(m.E, m.blk.d_perm.p, s->sdus32.p, s->scratch.p);
     else
       sdus_from_blocks<double><<<grid_for(ns, 256), 256, 0, s->st>>>(m.E, m.blk.d_perm.p, s->sdus64.p, s->scratch.p);

@@ -17,7 +17,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # FEMX_LIB selects another build of the same library (A/B comparisons).
 LIB_PATH = os.environ.get("FEMX_LIB", os.path.join(_HERE, "libfemx.so"))
 
-FEMX_FP64, FEMX_MIXED = 0, 1
+# FEMX_MIXED: FP32 element math, FP64 residual accumulation, state and update.
+# FEMX_MIXED_F32SUM (opt-in): as FEMX_MIXED with FP32 in-block node sums and FP32
+# block partials; the sum over blocks, the state and the update stay FP64.
+FEMX_FP64, FEMX_MIXED, FEMX_MIXED_F32SUM = 0, 1, 2
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_BC", 4: "ERR_NOMEM", 5: "ERR_UNSUPPORTED",
           6: "EXIT_CONDITION", 7: "ERR_NO_DEVICE"}
 EXIT_CONDITION = 6

@@ -35,7 +35,12 @@ enum {
   FEMX_ERR_NO_DEVICE = 7     /* no HIP device visible                          */
 };
 
-enum { FEMX_FP64 = 0, FEMX_MIXED = 1 }; /* MIXED: FP32 element math, FP64 state/accumulation */
+/* FEMX_MIXED: FP32 element math; the residual accumulation, nodal state and
+ * lumped-mass update in FP64.  FEMX_MIXED_F32SUM (opt-in, not the default
+ * mixed mode): as FEMX_MIXED, but a node's element contributions inside one
+ * element block are summed in FP32 and the block partials are FP32 records;
+ * the sum over blocks, the state and the update stay FP64. */
+enum { FEMX_FP64 = 0, FEMX_MIXED = 1, FEMX_MIXED_F32SUM = 2 };
 
 /* Mesh as the reference's femModel holds it: nodeList[i]->coords and
  * elementList[e]->elementConnections (0-based), src/femModel.h:24-25. */
@@ -97,7 +102,7 @@ typedef struct {
   double dt;
   double dual_step;
   int tot_dual_steps;
-  int precision;         /* FEMX_FP64 or FEMX_MIXED */
+  int precision;         /* FEMX_FP64, FEMX_MIXED or FEMX_MIXED_F32SUM */
 } femx_vms_params;
 
 /* Replaces the set-up part of femVMSExplicitFluidSolver::solve
@@ -186,7 +191,7 @@ typedef struct {
   int use_euler, use_bhat, k1, k2, k3, k4;
   int fixed_dofs;
   double exit_condition;
-  int precision;         /* FEMX_FP64 or FEMX_MIXED */
+  int precision;         /* FEMX_FP64, FEMX_MIXED or FEMX_MIXED_F32SUM */
 } femx_supg_params;
 
 /* Set-up of femSUPGExplicitFluidSolver::solve (:794-1010) incl.

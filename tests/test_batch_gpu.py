@@ -43,7 +43,7 @@ def _singles(femx, items, ics, steps):
     return out
 
 
-@pytest.mark.parametrize("precision,tot_dual", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("precision,tot_dual", [(0, 0), (1, 0), (0, 1), (2, 0)])
 def test_batch_equals_separate_solves(gpu, precision, tot_dual):
     items, ics = _items(1, precision, tot_dual)
     print("batch of", [m.n_elems for m, _, _ in items], "precision", precision, "dual", tot_dual, flush=True)

@@ -139,9 +139,11 @@ def test_deterministic(gpu):
         assert np.array_equal(outs[0][k], outs[1][k])
 
 
-def test_mixed_precision_error(gpu):
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_error(gpu, precision):
+    """FEMX_MIXED (1) and the opt-in FEMX_MIXED_F32SUM (2) vs the FP64 oracle."""
     m, bc, params, ic = _case("hexa", 6, 0.2, seed=10)
-    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), precision=1)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), precision=precision)
     o.step(60)
     g.step(60)
     err = rel_l2(g.get_state()["sol"], o.get("sol_n"))

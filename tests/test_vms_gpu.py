@@ -64,7 +64,7 @@ def test_fp64_per_step_parity(gpu, kind, n, jitter):
         assert t == o.get("scalars")[0]
 
 
-@pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5)])
+@pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5), (2, 1e-5)])
 def test_nonzero_pressure_and_offset_velocity(gpu, precision, tol):
     """Every node starts with a non-integer pressure, so the pressure
     gradient the projection takes from global nodes 0..3 (:489) is nonzero,
@@ -149,10 +149,12 @@ def test_c1_cavity_100_steps(gpu):
     assert worst <= TOL64, worst
 
 
-def test_mixed_precision_error(gpu):
-    """FP32 element / FP64 state vs the FP64 oracle after 200 steps."""
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_error(gpu, precision):
+    """FP32 element / FP64 state vs the FP64 oracle after 200 steps:
+    FEMX_MIXED (1) and the opt-in FEMX_MIXED_F32SUM (2)."""
     m, bc, params, ic = _case("channel", 8, 0.15, seed=6)
-    o, g, _ = _pair(gpu, m, bc, params, ic, precision=1)
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=precision)
     o.step(200)
     g.step(200)
     sol, _, _ = g.get_state()

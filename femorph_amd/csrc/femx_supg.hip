@@ -12,7 +12,8 @@
 // then k_e = -dt K u_e and Euler / AB2 accumulation into the nodes.
 // Element blocks (<= 128 elements, <= 256 nodes) are assembled in LDS exactly
 // like the VMS kernel: block-complete nodes are updated in the block, block
-// boundary nodes through FP64 partial sums and a fix-up launch.
+// boundary nodes through FP64 partial sums (FP32 in the opt-in
+// FEMX_MIXED_F32SUM) and a fix-up launch.
 //
 // Differences of form (not of value) from the reference, all within
 // rounding: the AB2 history is kept as the assembled nodal force of the

@@ -19,7 +19,8 @@
 //                   in a fixed order (no atomics: deterministic); nodes whose
 //                   elements all live in the block get the lumped-mass update
 //                   (:754-772) and their BCs (:1180-1197) right there, the
-//                   others write one FP64 partial sum.
+//                   others write one FP64 partial sum (FP32 in the opt-in
+//                   FEMX_MIXED_F32SUM).
 //   vms_fixup  one thread per block-boundary node: sums its partials in block
 //              order, then the same update and BCs.
 //

@@ -92,8 +92,20 @@ __device__ __forceinline__ void block_store(const double (&acc)[J], int cnt, dou
 // Sum of part[i * nb .. i * nb + nb) by one wave.
 __device__ __forceinline__ double wave_total(const double *part, int i, int nb) {
   const int lane = threadIdx.x & 63;
+  const double *p = part + (long)i * nb;
+  // per chunk of 1024 partials: all loads first, then the lane's sum in the
+  // order b = lane, lane + 64, ... (any nb: the solve's first reduction has
+  // one partial per block of all rows)
+  constexpr int K = MAXNB / 64;
   double s = 0.0;
-  for (int b = lane; b < nb; b += 64) s += part[(long)i * nb + b];
+  for (int c0 = 0; c0 < nb; c0 += 64 * K) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = c0 + lane + 64 * k < nb ? p[c0 + lane + 64 * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (c0 + lane + 64 * k < nb) s += v[k];
+  }
   return wave_sum(s);
 }
 
@@ -269,7 +281,7 @@ __device__ __forceinline__ void fin_start(GmresState *st, const double *part, in
 // h = the reduced dots; pass 0 sets column j of H, pass 1 adds to it.
 __device__ __forceinline__ void fin_h(GmresState *st, const double *part, int nb, int cnt, int j, int pass) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i = wv; i < cnt; i += RT / 64) {
+  for (int i = wv; i < cnt; i += (int)(blockDim.x >> 6)) {
     const double s = wave_total(part, i, nb);
     if (lane == 0) {
       st->h[i] = s;
@@ -482,7 +494,9 @@ __global__ __launch_bounds__(64) void gm_start(const double *part, int nb, Gmres
   if (!first && st->done) return;
   fin_start(st, part, nb, first != 0);
 }
-__global__ __launch_bounds__(RT) void gm_hfinal(const double *part, int nb, int cnt, GmresState *st, int j,
+// one wave per dot of the pass (MAXM + 1 at most: 16 waves, two dots each)
+constexpr int HFIN_T = 1024;
+__global__ __launch_bounds__(HFIN_T) void gm_hfinal(const double *part, int nb, int cnt, GmresState *st, int j,
                                                 int pass) {
   if (st->cyc_stop) return;
   fin_h(st, part, nb, cnt, j, pass);
@@ -741,7 +755,7 @@ femx_status ins_solve(femx_ins *s, femx_ins_stats *stats) {
         launch_orth(s, w, j, 0);
       }
       for (int mode = 1; mode <= 2; mode++) {
-        gm_hfinal<<<1, RT, 0, s->st>>>(s->part.p, nb, j + 1, s->gst.p, j, mode - 1);
+        gm_hfinal<<<1, HFIN_T, 0, s->st>>>(s->part.p, nb, j + 1, s->gst.p, j, mode - 1);
         TimedLaunch tl(s->timer, s->timer.kind("gm_orth", N * 8.0 * (j + 3) + (mode == 2 ? N * 16.0 : 0.0)),
                        s->st);
         launch_orth(s, w, j, mode);

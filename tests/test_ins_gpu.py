@@ -125,6 +125,23 @@ def test_reproducible_and_restarts(gpu):
     assert rel_l2(outs[2][0], outs[0][0]) <= 1e-9
 
 
+def test_reported_residual_on_a_large_mesh(gpu):
+    """More rows than one reduction pass has blocks (> 1024 x 256): the
+    solve's reductions cover every partial, so the reported relative
+    residual is the true ||b - A x|| / ||b|| of the assembled system."""
+    m, diff, dn, dv, phi, phid = _case("tet", 64, 7)
+    assert m.n_nodes > 1024 * 256
+    g = gpu.INSSolver(gpu.DeviceMesh(m), el_diff=diff, dir_node=dn, dir_val=dv, tol=1e-8, **PARAMS)
+    g.set_state(phi=phi, phi_dot=phid)
+    g.prescribe_velocity(0, 0.3)
+    st = g.advdiff_solve()
+    rp, col, val, rhs, x = g.get_system()
+    A = sp.csr_matrix((val, col, rp), shape=(m.n_nodes, m.n_nodes))
+    true_rel = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
+    assert st["converged"] and true_rel <= 1e-8
+    assert abs(st["rel_residual"] / true_rel - 1.0) <= 1e-6, (st, true_rel)
+
+
 def test_iteration_cap_and_errors(gpu):
     m, g, o = _pair(gpu, "tet", 4, 5, tol=1e-15, max_iter=3)
     g.prescribe_velocity(1, 0.0)

@@ -19,6 +19,11 @@ __constant__ double cN8[8][8], cdN8[8][8][3];
 __constant__ double cN4[4][4], cdN4[4][4][3];
 __constant__ double cW8, cW4;
 __constant__ double cGP8[8][3];  // hexa8 Gauss point coordinates (xi, eta, zeta)
+// the same tables rounded to FP32, for FP32 element math
+__constant__ float fN8[8][8], fdN8[8][8][3];
+__constant__ float fN4[4][4], fdN4[4][4][3];
+__constant__ float fW8, fW4;
+__constant__ float fGP8[8][3];
 
 template <int NPE>
 struct Shape;
@@ -37,11 +42,44 @@ struct Shape<4> {
   __device__ static double W() { return cW4; }
 };
 
-// femUtils::invert3x3Matrix (src/femUtils.cpp:716-770).
-__device__ __forceinline__ void inv3(const double m[3][3], double iv[3][3], double &det) {
+// Shape<NPE> in element-math type T: the FP64 tables, or their FP32 copies.
+template <typename T, int NPE>
+struct ShapeT;
+template <int NPE>
+struct ShapeT<double, NPE> {
+  __device__ static double N(int g, int a) { return Shape<NPE>::N(g, a); }
+  __device__ static double dN(int g, int a, int k) { return Shape<NPE>::dN(g, a, k); }
+  __device__ static double W() { return Shape<NPE>::W(); }
+  __device__ static double GP(int g, int k) { return cGP8[g][k]; }
+};
+template <>
+struct ShapeT<float, 8> {
+  __device__ static float N(int g, int a) { return fN8[g][a]; }
+  __device__ static float dN(int g, int a, int k) { return fdN8[g][a][k]; }
+  __device__ static float W() { return fW8; }
+  __device__ static float GP(int g, int k) { return fGP8[g][k]; }
+};
+template <>
+struct ShapeT<float, 4> {
+  __device__ static float N(int g, int a) { return fN4[g][a]; }
+  __device__ static float dN(int g, int a, int k) { return fdN4[g][a][k]; }
+  __device__ static float W() { return fW4; }
+  __device__ static float GP(int, int) { return 0.0f; }
+};
+
+// Reciprocal and square root in element-math type T: IEEE-rounded in FP64;
+// in FP32 the hardware approximations (about 1 ulp, one instruction each).
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double sqrt_e(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_e(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// femUtils::invert3x3Matrix (src/femUtils.cpp:716-770), in T.
+template <typename T>
+__device__ __forceinline__ void inv3(const T m[3][3], T iv[3][3], T &det) {
   det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-  const double id = 1.0 / det;
+  const T id = recip(det);
   iv[0][0] = (m[1][1] * m[2][2] - m[2][1] * m[1][2]) * id;
   iv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
   iv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
@@ -118,6 +156,29 @@ femx_status upload_tables(int dev) {
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW8), &w8, sizeof(w8)));
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW4), &w4, sizeof(w4)));
   FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cGP8), gp8, sizeof(gp8)));
+  {
+    float f8[8][8], fd8[8][8][3], f4[4][4], fd4[4][4][3], fg8[8][3];
+    for (int g = 0; g < 8; g++)
+      for (int a = 0; a < 8; a++) {
+        f8[g][a] = (float)N8[g][a];
+        for (int k = 0; k < 3; k++) fd8[g][a][k] = (float)dN8[g][a][k];
+      }
+    for (int g = 0; g < 4; g++)
+      for (int a = 0; a < 4; a++) {
+        f4[g][a] = (float)N4[g][a];
+        for (int k = 0; k < 3; k++) fd4[g][a][k] = (float)dN4[g][a][k];
+      }
+    for (int g = 0; g < 8; g++)
+      for (int k = 0; k < 3; k++) fg8[g][k] = (float)gp8[g][k];
+    const float fw8 = (float)w8, fw4 = (float)w4;
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fN8), f8, sizeof(f8)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fdN8), fd8, sizeof(fd8)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fN4), f4, sizeof(f4)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fdN4), fd4, sizeof(fd4)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fW8), &fw8, sizeof(fw8)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fW4), &fw4, sizeof(fw4)));
+    FEMX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fGP8), fg8, sizeof(fg8)));
+  }
   if (dev >= 0 && dev < 64) g_tables_ready[dev] = true;
   return FEMX_OK;
 }

@@ -213,56 +213,60 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
   __syncthreads();
   if (has_e) {
     const SupgConst &k = a.k;
+    using S = ShapeT<T, NPE>;
+    // element math in T: differences of the staged FP64 values are formed
+    // in FP64 and rounded once, so FP32 gradients keep their digits
+    const T mu = (T)k.mu, rho = (T)k.rho;
     // Element kinematics in local coordinates: the physical derivatives
     // DN = invJ dN/dxi are never formed.  Every term the reference builds
     // from DN (velocity gradient, divergence, B-hat strain, the K2/K4 node
     // forces, u.grad N) is the same arithmetic regrouped through the local
     // derivatives of the table and 3x3 products per Gauss point.
-    double U0[3], dU[NPE][3];  // node velocities relative to node 0 (dU[0] = 0)
-    U0[0] = su[id[0]];
-    U0[1] = sv[id[0]];
-    U0[2] = sw[id[0]];
+    T U0[3], dU[NPE][3];  // node velocities relative to node 0 (dU[0] = 0)
+    U0[0] = (T)su[id[0]];
+    U0[1] = (T)sv[id[0]];
+    U0[2] = (T)sw[id[0]];
 #pragma unroll
     for (int n = 0; n < NPE; n++) {
-      dU[n][0] = su[id[n]] - U0[0];
-      dU[n][1] = sv[id[n]] - U0[1];
-      dU[n][2] = sw[id[n]] - U0[2];
+      dU[n][0] = (T)(su[id[n]] - su[id[0]]);
+      dU[n][1] = (T)(sv[id[n]] - sv[id[0]]);
+      dU[n][2] = (T)(sw[id[n]] - sw[id[0]]);
     }
     // hexa8: the trilinear map's Jacobian rows are J_a = E_a + (xi terms),
     // from 7 coefficient vectors (femHexa8 node signs, src/femHexa8.cpp:15-17);
     // tet4 (femTetra4, N3 = 1 - sum xi): constant, J_a = x_a - x_3.
-    double E1[3], E2[3], E3[3], E12[3], E13[3], E23[3], E123[3], Jc[3][3];
+    T E1[3], E2[3], E3[3], E12[3], E13[3], E23[3], E123[3], Jc[3][3];
     {
-      double dx[NPE][3];
+      T dx[NPE][3];
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
-        dx[n][0] = sx[id[n]] - sx[id[0]];
-        dx[n][1] = sy[id[n]] - sy[id[0]];
-        dx[n][2] = sz[id[n]] - sz[id[0]];
+        dx[n][0] = (T)(sx[id[n]] - sx[id[0]]);
+        dx[n][1] = (T)(sy[id[n]] - sy[id[0]]);
+        dx[n][2] = (T)(sz[id[n]] - sz[id[0]]);
       }
       if constexpr (NPE == 8) {
-        constexpr double s1[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, s2[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
-                         s3[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+        constexpr int s1[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, s2[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                      s3[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-          double e1 = 0.0, e2 = 0.0, e3 = 0.0, e12 = 0.0, e13 = 0.0, e23 = 0.0, e123 = 0.0;
+          T e1 = (T)0, e2 = (T)0, e3 = (T)0, e12 = (T)0, e13 = (T)0, e23 = (T)0, e123 = (T)0;
 #pragma unroll
           for (int n = 1; n < 8; n++) {
-            e1 += s1[n] * dx[n][b];
-            e2 += s2[n] * dx[n][b];
-            e3 += s3[n] * dx[n][b];
-            e12 += s1[n] * s2[n] * dx[n][b];
-            e13 += s1[n] * s3[n] * dx[n][b];
-            e23 += s2[n] * s3[n] * dx[n][b];
-            e123 += s1[n] * s2[n] * s3[n] * dx[n][b];
+            e1 += (T)s1[n] * dx[n][b];
+            e2 += (T)s2[n] * dx[n][b];
+            e3 += (T)s3[n] * dx[n][b];
+            e12 += (T)(s1[n] * s2[n]) * dx[n][b];
+            e13 += (T)(s1[n] * s3[n]) * dx[n][b];
+            e23 += (T)(s2[n] * s3[n]) * dx[n][b];
+            e123 += (T)(s1[n] * s2[n] * s3[n]) * dx[n][b];
           }
-          E1[b] = 0.125 * e1;
-          E2[b] = 0.125 * e2;
-          E3[b] = 0.125 * e3;
-          E12[b] = 0.125 * e12;
-          E13[b] = 0.125 * e13;
-          E23[b] = 0.125 * e23;
-          E123[b] = 0.125 * e123;
+          E1[b] = (T)0.125 * e1;
+          E2[b] = (T)0.125 * e2;
+          E3[b] = (T)0.125 * e3;
+          E12[b] = (T)0.125 * e12;
+          E13[b] = (T)0.125 * e13;
+          E23[b] = (T)0.125 * e23;
+          E123[b] = (T)0.125 * e123;
         }
       } else {
 #pragma unroll
@@ -271,9 +275,9 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
           for (int b = 0; b < 3; b++) Jc[r][b] = dx[r][b] - dx[3][b];
       }
     }
-    auto jacobian = [&](int g, double J[3][3]) {
+    auto jacobian = [&](int g, T J[3][3]) {
       if constexpr (NPE == 8) {
-        const double xi = cGP8[g][0], et = cGP8[g][1], ze = cGP8[g][2];
+        const T xi = S::GP(g, 0), et = S::GP(g, 1), ze = S::GP(g, 2);
 #pragma unroll
         for (int b = 0; b < 3; b++) {
           J[0][b] = E1[b] + et * E12[b] + ze * E13[b] + (et * ze) * E123[b];
@@ -289,17 +293,17 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
     };
     // eval_avg_shapeDeriv (:660-707): volume average of DN (hexa only);
     // with it the B-hat strain needs P_k = sum_n avg_nk u_nk per element
-    double avg[NPE][3], P[3] = {0.0, 0.0, 0.0};
+    T avg[NPE][3], P[3] = {(T)0, (T)0, (T)0};
     if (BHAT) {
 #pragma unroll
-      for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = 0.0;
-      double den = 0.0;
+      for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = (T)0;
+      T den = (T)0;
       for (int g = 0; g < NGP; g++) {
-        double J[3][3], iJ[3][3], detJ;
+        T J[3][3], iJ[3][3], detJ;
         jacobian(g, J);
         inv3(J, iJ, detJ);
-        const double dv = detJ * Shape<NPE>::W();
-        double Q[3][3];
+        const T dv = detJ * S::W();
+        T Q[3][3];
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
@@ -308,11 +312,10 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
         for (int n = 0; n < NPE; n++)
 #pragma unroll
           for (int b = 0; b < 3; b++)
-            avg[n][b] += Q[b][0] * Shape<NPE>::dN(g, n, 0) + Q[b][1] * Shape<NPE>::dN(g, n, 1) +
-                         Q[b][2] * Shape<NPE>::dN(g, n, 2);
+            avg[n][b] += Q[b][0] * S::dN(g, n, 0) + Q[b][1] * S::dN(g, n, 1) + Q[b][2] * S::dN(g, n, 2);
         den += dv;
       }
-      const double iden = 1.0 / den;
+      const T iden = recip(den);
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
         avg[n][0] *= iden;
@@ -327,16 +330,16 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
 #pragma unroll
     for (int n = 0; n < NPE; n++) out[n][0] = out[n][1] = out[n][2] = (T)0;
     // per-thread diagnostic accumulators, columns D_TAU .. D_NU+1
-    double vol = 0.0, lam_int = 0.0, reg = 0.0, pnum = 0.0;
-    double Aacc[3] = {0.0, 0.0, 0.0};  // sum over Gauss points of f2 alpha (the B-hat avg term)
-    const double nu_k = k.mu / k.rho;
+    T vol = (T)0, lam_int = (T)0, reg = (T)0, pnum = (T)0;
+    T Aacc[3] = {(T)0, (T)0, (T)0};  // sum over Gauss points of f2 alpha (the B-hat avg term)
+    const T nu_k = mu / rho;
     // tet4: the Jacobian, its inverse and the metric are the same at every Gauss point
-    double iJc[3][3], detJc = 0.0;
+    T iJc[3][3], detJc = (T)0;
     if constexpr (NPE == 4) inv3(Jc, iJc, detJc);
     for (int g = 0; g < NGP; g++) {
-      double iJ[3][3], detJ;
+      T iJ[3][3], detJ;
       if constexpr (NPE == 8) {
-        double J[3][3];
+        T J[3][3];
         jacobian(g, J);
         inv3(J, iJ, detJ);
       } else {
@@ -346,48 +349,48 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
           for (int c = 0; c < 3; c++) iJ[r][c] = iJc[r][c];
         detJ = detJc;
       }
-      const double dv = detJ * Shape<NPE>::W();
+      const T dv = detJ * S::W();
       vol += dv;
       // u at the Gauss point and the velocity gradient gU[b][i] = du_i/dx_b
-      double ug[3] = {U0[0], U0[1], U0[2]}, L[3][3];
+      T ug[3] = {U0[0], U0[1], U0[2]}, L[3][3];
 #pragma unroll
-      for (int c = 0; c < 3; c++) L[c][0] = L[c][1] = L[c][2] = 0.0;
+      for (int c = 0; c < 3; c++) L[c][0] = L[c][1] = L[c][2] = (T)0;
 #pragma unroll
       for (int n = 1; n < NPE; n++) {
-        const double Nn = Shape<NPE>::N(g, n);
+        const T Nn = S::N(g, n);
 #pragma unroll
         for (int i = 0; i < 3; i++) ug[i] += Nn * dU[n][i];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const double d = Shape<NPE>::dN(g, n, c);
+          const T d = S::dN(g, n, c);
           L[c][0] += d * dU[n][0];
           L[c][1] += d * dU[n][1];
           L[c][2] += d * dU[n][2];
         }
       }
-      double gU[3][3];
+      T gU[3][3];
 #pragma unroll
       for (int b = 0; b < 3; b++)
 #pragma unroll
         for (int i = 0; i < 3; i++) gU[b][i] = iJ[b][0] * L[0][i] + iJ[b][1] * L[1][i] + iJ[b][2] * L[2][i];
-      const double div = gU[0][0] + gU[1][1] + gU[2][2];
+      const T div = gU[0][0] + gU[1][1] + gU[2][2];
       if (PRESS) {
         if (PSEP) {
-          double Lp[3][3];
+          T Lp[3][3];
 #pragma unroll
-          for (int c = 0; c < 3; c++) Lp[c][0] = Lp[c][1] = Lp[c][2] = 0.0;
+          for (int c = 0; c < 3; c++) Lp[c][0] = Lp[c][1] = Lp[c][2] = (T)0;
 #pragma unroll
           for (int n = 1; n < NPE; n++) {
-            const double q0 = pu[id[n]] - pu[id[0]], q1 = pv[id[n]] - pv[id[0]], q2 = pw[id[n]] - pw[id[0]];
+            const T q0 = (T)(pu[id[n]] - pu[id[0]]), q1 = (T)(pv[id[n]] - pv[id[0]]), q2 = (T)(pw[id[n]] - pw[id[0]]);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-              const double d = Shape<NPE>::dN(g, n, c);
+              const T d = S::dN(g, n, c);
               Lp[c][0] += d * q0;
               Lp[c][1] += d * q1;
               Lp[c][2] += d * q2;
             }
           }
-          double dp = 0.0;
+          T dp = (T)0;
 #pragma unroll
           for (int b = 0; b < 3; b++) dp += iJ[b][0] * Lp[0][b] + iJ[b][1] * Lp[1][b] + iJ[b][2] * Lp[2][b];
           pnum += dp * dv;
@@ -397,61 +400,63 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       }
       if (!DYN) continue;
       // metric G = invJ^T invJ in the reference's index order (src/femElement.cpp:222-231)
-      double G[3][3];
+      T G[3][3];
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) G[i][j] = iJ[0][i] * iJ[0][j] + iJ[1][i] * iJ[1][j] + iJ[2][i] * iJ[2][j];
       // eval_stabilization (:336-377), tau without the (2/dt)^2 term
-      double t2 = 0.0, t3 = 0.0;
+      T t2 = (T)0, t3 = (T)0;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         t2 += ug[i] * (G[i][0] * ug[0] + G[i][1] * ug[1] + G[i][2] * ug[2]);
         t3 += G[i][0] * G[i][0] + G[i][1] * G[i][1] + G[i][2] * G[i][2];
       }
-      t3 = t3 * nu_k * nu_k * k.cI;
-      const double tau = 1.0 / sqrt(t2 + t3);
-      const double trG = G[0][0] + G[1][1] + G[2][2];
-      const double nuC = 1.0 / (trG * tau);
+      t3 = t3 * nu_k * nu_k * (T)k.cI;
+      const T tau = recip(sqrt_e(t2 + t3));
+      const T trG = G[0][0] + G[1][1] + G[2][2];
+      const T nuC = recip(trG * tau);
       // eval_bulk_modulus (:382-441): lambda = alpha max(mu/rho, sqrt(u^T G^-1 u))
-      double iG[3][3], dG;
+      T iG[3][3], dG;
       inv3(G, iG, dG);
-      double l2 = 0.0;
+      T l2 = (T)0;
 #pragma unroll
       for (int i = 0; i < 3; i++) l2 += (iG[i][0] * ug[0] + iG[i][1] * ug[1] + iG[i][2] * ug[2]) * ug[i];
-      l2 = sqrt(l2);
-      const double lam = k.alpha * ((nu_k < l2) ? l2 : nu_k);
-      reg += (nu_k > l2) ? 1.0 : 2.0;
+      l2 = sqrt_e(l2);
+      const T lam = (T)k.alpha * ((nu_k < l2) ? l2 : nu_k);
+      reg += (nu_k > l2) ? (T)1 : (T)2;
       lam_int += lam * dv;
       if (DIAG) {
-        // get_el_lengths_from_G (:756-790) and the CFL / tau / lambda extrema (:1187-1240)
-        const double un = sqrt(ug[0] * ug[0] + ug[1] * ug[1] + ug[2] * ug[2]);
-        const double o0 = ug[0] / un, o1 = ug[1] / un, o2 = ug[2] / un;
-        const double el = 2.0 * sqrt((iG[0][0] * o0 + iG[0][1] * o1 + iG[0][2] * o2) * o0 +
-                                     (iG[1][0] * o0 + iG[1][1] * o1 + iG[1][2] * o2) * o1 +
-                                     (iG[2][0] * o0 + iG[2][1] * o1 + iG[2][2] * o2) * o2);
-        dg[D_ACFL - D_TAU] = fmax(dg[D_ACFL - D_TAU], fabs(ug[0] * k.dt) / el);
-        dg[D_VCFL - D_TAU] = fmax(dg[D_VCFL - D_TAU], 2.0 * nu_k * (k.dt / (el * el)));
-        dg[D_LCFL - D_TAU] = fmax(dg[D_LCFL - D_TAU], 2.0 * lam * (k.dt / (el * el)));
-        dg[0] = fmin(dg[0], tau);
-        dg[1] = fmax(dg[1], tau);
-        dg[2] = fmin(dg[2], lam);
-        dg[3] = fmax(dg[3], lam);
+        // get_el_lengths_from_G (:756-790) and the CFL / tau / lambda extrema (:1187-1240), in FP64
+        const double u0 = ug[0], u1 = ug[1], u2 = ug[2];
+        const double un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        const double o0 = u0 / un, o1 = u1 / un, o2 = u2 / un;
+        const double el = 2.0 * sqrt(((double)iG[0][0] * o0 + (double)iG[0][1] * o1 + (double)iG[0][2] * o2) * o0 +
+                                     ((double)iG[1][0] * o0 + (double)iG[1][1] * o1 + (double)iG[1][2] * o2) * o1 +
+                                     ((double)iG[2][0] * o0 + (double)iG[2][1] * o1 + (double)iG[2][2] * o2) * o2);
+        const double nk = nu_k, lm = lam, ta = tau;
+        dg[D_ACFL - D_TAU] = fmax(dg[D_ACFL - D_TAU], fabs(u0 * k.dt) / el);
+        dg[D_VCFL - D_TAU] = fmax(dg[D_VCFL - D_TAU], 2.0 * nk * (k.dt / (el * el)));
+        dg[D_LCFL - D_TAU] = fmax(dg[D_LCFL - D_TAU], 2.0 * lm * (k.dt / (el * el)));
+        dg[0] = fmin(dg[0], ta);
+        dg[1] = fmax(dg[1], ta);
+        dg[2] = fmin(dg[2], lm);
+        dg[3] = fmax(dg[3], lm);
       }
       // (u_g . grad) u
-      double c[3];
+      T c[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) c[i] = ug[0] * gU[0][i] + ug[1] * gU[1][i] + ug[2] * gU[2][i];
       // strain B u (:222-296): sum_n DN_nb u_ni = gU[b][i]; with B-hat the
       // diagonal uses b_n = (avg_n - DN_n)/3, whose sums are B_k = (P_k - D_k)/3
-      double eps[6];
+      const T third = (T)(1.0 / 3.0);
+      T eps[6];
       if (BHAT) {
-        const double B0 = (P[0] - gU[0][0]) * (1.0 / 3.0), B1 = (P[1] - gU[1][1]) * (1.0 / 3.0),
-                     B2 = (P[2] - gU[2][2]) * (1.0 / 3.0);
-        const double Bs = B0 + B1 + B2;
-        eps[0] = gU[0][0] - 2.0 * B0 + Bs;
-        eps[1] = gU[1][1] - 2.0 * B1 + Bs;
-        eps[2] = gU[2][2] - 2.0 * B2 + Bs;
+        const T B0 = (P[0] - gU[0][0]) * third, B1 = (P[1] - gU[1][1]) * third, B2 = (P[2] - gU[2][2]) * third;
+        const T Bs = B0 + B1 + B2;
+        eps[0] = gU[0][0] - (T)2 * B0 + Bs;
+        eps[1] = gU[1][1] - (T)2 * B1 + Bs;
+        eps[2] = gU[2][2] - (T)2 * B2 + Bs;
       } else {
         eps[0] = gU[0][0];
         eps[1] = gU[1][1];
@@ -460,28 +465,28 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       eps[3] = gU[1][0] + gU[0][1];
       eps[4] = gU[2][1] + gU[1][2];
       eps[5] = gU[2][0] + gU[0][2];
-      const double trE = eps[0] + eps[1] + eps[2];
-      const double s0 = 2.0 * k.mu * eps[0] + lam * trE, s1 = 2.0 * k.mu * eps[1] + lam * trE,
-                   s2 = 2.0 * k.mu * eps[2] + lam * trE;
-      const double s3 = k.mu * eps[3], s4 = k.mu * eps[4], s5 = k.mu * eps[5];
-      const double ssum = s0 + s1 + s2;
-      const double f1 = k.k1 ? k.rho * dv : 0.0, f3 = k.k3 ? k.rho * tau * dv : 0.0;
-      const double f4 = k.k4 ? k.rho * nuC * div * dv : 0.0, f2 = k.k2 ? dv : 0.0;
+      const T trE = eps[0] + eps[1] + eps[2];
+      const T s0 = (T)2 * mu * eps[0] + lam * trE, s1 = (T)2 * mu * eps[1] + lam * trE,
+              s2 = (T)2 * mu * eps[2] + lam * trE;
+      const T s3 = mu * eps[3], s4 = mu * eps[4], s5 = mu * eps[5];
+      const T ssum = s0 + s1 + s2;
+      const T f1 = k.k1 ? rho * dv : (T)0, f3 = k.k3 ? rho * tau * dv : (T)0;
+      const T f4 = k.k4 ? rho * nuC * div * dv : (T)0, f2 = k.k2 ? dv : (T)0;
       // node forces (K2 u)_n + f4 DN_n = (f2 S + f4 I) DN_n (+ f2 avg_n o alpha
       // with B-hat), with S the stress matrix (B-hat: diagonal (5 s_k - ssum)/3,
       // alpha_k = (ssum - 2 s_k)/3); DN_n = invJ dN_n gives Mg = (f2 S + f4 I) invJ
-      const double sd0 = BHAT ? (5.0 * s0 - ssum) * (1.0 / 3.0) : s0;
-      const double sd1 = BHAT ? (5.0 * s1 - ssum) * (1.0 / 3.0) : s1;
-      const double sd2 = BHAT ? (5.0 * s2 - ssum) * (1.0 / 3.0) : s2;
+      const T sd0 = BHAT ? ((T)5 * s0 - ssum) * third : s0;
+      const T sd1 = BHAT ? ((T)5 * s1 - ssum) * third : s1;
+      const T sd2 = BHAT ? ((T)5 * s2 - ssum) * third : s2;
       if (BHAT) {
-        Aacc[0] += f2 * ((ssum - 2.0 * s0) * (1.0 / 3.0));
-        Aacc[1] += f2 * ((ssum - 2.0 * s1) * (1.0 / 3.0));
-        Aacc[2] += f2 * ((ssum - 2.0 * s2) * (1.0 / 3.0));
+        Aacc[0] += f2 * ((ssum - (T)2 * s0) * third);
+        Aacc[1] += f2 * ((ssum - (T)2 * s1) * third);
+        Aacc[2] += f2 * ((ssum - (T)2 * s2) * third);
       }
-      const double A[3][3] = {{f2 * sd0 + f4, f2 * s3, f2 * s5},
-                              {f2 * s3, f2 * sd1 + f4, f2 * s4},
-                              {f2 * s5, f2 * s4, f2 * sd2 + f4}};
-      double Mg[3][3], w3[3];
+      const T A[3][3] = {{f2 * sd0 + f4, f2 * s3, f2 * s5},
+                         {f2 * s3, f2 * sd1 + f4, f2 * s4},
+                         {f2 * s5, f2 * s4, f2 * sd2 + f4}};
+      T Mg[3][3], w3[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
 #pragma unroll
@@ -490,26 +495,26 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       }
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
-        const double d0 = Shape<NPE>::dN(g, n, 0), d1 = Shape<NPE>::dN(g, n, 1), d2 = Shape<NPE>::dN(g, n, 2);
-        const double cv = f1 * Shape<NPE>::N(g, n) + f3 * (w3[0] * d0 + w3[1] * d1 + w3[2] * d2);
-        out[n][0] += (T)(cv * c[0] + (Mg[0][0] * d0 + Mg[0][1] * d1 + Mg[0][2] * d2));
-        out[n][1] += (T)(cv * c[1] + (Mg[1][0] * d0 + Mg[1][1] * d1 + Mg[1][2] * d2));
-        out[n][2] += (T)(cv * c[2] + (Mg[2][0] * d0 + Mg[2][1] * d1 + Mg[2][2] * d2));
+        const T d0 = S::dN(g, n, 0), d1 = S::dN(g, n, 1), d2 = S::dN(g, n, 2);
+        const T cv = f1 * S::N(g, n) + f3 * (w3[0] * d0 + w3[1] * d1 + w3[2] * d2);
+        out[n][0] += cv * c[0] + (Mg[0][0] * d0 + Mg[0][1] * d1 + Mg[0][2] * d2);
+        out[n][1] += cv * c[1] + (Mg[1][0] * d0 + Mg[1][1] * d1 + Mg[1][2] * d2);
+        out[n][2] += cv * c[2] + (Mg[2][0] * d0 + Mg[2][1] * d1 + Mg[2][2] * d2);
       }
     }
     if (BHAT && DYN) {
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
-        out[n][0] += (T)(avg[n][0] * Aacc[0]);
-        out[n][1] += (T)(avg[n][1] * Aacc[1]);
-        out[n][2] += (T)(avg[n][2] * Aacc[2]);
+        out[n][0] += avg[n][0] * Aacc[0];
+        out[n][1] += avg[n][1] * Aacc[1];
+        out[n][2] += avg[n][2] * Aacc[2];
       }
     }
-    if (PRESS) a.pres[e] = -lam_prev * (pnum / vol);  // compute_pressure of the previous step (:498)
+    if (PRESS) a.pres[e] = -lam_prev * ((double)pnum / (double)vol);  // compute_pressure of the previous step (:498)
     if (DYN) {
-      const double lame = lam_int / vol;  // :1446
+      const double lame = (double)lam_int / (double)vol;  // :1446
       a.lam[e] = lame;
-      a.regime[e] = reg / NGP;
+      a.regime[e] = (double)reg / NGP;
       if (DIAG) {
         const double cE = (k.mu * (3 * lame + 2 * k.mu)) / (lame + k.mu), cnu = lame / (2 * (lame + k.mu));
         dg[D_E - D_TAU] = fmin(dg[D_E - D_TAU], cE);

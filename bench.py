@@ -371,8 +371,8 @@ def workload_c2(args):
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
                         "kernel_ms": round(dom_ms, 4), "kernel_bytes": dom["bytes_per_launch"],
-                        "note": "VALU-bound kernel (about 7.4k VALU instructions per element-step, "
-                                "~69% VALU busy); the HBM fraction is reported for reference",
+                        "note": "compute-heavy kernel (about 6.4k VALU instructions per element-step, "
+                                "VALU issue ~55% busy); the HBM fraction is reported for reference",
                         "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4))
                                     for k, v in kt.items()}},
            other: {"value": round(E * args.steps / res[other][0], 1),

@@ -122,6 +122,26 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+CEILING_PROBE = "best of femx_hbm_probe mix_7r5w (7 read / 5 write float4 streams, 1 GiB) over 1024/4096/16384 workgroups"
+
+
+def stream_ceiling_gbs(reps: int = 10) -> float:
+    """Practical HBM ceiling for a read/write mix like the VMS block kernel's."""
+    return max(femx.hbm_probe("mix_7r5w", 1 << 30, g, reps) for g in (1024, 4096, 16384)) * 1e3
+
+
+def workload_hbm(args):
+    """Streaming bandwidth of float4 grid-stride kernels (femx_hbm_probe):
+    the practical ceilings the roofline fractions are read against."""
+    res = {}
+    for kind in ("copy", "read", "write", "mix_7r5w"):
+        res[kind] = {str(g): round(femx.hbm_probe(kind, 1 << 30, g, 20) * 1e3, 1) for g in (1024, 4096, 16384)}
+        print(f"hbm {kind}: {res[kind]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "HBM streaming bandwidth (float4 grid-stride kernels, 1 GiB per buffer)",
+                      "unit": "GB/s", "peak": HBM_PEAK_GBS, "by_kind_and_grid": res,
+                      "best": {k: max(v.values()) for k, v in res.items()}}), flush=True)
+
+
 def _c4_items(seed, precision, count=256):
     """256 lid-driven VMS tet4 cavities, n log-uniform in [8, 48], Re uniform
     in [50, 500], IC uniform +-1e-3 (the C1 physics per mesh)."""
@@ -574,8 +594,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins"], default="c3",
-                    help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request stream")
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins", "hbm"], default="c3",
+                    help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request "
+                         "stream; hbm: streaming-bandwidth probe")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ins-elem", choices=["tet4", "hexa8"], default="tet4", help="ins: element type")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
@@ -619,6 +640,9 @@ def main():
         if args.steps == 50:
             args.steps = 1000
         workload_accuracy(args)
+        return
+    if args.workload == "hbm":
+        workload_hbm(args)
         return
 
     m, bc, params, ic = make_workload(args.n)
@@ -672,6 +696,9 @@ def main():
         "finite_state": all(v["finite"] for v in results.values()),
         "event_step_ms": round(step_ms_events, 4),
     }
+    ceil_gbs = stream_ceiling_gbs()
+    roof["stream_ceiling"] = {"gbs": round(ceil_gbs, 1), "probe": CEILING_PROBE,
+                              "achieved_frac": round(achieved / ceil_gbs, 4)}
     if not args.no_cpu:
         cv, csteps, csecs = cpu_baseline(m, bc, params, ic, args.cpu_budget)
         out["cpu_baseline"] = {"value": round(cv, 1), "unit": "element-steps/s", "cores": 1, "kind": "port",

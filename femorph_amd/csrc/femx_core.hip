@@ -369,6 +369,77 @@ extern "C" int femx_device_count(void) {
   return n;
 }
 
+// ------------------------------------------------ HBM streaming probe
+// Grid-stride float4 streams over buffers of `n4` float4 each; every access
+// index is < n4 (the mix reads 7 rows and writes 5 rows of m4 = n4 / 7).
+namespace {
+__global__ void probe_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long n4) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+__global__ void probe_read(const float4 *__restrict__ a, float *__restrict__ out, long n4) {
+  float s = 0.f;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+    const float4 v = a[i];
+    s += (v.x + v.y) + (v.z + v.w);
+  }
+  if (s == 1234.5f) out[blockIdx.x] = s;  // never true for the zero-filled input; keeps the loads
+}
+__global__ void probe_write(float4 *__restrict__ b, long n4) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+    b[i] = make_float4(1.f, 2.f, 3.f, 4.f);
+}
+__global__ void probe_mix(const float4 *__restrict__ a, float4 *__restrict__ b, long m4) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m4; i += (long)gridDim.x * blockDim.x) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+      const float4 v = a[r * m4 + i];
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
+    }
+#pragma unroll
+    for (int w = 0; w < 5; w++) b[w * m4 + i] = s;
+  }
+}
+}  // namespace
+
+extern "C" femx_status femx_hbm_probe(int kind, long long bytes, int grid, int reps, double *tb_per_s) {
+  if (!tb_per_s || kind < 0 || kind > 3 || bytes < (1 << 20) || grid < 1 || grid > (1 << 20) || reps < 1)
+    return set_error("femx_hbm_probe: arguments"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  const long n4 = (long)(bytes / 16), m4 = n4 / 7;
+  DevBuf<float4> a, b;
+  DevBuf<float> out;
+  femx_status r;
+  if ((r = a.alloc(n4)) != FEMX_OK || (r = b.alloc(n4)) != FEMX_OK || (r = out.alloc(grid)) != FEMX_OK) return r;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto launch = [&]() {
+    if (kind == 0) probe_copy<<<grid, 256, 0, st>>>(a.p, b.p, n4);
+    else if (kind == 1) probe_read<<<grid, 256, 0, st>>>(a.p, out.p, n4);
+    else if (kind == 2) probe_write<<<grid, 256, 0, st>>>(b.p, n4);
+    else probe_mix<<<grid, 256, 0, st>>>(a.p, b.p, m4);
+  };
+  float ms = 0.f;
+  bool ok = hipMemsetAsync(a.p, 0, n4 * sizeof(float4), st) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  if (ok) {
+    launch();  // warm-up
+    ok = hipEventRecord(e0, st) == hipSuccess;
+    for (int i = 0; ok && i < reps; i++) launch();
+    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, st) == hipSuccess &&
+         hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!ok || ms <= 0.f) return set_error("femx_hbm_probe: HIP launch or timing failed"), FEMX_ERR_HIP;
+  const double moved = kind == 0 ? 2.0 * n4 * 16 : kind == 3 ? 12.0 * m4 * 16 : 1.0 * n4 * 16;
+  *tb_per_s = moved * reps / (ms * 1e-3) / 1e12;
+  return FEMX_OK;
+}
+
 femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
                              const std::vector<int> *elem_group) {
   m.N = (int)N;

@@ -166,6 +166,18 @@ template <typename T>
 struct ElemVel {
   T a0[4][3], grad[3][3];
 };
+
+// The subgrid velocity sdu(g, i) of the thread's element at Gauss point g:
+// in registers, or in LDS as row g * 3 + i of a [12][VB] block array.
+template <typename T>
+struct SduReg {
+  T (*v)[3];
+  __device__ T &operator()(int g, int i) const { return v[g][i]; }
+};
+struct SduLds {
+  float *p;  // &rows[0][thread]
+  __device__ float &operator()(int g, int i) const { return p[(g * 3 + i) * VB]; }
+};
 template <typename T>
 __device__ __forceinline__ void elem_vel(const T DN[3][4], const double uv[4][3], ElemVel<T> &ev) {
   T u[4][3], du[3][3];
@@ -212,9 +224,9 @@ __device__ __forceinline__ void elem_vel_rel(const float DN[3][4], const float r
 // :406-588 and update_subscales_explicit :593-646, element-based projection,
 // include_sg_p = false so the pressure subscale stays 0).
 // ev: pre-BC nodal velocity; q: pressures of global nodes 0..3.
-template <typename T>
+template <typename T, typename SDU>
 __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const ElemVel<T> &ev, const double q[4],
-                                               T sdu[4][3], const VmsConst &k) {
+                                               SDU sdu, const VmsConst &k) {
   T dq[3];
 #pragma unroll
   for (int a = 1; a < 4; a++) dq[a - 1] = (T)(q[a] - q[0]);
@@ -227,7 +239,7 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const Ele
   for (int g = 0; g < 4; g++) {
     T hah[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) hah[i] = ev.a0[g][i] + sdu[g][i];  // convective velocity :529-535
+    for (int i = 0; i < 3; i++) hah[i] = ev.a0[g][i] + sdu(g, i);  // convective velocity :529-535
     const T m2 = hah[0] * hah[0] + hah[1] * hah[1] + hah[2] * hah[2];
     maxA2 = m2 > maxA2 ? m2 : maxA2;
 #pragma unroll
@@ -248,7 +260,7 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const Ele
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) sdu[g][i] = ka * sdu[g][i] + (kb * r[g][i] + cs[i]);
+    for (int i = 0; i < 3; i++) sdu(g, i) = ka * sdu(g, i) + (kb * r[g][i] + cs[i]);
 }
 
 // Element residual lRes[4][4] of assemble_RHS (:93-399) with sgps = 0 and
@@ -256,15 +268,14 @@ __device__ __forceinline__ void vms_projection(const T DN[3][4], T ih, const Ele
 // read only with ACC, the acceleration term of later dual iterations.
 // Diffusion, pressure gradient and subgrid convection are one 3x3 matrix K
 // applied to DN: res[b][i] = sum_j K[i][j] DN[j][b] + convection.
-template <typename T, bool ACC>
+template <typename T, bool ACC, typename SDU>
 __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVel<T> &ev, const T p[4],
-                                             const T dd[4][3], const T sdu[4][3], T res[4][4],
-                                             const VmsConst &k) {
+                                             const T dd[4][3], SDU sdu, T res[4][4], const VmsConst &k) {
   const T(&gradU)[3][3] = ev.grad;
   T ss[3];
   const T psum = (p[0] + p[1]) + (p[2] + p[3]);
 #pragma unroll
-  for (int i = 0; i < 3; i++) ss[i] = (sdu[0][i] + sdu[1][i]) + (sdu[2][i] + sdu[3][i]);
+  for (int i = 0; i < 3; i++) ss[i] = (sdu(0, i) + sdu(1, i)) + (sdu(2, i) + sdu(3, i));
   const T tr = gradU[0][0] + gradU[1][1] + gradU[2][2];
   const T w = (T)0.25 * V;
   const T gb = (T)GB, gd = (T)GD;
@@ -279,13 +290,13 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
   for (int g = 0; g < 4; g++) {
     T ag[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) ag[i] = ev.a0[g][i] + sdu[g][i];
+    for (int i = 0; i < 3; i++) ag[i] = ev.a0[g][i] + sdu(g, i);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       agu[g][i] = gradU[i][0] * ag[0] + gradU[i][1] * ag[1] + gradU[i][2] * ag[2];
       ags[i] += agu[g][i];
 #pragma unroll
-      for (int j = 0; j < 3; j++) Tm[i][j] += sdu[g][i] * ag[j];
+      for (int j = 0; j < 3; j++) Tm[i][j] += sdu(g, i) * ag[j];
     }
   }
   // K = V nu grad u  -  w T  -  w sum_g p_g I   (sum_g w = V, sum_g p_g = S sum_b p_b)
@@ -323,7 +334,12 @@ constexpr int BC_BIT = 0x8000;
 template <typename T>
 struct BlockArgs {
   int nblk;
-  const int *elem_off, *node_off, *loc_node, *loc_pslot, *lcsr_off;
+  const int *elem_off, *node_off;
+  // per local node {global node, slot, run, 0}: slot >= 0 is the partial slot
+  // of a block-boundary node; a complete node holds -1 without a BC record
+  // and -2 - record with one.  run packs the node's contributions in the
+  // block's node-sorted order as begin | end << 16.
+  const int4 *lnode;
   // per element: node-sorted position of each corner's contribution (bits
   // 0..14) and BC_BIT when the corner's node has a BC record, so its pre-BC
   // and post-BC states may differ
@@ -337,7 +353,6 @@ struct BlockArgs {
   T *sdus;              // [12][E] (block order), index (g*3+i)*E + e
   int E;
   int first_dual;       // base of the update: Q at the first dual iteration, P later
-  const int *node_rec;
   const BCRecord *rec;
   const int *row_off;
   const double4 *rows;
@@ -420,11 +435,29 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   //   sfa = (x y z pu), sfb = (pv pw qu qv), sfc = (qw, qp)
   constexpr bool F32 = sizeof(T) == 4;
   __shared__ double2 snd[F32 ? 1 : VN][5];
-  __shared__ float4 sfa[F32 ? VN : 1], sfb[F32 ? VN : 1];
-  __shared__ float2 sfc[F32 ? VN : 1];
   // element-corner contributions (u v w p) in the block's node-sorted order,
   // one T4 record per corner
-  __shared__ T4 sres[RES ? 4 * VB : 1];
+  __shared__ T4 sres64[RES && !F32 ? 4 * VB : 1];
+  // FP32 element math: the phase-2 inputs (staged nodes, the subgrid
+  // velocities of the block's elements as rows [g * 3 + i][thread]) and the
+  // phase-3 contributions share one LDS region; a barrier separates the last
+  // phase-2 read from the first contribution write
+  struct F32Stage {
+    float4 sfa[VN], sfb[VN];
+    float2 sfc[VN];
+    float sdl[12][VB];
+  };
+  union F32Lds {
+    F32Stage st;
+    float4 sres[4 * VB];
+  };
+  __shared__ __align__(16) unsigned char lraw[F32 ? sizeof(F32Lds) : 16];
+  F32Lds &L = *reinterpret_cast<F32Lds *>(lraw);
+  float4 *sfa = L.st.sfa, *sfb = L.st.sfb;
+  float2 *sfc = L.st.sfc;
+  T4 *sres;
+  if constexpr (F32) sres = L.sres;
+  else sres = sres64;
   const int tid = threadIdx.x;
   const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
   VmsConst k = a.k;
@@ -459,24 +492,20 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 #pragma unroll
       for (int i = 0; i < 3; i++) sdu[g][i] = *row_ptr(a.sdus, (g * 3 + i) * E, eoff);
   }
-  // the local node this thread owns (nn <= VN <= VB); with FP32 element
-  // math its phase-3 data is re-read there (from L2) to keep phase 2 within
-  // 96 VGPRs, with FP64 it stays in registers
+  // the local node this thread owns (nn <= VN <= VB): one record holds all
+  // its block data, so no load of the node phase waits on another
   const bool has_n = tid < nn;
-  int g_own = 0, rec_own = -1, ps_keep = -1, s0_keep = 0, s1_keep = 0;
+  int g_own = 0, slot_own = -1, run_own = 0;
   if (has_n) {
-    g_own = a.loc_node[n0 + tid];
-    rec_own = a.node_rec[g_own];
-    if (!F32 && RES) {
-      ps_keep = a.loc_pslot[n0 + tid];
-      s0_keep = a.lcsr_off[n0 + tid];
-      s1_keep = a.lcsr_off[n0 + tid + 1];
-    }
+    const int4 ln = a.lnode[n0 + tid];
+    g_own = ln.x;
+    slot_own = ln.y;
+    run_own = ln.z;
   }
   // reference node of the FP32 relative values: the block's first local node
   double4 xr = make_double4(0, 0, 0, 0), pr = make_double4(0, 0, 0, 0), qr = make_double4(0, 0, 0, 0);
   if (F32) {
-    const int gr = uni(a.loc_node[n0]);
+    const int gr = uni(a.lnode[n0].x);
     xr = uni(a.xyzm[gr]);
     if (PROJ || ACC) pr = uni(a.P[gr]);
     if (RES) qr = uni(a.Q[gr]);
@@ -507,6 +536,18 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       snd[j][4] = make_double2(q_own.z, q_own.w);
     }
   }
+  if (F32 && has_e) {
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) L.st.sdl[g * 3 + i][tid] = (float)sdu[g][i];
+  }
+  // the thread's subgrid velocities: LDS rows with FP32 element math (out of
+  // registers through phase 2), registers with FP64
+  using SD = std::conditional_t<F32, SduLds, SduReg<T>>;
+  SD sd;
+  if constexpr (F32) sd.p = &L.st.sdl[0][tid];
+  else sd.v = sdu;
   double q4[4];
   if (PROJ) {
     // p[i] = sol_n[i][3]: pressures of GLOBAL nodes 0..3 (:489)
@@ -515,6 +556,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   }
   __syncthreads();
   // 2. one element per thread
+  T res[4][4];
   if (has_e) {
     const int id[4] = {c.x, c.y, c.z, c.w};
     T DN[3][4], V, ih;
@@ -539,7 +581,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       const float pref[3] = {(float)pr.x, (float)pr.y, (float)pr.z};
       if (PROJ) {
         elem_vel_rel(DN, pref, up, ev);
-        vms_projection<float>(DN, ih, ev, q4, sdu, k);
+        vms_projection<float>(DN, ih, ev, q4, sd, k);
       }
       if (RES) {
 #pragma unroll
@@ -583,7 +625,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       tet_geometry_t<T>(X, DN, V, ih);
       if (PROJ) {
         elem_vel<T>(DN, up, ev);
-        vms_projection<T>(DN, ih, ev, q4, sdu, k);
+        vms_projection<T>(DN, ih, ev, q4, sd, k);
       }
       if (RES) {
 #pragma unroll
@@ -610,11 +652,10 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 #pragma unroll
       for (int g = 0; g < 4; g++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) *row_ptr(a.sdus, (g * 3 + i) * E, eoff) = sdu[g][i];
+        for (int i = 0; i < 3; i++) *row_ptr(a.sdus, (g * 3 + i) * E, eoff) = sd(g, i);
     }
     if (RES) {
-      T res[4][4];
-      vms_residual<T, ACC>(DN, V, ev, p, dd, sdu, res, k);
+      vms_residual<T, ACC>(DN, V, ev, p, dd, sd, res, k);
       if (a.diag_res) {
         const size_t eo = (size_t)a.perm[e];
         double *d = a.diag_res + eo * 16;
@@ -626,48 +667,47 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
         for (int i = 0; i < 3; i++) {
           double sm = 0.0;
 #pragma unroll
-          for (int g = 0; g < 4; g++) sm += (double)sdu[g][i] * 0.25;
+          for (int g = 0; g < 4; g++) sm += (double)sd(g, i) * 0.25;
           a.diag_sdu[eo * 3 + i] = sm;
         }
-      }
-      const int ps[4] = {pos.x & ~BC_BIT, pos.y & ~BC_BIT, pos.z & ~BC_BIT, pos.w & ~BC_BIT};
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        T4 v;
-        v.x = res[b][0];
-        v.y = res[b][1];
-        v.z = res[b][2];
-        v.w = res[b][3];
-        sres[ps[b]] = v;
       }
     }
   }
   if (!RES) return;
-  // FP32 element math re-reads the owned node's slot and run bounds (from
-  // L2): issued before the barrier, their latency overlaps its wait
-  int ps_own = ps_keep, s0 = s0_keep, s1 = s1_keep;
-  if (F32 && has_n) {
-    ps_own = a.loc_pslot[n0 + tid];
-    s0 = a.lcsr_off[n0 + tid];
-    s1 = a.lcsr_off[n0 + tid + 1];
+  // with FP32 element math the contributions overwrite the phase-2 inputs
+  if (F32) __syncthreads();
+  if (has_e) {
+    const int ps[4] = {pos.x & ~BC_BIT, pos.y & ~BC_BIT, pos.z & ~BC_BIT, pos.w & ~BC_BIT};
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      T4 v;
+      v.x = res[b][0];
+      v.y = res[b][1];
+      v.z = res[b][2];
+      v.w = res[b][3];
+      sres[ps[b]] = v;
+    }
+  }
+  // base of update_dual_solution (post-BC at the first dual iteration,
+  // pre-BC after) and 1/m of a complete node: FP32 element math re-reads
+  // them (from L2) to keep phase 2 within 96 VGPRs, issued before the
+  // barrier so that their latency overlaps its wait; FP64 keeps them
+  double4 base = base_keep;
+  double im = im_keep;
+  if (F32 && has_n && slot_own < 0) {
+    base = !ACC ? a.Q[g_own] : a.P[g_own];
+    im = a.xyzm[g_own].w;
   }
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
   double m2 = 0.0;
   if (has_n) {
     const int g = g_own;
-    // base of update_dual_solution (post-BC at the first dual iteration,
-    // pre-BC after) and 1/m of an owned node, loaded ahead of the sum
-    double4 base = base_keep;
-    double im = im_keep;
-    if (F32 && ps_own < 0) {
-      base = !ACC ? a.Q[g] : a.P[g];
-      im = a.xyzm[g].w;
-    }
     // the node's contributions summed in S (FP64 unless FEMX_MIXED_F32SUM)
     S Rt[4] = {(S)0, (S)0, (S)0, (S)0};
+    const int s1 = run_own >> 16;
 #pragma unroll 4
-    for (int s = s0 - e0 * 4; s < s1 - e0 * 4; s++) {
+    for (int s = run_own & 0xffff; s < s1; s++) {
       const T4 v = sres[s];
       Rt[0] += (S)v.x;
       Rt[1] += (S)v.y;
@@ -675,9 +715,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       Rt[3] += (S)v.w;
     }
     double R[4] = {(double)Rt[0], (double)Rt[1], (double)Rt[2], (double)Rt[3]};
-    if (ps_own < 0) {
+    if (slot_own < 0) {
       double4 pre, post;
-      m2 = node_update(base, im, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, im, R, k, pre, post, -2 - slot_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -686,7 +726,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       v.y = Rt[1];
       v.z = Rt[2];
       v.w = Rt[3];
-      static_cast<typename Vec4<S>::type *>(a.partials)[ps_own] = v;
+      static_cast<typename Vec4<S>::type *>(a.partials)[slot_own] = v;
     }
   }
   block_max_store<VB>(m2, a.bmax);
@@ -938,6 +978,7 @@ struct femx_vms {
   DevBuf<double> scratch;  // staging of reference-layout arrays
   DevBuf<int> rec_node;
   DevBuf<uint16_t> lpos_bc;  // the mesh's corner positions with BC_BIT set per the BC records
+  DevBuf<int4> lnode;        // per local node: global node, slot or BC record, contribution run
   int cur = 0;
   bool proj_pending = false;
   long iter = 0;
@@ -1004,9 +1045,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.nblk = b.nb;
   a.elem_off = b.d_elem_off.p;
   a.node_off = b.d_node_off.p;
-  a.loc_node = b.d_loc_node.p;
-  a.loc_pslot = b.d_loc_pslot.p;
-  a.lcsr_off = b.d_lcsr_off.p;
+  a.lnode = s->lnode.p;
   a.lpos = reinterpret_cast<const ushort4 *>(s->lpos_bc.p);
   a.lconn = reinterpret_cast<const uchar4 *>(b.d_lconn.p);
   a.xyzm = s->xyzm.p;
@@ -1018,7 +1057,6 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.sdus = sdus;
   a.E = m.E;
   a.first_dual = first_dual ? 1 : 0;
-  a.node_rec = s->bc.node_rec.p;
   a.rec = s->bc.rec.p;
   a.row_off = s->bc.row_off.p;
   a.rows = s->bc.rows.p;
@@ -1162,6 +1200,22 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
           if (has_rec[g]) v |= BC_BIT;
         }
     if ((r = s->lpos_bc.upload(lp.data(), lp.size(), s->st)) != FEMX_OK) return fail(r);
+    // per local node record of the block kernel: a block's contributions
+    // number < 4 x 256, so the run bounds fit 16 bits each
+    const Blocks &b = m.blk;
+    std::vector<int> node_rec(m.N, -1);
+    for (size_t i = 0; i < s->bc.h_rec_node.size(); i++) node_rec[s->bc.h_rec_node[i]] = (int)i;
+    std::vector<int4> ln(std::max(b.total_local, 1), make_int4(0, -1, 0, 0));
+    for (int blk = 0; blk < b.nb; blk++) {
+      const int cbase = 4 * b.elem_off[blk];
+      for (int l = b.node_off[blk]; l < b.node_off[blk + 1]; l++) {
+        const int g = b.loc_node[l], ps = b.loc_pslot[l];
+        const int c0 = b.lcsr_off[l] - cbase, c1 = b.lcsr_off[l + 1] - cbase;
+        if (c0 < 0 || c1 > 0xffff || c0 > c1) return fail((set_error("block contribution runs overflow 16 bits"), FEMX_ERR_ARG));
+        ln[l] = make_int4(g, ps >= 0 ? ps : -2 - node_rec[g], c0 | (c1 << 16), 0);
+      }
+    }
+    if ((r = s->lnode.upload(ln.data(), ln.size(), s->st)) != FEMX_OK) return fail(r);
   }
   if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->mass.alloc(m.N)) != FEMX_OK) return fail(r);
   for (int k = 0; k < 2; k++)

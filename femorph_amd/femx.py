@@ -149,6 +149,7 @@ SIGNATURES = [
     ("femx_ins_stream", C.c_void_p, [C.c_void_p]),
     ("femx_ins_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_ins_kernel_times", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int]),
+    ("femx_hbm_probe", C.c_int, [C.c_int, C.c_longlong, C.c_int, C.c_int, _dp]),
 ]
 
 _LIB = None
@@ -207,6 +208,17 @@ def block_stats(mesh: Mesh, block_elems: int) -> dict:
 
 def device_count() -> int:
     return load().femx_device_count()
+
+
+HBM_PROBE_KINDS = {"copy": 0, "read": 1, "write": 2, "mix_7r5w": 3}
+
+
+def hbm_probe(kind: str, nbytes: int = 1 << 30, grid: int = 4096, reps: int = 20) -> float:
+    """TB/s of a float4 streaming kernel (femx_hbm_probe): copy, read,
+    write, or a 7-read / 5-write mix, over buffers of nbytes."""
+    out = C.c_double(0.0)
+    _check(load().femx_hbm_probe(HBM_PROBE_KINDS[kind], int(nbytes), int(grid), int(reps), C.byref(out)))
+    return out.value
 
 
 class DeviceMesh:

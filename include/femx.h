@@ -365,6 +365,15 @@ int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int max_entries)
 femx_status femx_ins_kernel_timing(femx_ins *s, int enable);
 int femx_ins_kernel_times(femx_ins *s, femx_kernel_time *out, int max_entries);
 
+/* HBM streaming probe (no counterpart in the reference: it measures the
+ * practical bandwidth ceiling the kernels' roofline fractions are read
+ * against).  kind: 0 copy (1 read + 1 write stream), 1 read, 2 write,
+ * 3 mix of 7 read and 5 write streams (about the VMS block kernel's ratio);
+ * float4 grid-stride kernels over `bytes` per buffer, `grid` workgroups of
+ * 256 threads, `reps` timed launches after one warm-up.  Allocates two
+ * buffers of `bytes` on the current device and frees them. */
+femx_status femx_hbm_probe(int kind, long long bytes, int grid, int reps, double *tb_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,9 +105,9 @@ struct Request {
 // Device bytes a request will hold (upper estimate of the mesh, block
 // tables, state and per-element arrays of femx_mesh_upload + the solver).
 size_t footprint(long N, long E, int npe, bool vms, bool mixed) {
-  // a VMS solver also holds a 16-byte record per local node, and a mesh has
+  // a VMS solver also holds an 8-byte record per local node, and a mesh has
   // at most npe local nodes per element
-  const double per_elem = npe * (4.0 + 4.0 + 1.0 + 2.0) + 4.0 + (vms ? 12.0 * (mixed ? 4 : 8) + npe * 16.0 : 24.0 + 64.0);
+  const double per_elem = npe * (4.0 + 4.0 + 1.0 + 2.0) + 4.0 + (vms ? 12.0 * (mixed ? 4 : 8) + npe * 8.0 : 24.0 + 64.0);
   const double per_node = 32.0 + 32.0 + 8.0 + 4 * 32.0 + 4.0 + 4.0 + 3 * 12.0 + 32.0 + 16.0;
   return (size_t)(E * per_elem + N * per_node) + (1u << 20);
 }

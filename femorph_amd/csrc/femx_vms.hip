@@ -330,21 +330,25 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
 }
 
 constexpr int BC_BIT = 0x8000;
+constexpr int EC_ID = 0xff, EC_RANK_SHIFT = 8, EC_RANK = 0x7f;
+constexpr int LN_BEGIN = 0x3ff, LN_RANK_SHIFT = 10, LN_RANK = 0xff, LN_SHARED = 1 << 18, LN_BC = 1 << 19;
 
 template <typename T>
 struct BlockArgs {
   int nblk;
   const int *elem_off, *node_off;
-  // per local node {global node, slot, run, 0}: slot >= 0 is the partial slot
-  // of a block-boundary node; a complete node holds -1 without a BC record
-  // and -2 - record with one.  run packs the node's contributions in the
-  // block's node-sorted order as begin | end << 16.
-  const int4 *lnode;
-  // per element: node-sorted position of each corner's contribution (bits
-  // 0..14) and BC_BIT when the corner's node has a BC record, so its pre-BC
-  // and post-BC states may differ
-  const ushort4 *lpos;
-  const uchar4 *lconn;
+  const int *slot0;  // per block: its first partial slot
+  // per local node {global node, meta}: meta holds the first position of the
+  // node's contributions in the block's node-sorted order (LN_BEGIN), the
+  // rank of its partial slot among the block's slots (LN_RANK), LN_SHARED
+  // for a block-boundary node and LN_BC for a node with a BC record
+  const int2 *lnode;
+  // per element corner: local node (EC_ID), rank of the corner among the
+  // node's contributions in ascending element order (EC_RANK), and BC_BIT
+  // when the corner's node has a BC record, so its pre-BC and post-BC
+  // states may differ
+  const ushort4 *ecorn;
+  const int *node_rec;  // per node: BC record or -1
   const double4 *xyzm;  // x y z, lumped nodal mass
   const double4 *Q, *P; // post-BC / pre-BC state of the previous iteration
   double4 *Qn, *Pn;     // outputs of this iteration
@@ -452,6 +456,8 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     float4 sres[4 * VB];
   };
   __shared__ __align__(16) unsigned char lraw[F32 ? sizeof(F32Lds) : 16];
+  // first position of each local node's contributions, and 4 ne after the last
+  __shared__ unsigned short sbeg[RES ? VN + 1 : 1];
   F32Lds &L = *reinterpret_cast<F32Lds *>(lraw);
   float4 *sfa = L.st.sfa, *sfb = L.st.sfb;
   float2 *sfc = L.st.sfc;
@@ -476,17 +482,16 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   }
   const int n0 = uni(a.node_off[blk]), nn = uni(a.node_off[blk + 1]) - n0;
   const int e0 = uni(a.elem_off[blk]), ne = uni(a.elem_off[blk + 1]) - e0;
+  const int slot_base = RES ? uni(a.slot0[blk]) : 0;
   // 0. issue this thread's element loads before staging (they overlap)
   const bool has_e = tid < ne;
   const unsigned e = (unsigned)(e0 + tid);
   const size_t E = (size_t)a.E;
   const unsigned eoff = e * (unsigned)sizeof(T);  // byte offset inside a row (rows are < 4 GiB)
-  uchar4 c = make_uchar4(0, 0, 0, 0);
-  ushort4 pos = make_ushort4(0, 0, 0, 0);
+  ushort4 ec = make_ushort4(0, 0, 0, 0);
   T sdu[4][3];
   if (has_e) {
-    c = a.lconn[e];
-    if (RES) pos = a.lpos[e];
+    ec = a.ecorn[e];
 #pragma unroll
     for (int g = 0; g < 4; g++)
 #pragma unroll
@@ -495,13 +500,14 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   // the local node this thread owns (nn <= VN <= VB): one record holds all
   // its block data, so no load of the node phase waits on another
   const bool has_n = tid < nn;
-  int g_own = 0, slot_own = -1, run_own = 0;
+  int g_own = 0, meta_own = 0;
   if (has_n) {
-    const int4 ln = a.lnode[n0 + tid];
+    const int2 ln = a.lnode[n0 + tid];
     g_own = ln.x;
-    slot_own = ln.y;
-    run_own = ln.z;
+    meta_own = ln.y;
+    if (RES) sbeg[tid] = (unsigned short)(meta_own & LN_BEGIN);
   }
+  if (RES && tid == 0) sbeg[nn] = (unsigned short)(4 * ne);
   // reference node of the FP32 relative values: the block's first local node
   double4 xr = make_double4(0, 0, 0, 0), pr = make_double4(0, 0, 0, 0), qr = make_double4(0, 0, 0, 0);
   if (F32) {
@@ -558,7 +564,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   // 2. one element per thread
   T res[4][4];
   if (has_e) {
-    const int id[4] = {c.x, c.y, c.z, c.w};
+    const int id[4] = {ec.x & EC_ID, ec.y & EC_ID, ec.z & EC_ID, ec.w & EC_ID};
     T DN[3][4], V, ih;
     ElemVel<T> ev;
     T p[4], dd[4][3];
@@ -593,7 +599,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
           uq[n][2] = rc.x;
           p[n] = rc.y;
         }
-        const bool reuse = PROJ && !ACC && !((pos.x | pos.y | pos.z | pos.w) & BC_BIT);
+        const bool reuse = PROJ && !ACC && !((ec.x | ec.y | ec.z | ec.w) & BC_BIT);
         if (!reuse) {
           const float qref[3] = {(float)qr.x, (float)qr.y, (float)qr.z};
           elem_vel_rel(DN, qref, uq, ev);
@@ -638,7 +644,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
         }
         // without BC nodes the post-BC velocities equal the pre-BC ones the
         // projection just used: reuse their gradient
-        const bool reuse = PROJ && !ACC && !((pos.x | pos.y | pos.z | pos.w) & BC_BIT);
+        const bool reuse = PROJ && !ACC && !((ec.x | ec.y | ec.z | ec.w) & BC_BIT);
         if (!reuse) elem_vel<T>(DN, uq, ev);
         if (ACC) {
 #pragma unroll
@@ -674,10 +680,16 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     }
   }
   if (!RES) return;
+  // node-sorted position of each corner's contribution
+  int ps[4] = {0, 0, 0, 0};
+  if (has_e) {
+    const unsigned short ecs[4] = {ec.x, ec.y, ec.z, ec.w};
+#pragma unroll
+    for (int b = 0; b < 4; b++) ps[b] = sbeg[ecs[b] & EC_ID] + ((ecs[b] >> EC_RANK_SHIFT) & EC_RANK);
+  }
   // with FP32 element math the contributions overwrite the phase-2 inputs
   if (F32) __syncthreads();
   if (has_e) {
-    const int ps[4] = {pos.x & ~BC_BIT, pos.y & ~BC_BIT, pos.z & ~BC_BIT, pos.w & ~BC_BIT};
 #pragma unroll
     for (int b = 0; b < 4; b++) {
       T4 v;
@@ -694,10 +706,12 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   // barrier so that their latency overlaps its wait; FP64 keeps them
   double4 base = base_keep;
   double im = im_keep;
-  if (F32 && has_n && slot_own < 0) {
+  const bool complete = !(meta_own & LN_SHARED);
+  if (F32 && has_n && complete) {
     base = !ACC ? a.Q[g_own] : a.P[g_own];
     im = a.xyzm[g_own].w;
   }
+  const int rec_own = has_n && complete && (meta_own & LN_BC) ? a.node_rec[g_own] : -1;
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
   double m2 = 0.0;
@@ -705,9 +719,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     const int g = g_own;
     // the node's contributions summed in S (FP64 unless FEMX_MIXED_F32SUM)
     S Rt[4] = {(S)0, (S)0, (S)0, (S)0};
-    const int s1 = run_own >> 16;
+    const int s1 = sbeg[tid + 1];
 #pragma unroll 4
-    for (int s = run_own & 0xffff; s < s1; s++) {
+    for (int s = sbeg[tid]; s < s1; s++) {
       const T4 v = sres[s];
       Rt[0] += (S)v.x;
       Rt[1] += (S)v.y;
@@ -715,9 +729,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       Rt[3] += (S)v.w;
     }
     double R[4] = {(double)Rt[0], (double)Rt[1], (double)Rt[2], (double)Rt[3]};
-    if (slot_own < 0) {
+    if (complete) {
       double4 pre, post;
-      m2 = node_update(base, im, R, k, pre, post, -2 - slot_own, a.rec, a.row_off, a.rows, a.err);
+      m2 = node_update(base, im, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
       a.Pn[g] = pre;
       a.Qn[g] = post;
     } else {
@@ -726,7 +740,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       v.y = Rt[1];
       v.z = Rt[2];
       v.w = Rt[3];
-      static_cast<typename Vec4<S>::type *>(a.partials)[slot_own] = v;
+      static_cast<typename Vec4<S>::type *>(a.partials)[slot_base + ((meta_own >> LN_RANK_SHIFT) & LN_RANK)] = v;
     }
   }
   block_max_store<VB>(m2, a.bmax);
@@ -977,8 +991,10 @@ struct femx_vms {
   int n_bmax = 0;
   DevBuf<double> scratch;  // staging of reference-layout arrays
   DevBuf<int> rec_node;
-  DevBuf<uint16_t> lpos_bc;  // the mesh's corner positions with BC_BIT set per the BC records
-  DevBuf<int4> lnode;        // per local node: global node, slot or BC record, contribution run
+  // block records of the block kernel (the BC flags depend on the handle)
+  DevBuf<ushort4> ecorn;  // per element corner: local node, rank in the node's run, BC flag
+  DevBuf<int2> lnode;     // per local node: global node, run begin, slot rank, flags
+  DevBuf<int> slot0;      // per block: first partial slot
   int cur = 0;
   bool proj_pending = false;
   long iter = 0;
@@ -1024,16 +1040,15 @@ VmsConst vms_const(const femx_vms *s, double t) {
 }
 
 // Algorithmic bytes of one block launch (compulsory traffic, ideal node
-// reuse): u8 local connectivity + u16 sorted positions (12 B per element), the
-// subgrid velocity read and written, coordinates + mass, the state read
-// (post-BC; also pre-BC for the projection) and the state written (both
-// arrays) once per node.
+// reuse): the 8-byte corner record per element, the subgrid velocity read
+// and written, coordinates + mass, the state read (post-BC; also pre-BC for
+// the projection) and the state written (both arrays) once per node.
 double block_bytes(const femx_vms *s, bool proj, bool res) {
   const double E = s->mesh->E, N = s->mesh->N;
   const double tb = f32_elem(s->p.precision) ? 4.0 : 8.0;
-  double b = E * 4.0 + E * 12 * tb + N * 32.0;
+  double b = E * 8.0 + E * 12 * tb + N * 32.0;
   if (proj) b += E * 12 * tb + N * 32.0;
-  if (res) b += E * 8.0 + N * (32.0 + 64.0);
+  if (res) b += N * (32.0 + 64.0);
   return b;
 }
 
@@ -1045,9 +1060,10 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.nblk = b.nb;
   a.elem_off = b.d_elem_off.p;
   a.node_off = b.d_node_off.p;
+  a.slot0 = s->slot0.p;
   a.lnode = s->lnode.p;
-  a.lpos = reinterpret_cast<const ushort4 *>(s->lpos_bc.p);
-  a.lconn = reinterpret_cast<const uchar4 *>(b.d_lconn.p);
+  a.ecorn = s->ecorn.p;
+  a.node_rec = s->bc.node_rec.p;
   a.xyzm = s->xyzm.p;
   a.Q = s->Q[s->cur].p;
   a.P = s->P[s->cur].p;
@@ -1186,36 +1202,48 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   if ((r = s->bc.build(bc, m.N, true, s->st)) != FEMX_OK) return fail(r);
   if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
   {
-    // the positions need 10 bits (< 4 x 256 corners per block): bit 15 carries
-    // whether the corner's node has a BC record
-    std::vector<uint16_t> lp(m.blk.lcsr);
+    // block records of the block kernel.  A corner's contribution goes to
+    // the node-sorted position begin(node) + rank: the rank (its order among
+    // the node's corners in the block, ascending elements) must fit 7 bits,
+    // the local node 8 bits (<= 256 per block), begin 10 bits (< 4 x 256
+    // corners) and the slot rank 8 bits (<= 256 shared nodes per block).
+    const Blocks &b = m.blk;
     std::vector<char> has_rec(m.N, 0);
     for (int nd : s->bc.h_rec_node) has_rec[nd] = 1;
-    for (int blk = 0; blk < m.blk.nb; blk++)
-      for (int e = m.blk.elem_off[blk]; e < m.blk.elem_off[blk + 1]; e++)
-        for (int a = 0; a < 4; a++) {
-          uint16_t &v = lp[4L * e + a];
-          if (v & BC_BIT) return fail((set_error("block positions overflow 15 bits"), FEMX_ERR_ARG));
-          const int g = m.blk.loc_node[m.blk.node_off[blk] + m.blk.lconn[4L * e + a]];
-          if (has_rec[g]) v |= BC_BIT;
-        }
-    if ((r = s->lpos_bc.upload(lp.data(), lp.size(), s->st)) != FEMX_OK) return fail(r);
-    // per local node record of the block kernel: a block's contributions
-    // number < 4 x 256, so the run bounds fit 16 bits each
-    const Blocks &b = m.blk;
-    std::vector<int> node_rec(m.N, -1);
-    for (size_t i = 0; i < s->bc.h_rec_node.size(); i++) node_rec[s->bc.h_rec_node[i]] = (int)i;
-    std::vector<int4> ln(std::max(b.total_local, 1), make_int4(0, -1, 0, 0));
+    std::vector<ushort4> ecr(std::max<long>(m.E, 1), make_ushort4(0, 0, 0, 0));
+    std::vector<int2> ln(std::max(b.total_local, 1), make_int2(0, 0));
+    std::vector<int> slot0(b.nb + 1, 0);
+    int slot = 0;
     for (int blk = 0; blk < b.nb; blk++) {
-      const int cbase = 4 * b.elem_off[blk];
-      for (int l = b.node_off[blk]; l < b.node_off[blk + 1]; l++) {
-        const int g = b.loc_node[l], ps = b.loc_pslot[l];
-        const int c0 = b.lcsr_off[l] - cbase, c1 = b.lcsr_off[l + 1] - cbase;
-        if (c0 < 0 || c1 > 0xffff || c0 > c1) return fail((set_error("block contribution runs overflow 16 bits"), FEMX_ERR_ARG));
-        ln[l] = make_int4(g, ps >= 0 ? ps : -2 - node_rec[g], c0 | (c1 << 16), 0);
+      const int n0 = b.node_off[blk], cbase = 4 * b.elem_off[blk];
+      slot0[blk] = slot;
+      for (int l = n0; l < b.node_off[blk + 1]; l++) {
+        const int g = b.loc_node[l], ps = b.loc_pslot[l], c0 = b.lcsr_off[l] - cbase;
+        if (c0 < 0 || c0 > LN_BEGIN) return fail((set_error("block contribution runs overflow 10 bits"), FEMX_ERR_ARG));
+        int meta = c0 | (has_rec[g] ? LN_BC : 0);
+        if (ps >= 0) {
+          if (ps - slot0[blk] > LN_RANK || ps != slot) return fail((set_error("block partial slots out of order"), FEMX_ERR_ARG));
+          meta |= LN_SHARED | ((ps - slot0[blk]) << LN_RANK_SHIFT);
+          slot++;
+        }
+        ln[l] = make_int2(g, meta);
+      }
+      for (int e = b.elem_off[blk]; e < b.elem_off[blk + 1]; e++) {
+        unsigned short c[4];
+        for (int a = 0; a < 4; a++) {
+          const int j = b.lconn[4L * e + a], l = n0 + j, g = b.loc_node[l];
+          const int rank = (int)b.lcsr[4L * e + a] - (b.lcsr_off[l] - cbase);
+          if (j > EC_ID || rank < 0 || rank > EC_RANK)
+            return fail((set_error("a node has more than 128 elements in one block"), FEMX_ERR_ARG));
+          c[a] = (unsigned short)(j | (rank << EC_RANK_SHIFT) | (has_rec[g] ? BC_BIT : 0));
+        }
+        ecr[e] = make_ushort4(c[0], c[1], c[2], c[3]);
       }
     }
+    slot0[b.nb] = slot;
+    if ((r = s->ecorn.upload(ecr.data(), ecr.size(), s->st)) != FEMX_OK) return fail(r);
     if ((r = s->lnode.upload(ln.data(), ln.size(), s->st)) != FEMX_OK) return fail(r);
+    if ((r = s->slot0.upload(slot0.data(), slot0.size(), s->st)) != FEMX_OK) return fail(r);
   }
   if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->mass.alloc(m.N)) != FEMX_OK) return fail(r);
   for (int k = 0; k < 2; k++)

@@ -586,7 +586,9 @@ def workload_accuracy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default per workload: long enough that the start-up of a timed "
+                         "window, about 1 ms, stays below 1%% of it)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=96, help="cube edge count (96 -> 5.31M tets)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
@@ -609,6 +611,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="serve: hardware queues of the process (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"c3": 500, "c2": 300, "c4": 200, "accuracy": 1000}.get(args.workload, 50)
     if args.workload == "serve" and args.hw_queues:
         # before the first HIP call: resident requests each run on their own stream
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
@@ -623,8 +627,6 @@ def main():
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
     if args.workload == "c4":
-        if args.steps == 50:
-            args.steps = 200
         workload_c4(args)
         return
     if args.workload == "serve":
@@ -637,8 +639,6 @@ def main():
         workload_c2(args)
         return
     if args.workload == "accuracy":
-        if args.steps == 50:
-            args.steps = 1000
         workload_accuracy(args)
         return
     if args.workload == "hbm":

@@ -333,6 +333,16 @@ constexpr int BC_BIT = 0x8000;
 constexpr int EC_ID = 0xff, EC_RANK_SHIFT = 8, EC_RANK = 0x7f;
 constexpr int LN_BEGIN = 0x3ff, LN_RANK_SHIFT = 10, LN_RANK = 0xff, LN_SHARED = 1 << 18, LN_BC = 1 << 19;
 
+// The subgrid velocity of element e (block order), component r = g * 3 + i,
+// lives in tiles of 256 elements: [E / 256][12][256].  A block's 256
+// elements read and write one contiguous 12 KB run instead of 12 rows
+// far apart.
+constexpr long SD_TILE = 256;
+__host__ __device__ __forceinline__ long sdu_at(long e, int r) {
+  return (e & ~(SD_TILE - 1)) * 12 + (long)r * SD_TILE + (e & (SD_TILE - 1));
+}
+inline long sdus_len(long E) { return 12 * ((E + SD_TILE - 1) / SD_TILE) * SD_TILE; }
+
 template <typename T>
 struct BlockArgs {
   int nblk;
@@ -354,7 +364,7 @@ struct BlockArgs {
   double4 *Qn, *Pn;     // outputs of this iteration
   // partial residual sums of block-boundary nodes: one record of S per slot
   void *partials;
-  T *sdus;              // [12][E] (block order), index (g*3+i)*E + e
+  T *sdus;              // block order, tiled: sdu_at(e, g * 3 + i)
   int E;
   int first_dual;       // base of the update: Q at the first dual iteration, P later
   const BCRecord *rec;
@@ -407,13 +417,6 @@ __device__ __forceinline__ double node_update(const double4 base, double im, con
   if (ri >= 0) apply_bc(rec[ri], row_off, rows, k.t, u, v, w, &p, err);
   post = make_double4(u, v, w, p);
   return m2;
-}
-
-// Row `row_elems` of an SoA array, addressed as a wave-uniform base plus a
-// zero-extended 32-bit byte offset (SGPR base + VGPR offset addressing).
-template <typename T>
-__device__ __forceinline__ T *row_ptr(T *base, size_t row_elems, unsigned byte_off) {
-  return reinterpret_cast<T *>(reinterpret_cast<char *>(base + row_elems) + (size_t)byte_off);
 }
 
 // A value every lane of the wave holds, moved to SGPRs bit for bit
@@ -486,8 +489,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   // 0. issue this thread's element loads before staging (they overlap)
   const bool has_e = tid < ne;
   const unsigned e = (unsigned)(e0 + tid);
-  const size_t E = (size_t)a.E;
-  const unsigned eoff = e * (unsigned)sizeof(T);  // byte offset inside a row (rows are < 4 GiB)
+  T *const sdt = a.sdus + sdu_at(e, 0);  // this element's tile column
   ushort4 ec = make_ushort4(0, 0, 0, 0);
   T sdu[4][3];
   if (has_e) {
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 #pragma unroll
     for (int g = 0; g < 4; g++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) sdu[g][i] = *row_ptr(a.sdus, (g * 3 + i) * E, eoff);
+      for (int i = 0; i < 3; i++) sdu[g][i] = sdt[(g * 3 + i) * SD_TILE];
   }
   // the local node this thread owns (nn <= VN <= VB): one record holds all
   // its block data, so no load of the node phase waits on another
@@ -658,7 +660,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
 #pragma unroll
       for (int g = 0; g < 4; g++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) *row_ptr(a.sdus, (g * 3 + i) * E, eoff) = sd(g, i);
+        for (int i = 0; i < 3; i++) sdt[(g * 3 + i) * SD_TILE] = sd(g, i);
     }
     if (RES) {
       vms_residual<T, ACC>(DN, V, ev, p, dd, sd, res, k);
@@ -953,20 +955,20 @@ __global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nma
   }
 }
 
-// reference layout sdus[(g*3+i)*E + original e]  <->  block order
+// reference layout sdus[(g*3+i)*E + original e]  <->  block order, tiled
 template <typename T>
 __global__ void sdus_to_blocks(long E, const int *perm, const double *src, T *dst) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 12 * E) return;
   const long row = i / E, e = i - row * E;
-  dst[i] = (T)src[row * E + perm[e]];
+  dst[sdu_at(e, (int)row)] = (T)src[row * E + perm[e]];
 }
 template <typename T>
 __global__ void sdus_from_blocks(long E, const int *perm, const T *src, double *dst) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 12 * E) return;
   const long row = i / E, e = i - row * E;
-  dst[row * E + perm[e]] = (double)src[i];
+  dst[row * E + perm[e]] = (double)src[sdu_at(e, (int)row)];
 }
 
 }  // namespace
@@ -1251,7 +1253,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   const int nparts = m.blk.n_sh ? m.blk.sh_off[m.blk.n_sh] : 0;
   if ((r = s->partials.alloc(std::max(nparts, 1))) != FEMX_OK) return fail(r);
   const bool mixed = f32_elem(p->precision);
-  if ((r = mixed ? s->sdus32.alloc(12L * m.E) : s->sdus64.alloc(12L * m.E)) != FEMX_OK) return fail(r);
+  if ((r = mixed ? s->sdus32.alloc(sdus_len(m.E)) : s->sdus64.alloc(sdus_len(m.E))) != FEMX_OK) return fail(r);
   s->n_bmax = m.blk.nb + grid_for(std::max(m.blk.n_sh, 1), 256);
   if ((r = s->bmax.alloc(s->n_bmax)) != FEMX_OK) return fail(r);
   if (hipMemsetAsync(s->bmax.p, 0, s->n_bmax * sizeof(double), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
@@ -1327,9 +1329,9 @@ extern "C" femx_status femx_vms_set_state(femx_vms *s, const double *sol, const 
     FEMX_HIP(hipGetLastError());
   } else {
     if (mixed)
-      FEMX_HIP(hipMemsetAsync(s->sdus32.p, 0, ns * sizeof(float), s->st));
+      FEMX_HIP(hipMemsetAsync(s->sdus32.p, 0, sdus_len(m.E) * sizeof(float), s->st));
     else
-      FEMX_HIP(hipMemsetAsync(s->sdus64.p, 0, ns * sizeof(double), s->st));
+      FEMX_HIP(hipMemsetAsync(s->sdus64.p, 0, sdus_len(m.E) * sizeof(double), s->st));
   }
   return check_err(s);
 }

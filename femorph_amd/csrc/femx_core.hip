@@ -140,8 +140,11 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   for (long i = 0; i < E; i++) b.perm[i] = key[i].e;
   key.clear();
   key.shrink_to_fit();
-  // greedy blocks: <= B elements, <= 256 local nodes
-  std::vector<int> stamp(m.N, -1), local_id(m.N, 0);
+  // greedy blocks: <= B elements, <= 256 local nodes, and at most
+  // MAX_NODE_CORNERS corners of one node (so a corner's rank among its
+  // node's corners in the block fits 7 bits)
+  constexpr int MAX_NODE_CORNERS = 128;
+  std::vector<int> stamp(m.N, -1), local_id(m.N, 0), corners(m.N, 0);
   b.elem_off.assign(1, 0);
   b.node_off.assign(1, 0);
   b.loc_node.clear();
@@ -166,16 +169,24 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   };
   for (long i = 0; i < E; i++) {
     const int eo = b.perm[i];
-    int fresh = 0;
-    for (int a = 0; a < npe; a++) fresh += stamp[m.h_conn[(long)eo * npe + a]] != blk;
+    int fresh = 0, most = 0;
+    for (int a = 0; a < npe; a++) {
+      const int n = m.h_conn[(long)eo * npe + a];
+      fresh += stamp[n] != blk;
+      if (stamp[n] == blk) most = std::max(most, corners[n]);
+    }
     const bool new_group = elem_group && (*elem_group)[eo] != (*elem_group)[b.perm[e_begin]];
-    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256 || new_group)) close_block(i);
+    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256 || new_group ||
+                        most + npe > MAX_NODE_CORNERS))
+      close_block(i);
     for (int a = 0; a < npe; a++) {
       const int n = m.h_conn[(long)eo * npe + a];
       if (stamp[n] != blk) {
         stamp[n] = blk;
+        corners[n] = 0;
         cur_nodes.push_back(n);
       }
+      corners[n]++;
     }
   }
   close_block(E);

@@ -137,3 +137,33 @@ def _tet_det(mesh: Mesh) -> np.ndarray:
 def min_tet_h(mesh: Mesh) -> float:
     """min over elements of (6 V)^(1/3), the reference's VMS length scale."""
     return float(np.min(np.cbrt(_tet_det(mesh))))
+
+
+def tet_fan(level: int = 3) -> Mesh:
+    """Tets (centre, face) over an octahedron subdivided `level` times and
+    projected to the unit sphere: node 0 is a corner of every one of the
+    8 * 4**level tets, far above what one element block may hold of one node."""
+    verts = [np.array(v, float) for v in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    faces = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                v = verts[i] + verts[j]
+                verts.append(v / np.linalg.norm(v))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        new = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new
+    coords = np.vstack([np.zeros(3)] + verts)
+    conn = np.array([(0, a + 1, b + 1, c + 1) for a, b, c in faces], np.int32)
+    m = Mesh(coords, conn)
+    neg = _tet_det(m) < 0
+    conn[neg, 2], conn[neg, 3] = conn[neg, 3].copy(), conn[neg, 2].copy()
+    return Mesh(coords, conn)

@@ -81,3 +81,13 @@ def test_bc_node_out_of_range_rejected():
     m = meshgen.tet_cube(2)
     bc = BoundaryConditions(dir_node=[10 ** 6], dir_val=[[0, 0, 0]])
     assert bc.dir_node[0] == 10 ** 6 and m.n_nodes < 10 ** 6
+
+
+def test_block_cap_on_corners_of_one_node():
+    """Node 0 is a corner of all 512 tets: every block holds at most 128 of
+    its corners, so a corner's rank in its node's run fits 7 bits."""
+    m = meshgen.tet_fan(3)
+    assert m.n_elems == 512 and np.bincount(m.conn.ravel())[0] == 512
+    s = femx.block_stats(m, 256)
+    assert s["elements"] == 512 and s["max_elems"] <= 128 and s["blocks"] >= 4
+    assert s["complete"] + s["partials"] == s["local_nodes"]

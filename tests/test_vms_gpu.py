@@ -193,3 +193,26 @@ def test_element_diagnostics(gpu, tot_dual):
     g.step()
     sol, _, _ = g.get_state()
     assert rel_l2(sol, o.get("sol_n")) <= TOL64
+
+
+@pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5)])
+def test_high_valence_node(gpu, precision, tol):
+    """A node with 512 incident tets: the block builder splits its corners
+    over blocks of at most 128 corners of one node; per-step parity with the
+    oracle, no BCs."""
+    m = meshgen.tet_fan(3)
+    assert m.n_elems == 512 and np.bincount(m.conn.ravel())[0] == 512
+    assert (meshgen._tet_det(m) > 0).all()
+    s = gpu.block_stats(m, 256)
+    assert s["blocks"] >= 4 and s["shared_nodes"] >= 1
+    bc = meshgen.BoundaryConditions()
+    dt = 0.1 * meshgen.min_tet_h(m)
+    params = dict(rho=1.0, mu=0.01, inv_eps=1.0, c1=4.0, c2=2.0, dt=dt)
+    ic = meshgen.random_velocity(m, 1e-3, 5)
+    ic[:, 3] = np.random.default_rng(5).uniform(0.2, 0.8, m.n_nodes)
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=precision)
+    for step in range(8):
+        o.step()
+        g.step()
+        sol, _, _ = g.get_state()
+        assert rel_l2(sol, o.get("sol_n")) <= tol, (step, rel_l2(sol, o.get("sol_n")))

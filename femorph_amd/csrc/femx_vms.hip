@@ -360,8 +360,11 @@ struct BlockArgs {
   const ushort4 *ecorn;
   const int *node_rec;  // per node: BC record or -1
   const double4 *xyzm;  // x y z, lumped nodal mass
-  const double4 *Q, *P; // post-BC / pre-BC state of the previous iteration
-  double4 *Qn, *Pn;     // outputs of this iteration
+  // post-BC state of the previous iteration, and this iteration's.  The
+  // pre-BC state equals the post-BC one except at nodes with a BC record,
+  // so it is kept per BC record (Pr, Prn).
+  const double4 *Q, *Pr;
+  double4 *Qn, *Prn;
   // partial residual sums of block-boundary nodes: one record of S per slot
   void *partials;
   T *sdus;              // block order, tiled: sdu_at(e, g * 3 + i)
@@ -515,8 +518,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   if (F32) {
     const int gr = uni(a.lnode[n0].x);
     xr = uni(a.xyzm[gr]);
-    if (PROJ || ACC) pr = uni(a.P[gr]);
-    if (RES) qr = uni(a.Q[gr]);
+    // one reference for both states (any reference gives the same sums)
+    qr = uni(a.Q[gr]);
+    pr = qr;
   }
   // 1. stage the block's nodes
   double4 base_keep = make_double4(0, 0, 0, 0);
@@ -525,8 +529,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     const int j = tid, g = g_own;
     const double4 x = a.xyzm[g];
     double4 p_own = make_double4(0, 0, 0, 0), q_own = make_double4(0, 0, 0, 0);
-    if (PROJ || ACC) p_own = a.P[g];
-    if (RES) q_own = a.Q[g];
+    q_own = a.Q[g];
+    p_own = q_own;
+    if ((PROJ || ACC) && (meta_own & LN_BC)) p_own = a.Pr[a.node_rec[g]];
     if (!F32) {
       base_keep = !ACC ? q_own : p_own;
       im_keep = x.w;
@@ -560,7 +565,10 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   if (PROJ) {
     // p[i] = sol_n[i][3]: pressures of GLOBAL nodes 0..3 (:489)
 #pragma unroll
-    for (int n = 0; n < 4; n++) q4[n] = uni(a.P[node0 + n].w);
+    for (int n = 0; n < 4; n++) {
+      const int rn = uni(a.node_rec[node0 + n]);
+      q4[n] = uni(rn >= 0 ? a.Pr[rn].w : a.Q[node0 + n].w);
+    }
   }
   __syncthreads();
   // 2. one element per thread
@@ -709,11 +717,11 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   double4 base = base_keep;
   double im = im_keep;
   const bool complete = !(meta_own & LN_SHARED);
+  const int rec_own = has_n && complete && (meta_own & LN_BC) ? a.node_rec[g_own] : -1;
   if (F32 && has_n && complete) {
-    base = !ACC ? a.Q[g_own] : a.P[g_own];
+    base = ACC && rec_own >= 0 ? a.Pr[rec_own] : a.Q[g_own];
     im = a.xyzm[g_own].w;
   }
-  const int rec_own = has_n && complete && (meta_own & LN_BC) ? a.node_rec[g_own] : -1;
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
   double m2 = 0.0;
@@ -734,7 +742,7 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     if (complete) {
       double4 pre, post;
       m2 = node_update(base, im, R, k, pre, post, rec_own, a.rec, a.row_off, a.rows, a.err);
-      a.Pn[g] = pre;
+      if (rec_own >= 0) a.Prn[rec_own] = pre;
       a.Qn[g] = post;
     } else {
       typename Vec4<S>::type v;
@@ -755,8 +763,8 @@ struct FixArgs {
   const double *sh_im;  // 1 / nd_mass of every shared node, contiguous
   const int *sh_rec;    // its BC record or -1, contiguous
   const void *partials;  // T4 records of the block kernel
-  const double4 *Q, *P;
-  double4 *Qn, *Pn;
+  const double4 *Q, *Pr;  // post-BC state; pre-BC state per BC record
+  double4 *Qn, *Prn;
   int first_dual;
   const BCRecord *rec;
   const int *row_off;
@@ -784,8 +792,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     const int4 f4 = a.sh_first4[s];
     const double im = a.sh_im[s];
     const int rec = a.sh_rec[s];
-    const double4 *B = a.first_dual ? a.Q : a.P;
-    const double4 base = B[g];
+    const double4 base = !a.first_dual && rec >= 0 ? a.Pr[rec] : a.Q[g];
     const int fs[FIX_BATCH] = {f4.x, f4.y, f4.z, f4.w};
     T4 p[FIX_BATCH];
 #pragma unroll
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
     m2 = node_update(base, im, R, k, pre, post, rec, a.rec, a.row_off, a.rows, a.err);
-    a.Pn[g] = pre;
+    if (rec >= 0) a.Prn[rec] = pre;
     a.Qn[g] = post;
   }
   block_max_store<256>(m2, a.bmax);
@@ -850,6 +857,12 @@ __global__ __launch_bounds__(256) void vms_shared_meta(int n_sh, const int *sh_n
   const int g = sh_node[s];
   sh_im[s] = xyzm[g].w;
   sh_rec[s] = node_rec[g];
+}
+
+// Pre-BC state per BC record from the state of its node.
+__global__ void rec_gather(int nrec, const int *rec_node, const double4 *Q, double4 *Pr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nrec) Pr[i] = Q[rec_node[i]];
 }
 
 // BCs of the initial state (solve() :883-889).
@@ -983,7 +996,8 @@ struct femx_vms {
   BCDev bc;
   DevBuf<double4> xyzm;   // x y z and 1 / nd_mass
   DevBuf<double> mass;    // nd_mass
-  DevBuf<double4> P[2], Q[2];
+  DevBuf<double4> Q[2];   // post-BC state
+  DevBuf<double4> Pr[2];  // pre-BC state of the nodes with a BC record, per record
   DevBuf<double4> partials;
   DevBuf<double> sh_im;  // per shared node: 1 / nd_mass
   DevBuf<int> sh_rec;    // per shared node: BC record or -1
@@ -1043,14 +1057,15 @@ VmsConst vms_const(const femx_vms *s, double t) {
 
 // Algorithmic bytes of one block launch (compulsory traffic, ideal node
 // reuse): the 8-byte corner record per element, the subgrid velocity read
-// and written, coordinates + mass, the state read (post-BC; also pre-BC for
-// the projection) and the state written (both arrays) once per node.
+// (and written by the projection), coordinates + mass and the post-BC state
+// read once per node, the post-BC state written once per node, and the
+// pre-BC state of the BC records read and written.
 double block_bytes(const femx_vms *s, bool proj, bool res) {
-  const double E = s->mesh->E, N = s->mesh->N;
+  const double E = s->mesh->E, N = s->mesh->N, R = s->bc.n_rec;
   const double tb = f32_elem(s->p.precision) ? 4.0 : 8.0;
-  double b = E * 8.0 + E * 12 * tb + N * 32.0;
-  if (proj) b += E * 12 * tb + N * 32.0;
-  if (res) b += N * (32.0 + 64.0);
+  double b = E * 8.0 + E * 12 * tb + N * (32.0 + 32.0) + R * 32.0;
+  if (proj) b += E * 12 * tb;
+  if (res) b += N * 32.0 + R * 32.0;
   return b;
 }
 
@@ -1068,9 +1083,9 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.node_rec = s->bc.node_rec.p;
   a.xyzm = s->xyzm.p;
   a.Q = s->Q[s->cur].p;
-  a.P = s->P[s->cur].p;
+  a.Pr = s->Pr[s->cur].p;
   a.Qn = s->Q[s->cur ^ 1].p;
-  a.Pn = s->P[s->cur ^ 1].p;
+  a.Prn = s->Pr[s->cur ^ 1].p;
   a.partials = s->partials.p;
   a.sdus = sdus;
   a.E = m.E;
@@ -1121,7 +1136,9 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   if (b.n_sh == 0) return FEMX_OK;
   const bool f32p = f32_sum(s->p.precision);
   const double rec_bytes = f32p ? 16.0 : 32.0;  // one partial slot record
-  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 16.0 + 32.0 + 32.0 + 64.0);
+  // partials written and read; per shared node its index, slot range, first
+  // slots, 1/m and BC record, the base state read and the post-BC state written
+  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 16.0 + 8.0 + 4.0 + 32.0 + 32.0);
   TimedLaunch tl(s->timer, s->timer.kind("vms_fixup", bytes), s->st);
   FixArgs a;
   a.n_sh = b.n_sh;
@@ -1133,9 +1150,9 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.sh_rec = s->sh_rec.p;
   a.partials = s->partials.p;
   a.Q = s->Q[s->cur].p;
-  a.P = s->P[s->cur].p;
+  a.Pr = s->Pr[s->cur].p;
   a.Qn = s->Q[s->cur ^ 1].p;
-  a.Pn = s->P[s->cur ^ 1].p;
+  a.Prn = s->Pr[s->cur ^ 1].p;
   a.first_dual = first_dual ? 1 : 0;
   a.rec = s->bc.rec.p;
   a.row_off = s->bc.row_off.p;
@@ -1249,7 +1266,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   }
   if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->mass.alloc(m.N)) != FEMX_OK) return fail(r);
   for (int k = 0; k < 2; k++)
-    if ((r = s->P[k].alloc(m.N)) != FEMX_OK || (r = s->Q[k].alloc(m.N)) != FEMX_OK) return fail(r);
+    if ((r = s->Pr[k].alloc(std::max(s->bc.n_rec, 1))) != FEMX_OK || (r = s->Q[k].alloc(m.N)) != FEMX_OK) return fail(r);
   const int nparts = m.blk.n_sh ? m.blk.sh_off[m.blk.n_sh] : 0;
   if ((r = s->partials.alloc(std::max(nparts, 1))) != FEMX_OK) return fail(r);
   const bool mixed = f32_elem(p->precision);
@@ -1258,7 +1275,8 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
   if ((r = s->bmax.alloc(s->n_bmax)) != FEMX_OK) return fail(r);
   if (hipMemsetAsync(s->bmax.p, 0, s->n_bmax * sizeof(double), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   for (int k = 0; k < 2; k++) {
-    if (hipMemsetAsync(s->P[k].p, 0, m.N * sizeof(double4), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
+    if (hipMemsetAsync(s->Pr[k].p, 0, std::max(s->bc.n_rec, 1) * sizeof(double4), s->st) != hipSuccess)
+      return fail(FEMX_ERR_HIP);
     if (hipMemsetAsync(s->Q[k].p, 0, m.N * sizeof(double4), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   }
   vms_mass<<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, reinterpret_cast<const int4 *>(m.conn.p),
@@ -1315,7 +1333,9 @@ extern "C" femx_status femx_vms_set_state(femx_vms *s, const double *sol, const 
                                                                s->bc.err.p);
   FEMX_HIP(hipGetLastError());
   // sol_prev = sol_n = sol_tau (:893-898)
-  FEMX_HIP(hipMemcpyAsync(s->P[0].p, s->Q[0].p, m.N * sizeof(double4), hipMemcpyDeviceToDevice, s->st));
+  if (s->bc.n_rec > 0)
+    rec_gather<<<grid_for(s->bc.n_rec, 256), 256, 0, s->st>>>(s->bc.n_rec, s->rec_node.p, s->Q[0].p, s->Pr[0].p);
+  FEMX_HIP(hipGetLastError());
   const long ns = 12L * m.E;
   const bool mixed = f32_elem(s->p.precision);
   if (sdus) {

@@ -332,6 +332,8 @@ __device__ __forceinline__ void vms_residual(const T DN[3][4], T V, const ElemVe
 constexpr int BC_BIT = 0x8000;
 constexpr int EC_ID = 0xff, EC_RANK_SHIFT = 8, EC_RANK = 0x7f;
 constexpr int LN_BEGIN = 0x3ff, LN_RANK_SHIFT = 10, LN_RANK = 0xff, LN_SHARED = 1 << 18, LN_BC = 1 << 19;
+// the block holding a shared node's first partial slot
+constexpr int LN_FIRST = 1 << 20;
 
 // The subgrid velocity of element e (block order), component r = g * 3 + i,
 // lives in tiles of 256 elements: [E / 256][12][256].  A block's 256
@@ -351,7 +353,8 @@ struct BlockArgs {
   // per local node {global node, meta}: meta holds the first position of the
   // node's contributions in the block's node-sorted order (LN_BEGIN), the
   // rank of its partial slot among the block's slots (LN_RANK), LN_SHARED
-  // for a block-boundary node and LN_BC for a node with a BC record
+  // for a block-boundary node, LN_FIRST in the block of its first slot, and
+  // LN_BC for a node with a BC record
   const int2 *lnode;
   // per element corner: local node (EC_ID), rank of the corner among the
   // node's contributions in ascending element order (EC_RANK), and BC_BIT
@@ -409,6 +412,17 @@ __device__ __forceinline__ void block_max_store(double v, double *out) {
 
 // update_dual_solution of one node (:764; im = 1 / nd_mass) + its BCs;
 // returns the pre-BC |u|^2.
+// BCs of one node's updated (pre-BC) state; returns the pre-BC |u|^2.
+__device__ __forceinline__ double node_finish(const double4 pre, const VmsConst &k, double4 &post, int ri,
+                                              const BCRecord *rec, const int *row_off, const double4 *rows,
+                                              int *err) {
+  double u = pre.x, v = pre.y, w = pre.z, p = pre.w;
+  const double m2 = u * u + v * v + w * w;
+  if (ri >= 0) apply_bc(rec[ri], row_off, rows, k.t, u, v, w, &p, err);
+  post = make_double4(u, v, w, p);
+  return m2;
+}
+
 __device__ __forceinline__ double node_update(const double4 base, double im, const double R[4], const VmsConst &k,
                                               double4 &pre, double4 &post, int ri, const BCRecord *rec,
                                               const int *row_off, const double4 *rows, int *err) {
@@ -464,6 +478,9 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   __shared__ __align__(16) unsigned char lraw[F32 ? sizeof(F32Lds) : 16];
   // first position of each local node's contributions, and 4 ne after the last
   __shared__ unsigned short sbeg[RES ? VN + 1 : 1];
+  // the base state of each local node, staged for its update or scaled
+  // partial
+  __shared__ double4 sbase[RES && F32 ? VN : 1];
   F32Lds &L = *reinterpret_cast<F32Lds *>(lraw);
   float4 *sfa = L.st.sfa, *sfb = L.st.sfb;
   float2 *sfc = L.st.sfc;
@@ -535,6 +552,8 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
     if (!F32) {
       base_keep = !ACC ? q_own : p_own;
       im_keep = x.w;
+    } else if (RES) {
+      sbase[j] = !ACC ? q_own : p_own;
     }
     if (F32) {
       sfa[j] = make_float4((float)(x.x - xr.x), (float)(x.y - xr.y), (float)(x.z - xr.z), (float)(p_own.x - pr.x));
@@ -710,16 +729,20 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       sres[ps[b]] = v;
     }
   }
+  // FP64 partials carry -dual_step / m times the block's sum, and the block
+  // of a node's first slot adds the node's base: the fix-up only sums them
+  constexpr bool SCALED = sizeof(S) == 8;
   // base of update_dual_solution (post-BC at the first dual iteration,
-  // pre-BC after) and 1/m of a complete node: FP32 element math re-reads
-  // them (from L2) to keep phase 2 within 96 VGPRs, issued before the
-  // barrier so that their latency overlaps its wait; FP64 keeps them
+  // pre-BC after) and 1/m: FP32 element math staged the base in LDS and
+  // re-reads 1/m (from L2, issued before the barrier so that its latency
+  // overlaps the wait) to keep phase 2 within 96 VGPRs; FP64 keeps both
   double4 base = base_keep;
   double im = im_keep;
   const bool complete = !(meta_own & LN_SHARED);
+  const bool first = SCALED && (meta_own & LN_FIRST);
   const int rec_own = has_n && complete && (meta_own & LN_BC) ? a.node_rec[g_own] : -1;
-  if (F32 && has_n && complete) {
-    base = ACC && rec_own >= 0 ? a.Pr[rec_own] : a.Q[g_own];
+  if (F32 && has_n) {
+    base = sbase[tid];
     im = a.xyzm[g_own].w;
   }
   __syncthreads();
@@ -746,10 +769,18 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       a.Qn[g] = post;
     } else {
       typename Vec4<S>::type v;
-      v.x = Rt[0];
-      v.y = Rt[1];
-      v.z = Rt[2];
-      v.w = Rt[3];
+      if constexpr (SCALED) {
+        const double ds = k.dual_step * im;
+        v.x = (first ? base.x : 0.0) - ds * R[0];
+        v.y = (first ? base.y : 0.0) - ds * R[1];
+        v.z = (first ? base.z : 0.0) - ds * R[2];
+        v.w = (first ? base.w : 0.0) - ds * R[3];
+      } else {
+        v.x = Rt[0];
+        v.y = Rt[1];
+        v.z = Rt[2];
+        v.w = Rt[3];
+      }
       static_cast<typename Vec4<S>::type *>(a.partials)[slot_base + ((meta_own >> LN_RANK_SHIFT) & LN_RANK)] = v;
     }
   }
@@ -784,15 +815,25 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
   double m2 = 0.0;
   if (s < a.n_sh) {
     // two rounds of independent loads: what s indexes (incl. its first 4
-    // slots, the last repeated when it has fewer), then the base state and
-    // those slots' partials (most nodes have at most 4)
+    // slots, the last repeated when it has fewer), then those slots'
+    // partials (most nodes have at most 4), and with FP32 partials the base
+    // state.  FP64 partials already hold -dual_step / m times each block's
+    // sum, the first one plus the base, so their sum is the updated state.
+    constexpr bool SCALED = sizeof(T) == 8;
     constexpr int FIX_BATCH = 4;
     const int g = a.sh_node[s];
     const int k0 = a.sh_off[s], k1 = a.sh_off[s + 1];
     const int4 f4 = a.sh_first4[s];
-    const double im = a.sh_im[s];
     const int rec = a.sh_rec[s];
-    const double4 base = !a.first_dual && rec >= 0 ? a.Pr[rec] : a.Q[g];
+    // a node of no element has no partial: it takes the reference's update
+    // with a zero residual and a zero mass
+    const bool unscaled = !SCALED || k1 == k0;
+    double im = 0.0;
+    double4 base = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (unscaled) {
+      im = a.sh_im[s];
+      base = !a.first_dual && rec >= 0 ? a.Pr[rec] : a.Q[g];
+    }
     const int fs[FIX_BATCH] = {f4.x, f4.y, f4.z, f4.w};
     T4 p[FIX_BATCH];
 #pragma unroll
@@ -815,7 +856,12 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     }
     const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
     double4 pre, post;
-    m2 = node_update(base, im, R, k, pre, post, rec, a.rec, a.row_off, a.rows, a.err);
+    if (unscaled) {
+      m2 = node_update(base, im, R, k, pre, post, rec, a.rec, a.row_off, a.rows, a.err);
+    } else {
+      pre = make_double4(R[0], R[1], R[2], R[3]);
+      m2 = node_finish(pre, k, post, rec, a.rec, a.row_off, a.rows, a.err);
+    }
     if (rec >= 0) a.Prn[rec] = pre;
     a.Qn[g] = post;
   }
@@ -1231,7 +1277,9 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
     for (int nd : s->bc.h_rec_node) has_rec[nd] = 1;
     std::vector<ushort4> ecr(std::max<long>(m.E, 1), make_ushort4(0, 0, 0, 0));
     std::vector<int2> ln(std::max(b.total_local, 1), make_int2(0, 0));
-    std::vector<int> slot0(b.nb + 1, 0);
+    std::vector<int> slot0(b.nb + 1, 0), first_slot(m.N, -1);
+    for (int si = 0; si < b.n_sh; si++)
+      if (b.sh_off[si + 1] > b.sh_off[si]) first_slot[b.sh_node[si]] = b.sh_slot[b.sh_off[si]];
     int slot = 0;
     for (int blk = 0; blk < b.nb; blk++) {
       const int n0 = b.node_off[blk], cbase = 4 * b.elem_off[blk];
@@ -1242,7 +1290,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
         int meta = c0 | (has_rec[g] ? LN_BC : 0);
         if (ps >= 0) {
           if (ps - slot0[blk] > LN_RANK || ps != slot) return fail((set_error("block partial slots out of order"), FEMX_ERR_ARG));
-          meta |= LN_SHARED | ((ps - slot0[blk]) << LN_RANK_SHIFT);
+          meta |= LN_SHARED | ((ps - slot0[blk]) << LN_RANK_SHIFT) | (ps == first_slot[g] ? LN_FIRST : 0);
           slot++;
         }
         ln[l] = make_int2(g, meta);

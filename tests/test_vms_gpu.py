@@ -216,3 +216,21 @@ def test_high_valence_node(gpu, precision, tol):
         g.step()
         sol, _, _ = g.get_state()
         assert rel_l2(sol, o.get("sol_n")) <= tol, (step, rel_l2(sol, o.get("sol_n")))
+
+
+@pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5), (2, 1e-5)])
+def test_node_of_no_element(gpu, precision, tol):
+    """A node that no element uses: the update divides its zero residual by
+    its zero mass, as the reference does, so its state turns NaN; every
+    other node keeps parity with the oracle."""
+    m0, bc, params, _ = _case("channel", 4, 0.1, seed=3)
+    m = meshgen.Mesh(np.vstack([m0.coords, [[5.0, 5.0, 5.0]]]), m0.conn)
+    ic = meshgen.random_velocity(m, 1e-3, 3)
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=precision)
+    for step in range(4):
+        o.step()
+        g.step()
+        sol, _, _ = g.get_state()
+        ref = o.get("sol_n")
+        assert np.isnan(ref[-1]).all() and np.isnan(sol[-1]).all(), (step, sol[-1])
+        assert rel_l2(sol[:-1], ref[:-1]) <= tol, (step, rel_l2(sol[:-1], ref[:-1]))

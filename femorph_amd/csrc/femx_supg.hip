@@ -52,6 +52,18 @@ __host__ __device__ constexpr double d_identity(int c) {
          : d_is_min(c) ? 1.79769313486231570815e+308 : -1.79769313486231570815e+308;
 }
 
+// Folds one node's new state into the velocity extrema (columns D_UMIN ..
+// D_UMIN + 5: min/max of u, v, w) and max |u|^2; NaN never wins.
+__device__ __forceinline__ void node_extrema(const double4 u, double v[6], double &m2) {
+  v[0] = fmin(v[0], u.x);
+  v[1] = fmax(v[1], u.x);
+  v[2] = fmin(v[2], u.y);
+  v[3] = fmax(v[3], u.y);
+  v[4] = fmin(v[4], u.z);
+  v[5] = fmax(v[5], u.z);
+  m2 = fmax(m2, u.x * u.x + u.y * u.y + u.z * u.z);
+}
+
 // Block-wide min/max of the columns [c0, c0+K) of a diagnostic row; thread 0
 // stores the block's values to out[blockIdx.x * D_COLS + c]. NaN never wins
 // (like the reference's `if (x > max)` tests).
@@ -107,7 +119,7 @@ struct SupgArgs {
   const int *row_off;
   const double4 *rows;
   int *err;
-  double *eblk;              // [nblk][D_COLS] per-block element diagnostics (DIAG)
+  double *eblk;              // [nblk][D_COLS] per-block rows: node extrema, element diagnostics (DIAG)
   int *halt;                 // INT_MAX or the step that exceeded EXITCOND
   const double *prev_row;    // previous step's diagnostic row (exit check), or null
   int step_id;
@@ -532,6 +544,11 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
   if (!DYN) return;
   if (DIAG) block_diag_store<SB, 11>(dg, D_TAU, a.eblk);
   __syncthreads();
+  // velocity extrema and max |u|^2 of the new states this block completes
+  // (get_solution_extrema :604-655), into the node columns of its row
+  double nv[6], nm2[1] = {0.0};
+#pragma unroll
+  for (int i = 0; i < 6; i++) nv[i] = d_identity(D_UMIN + i);
   const int base_s = e0 * NPE;
   for (int j = tid; j < nn; j += SB) {
     // the node's contributions summed in FP64, or in FP32 with f32sum
@@ -566,6 +583,7 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       a.Un[g] = next;
       if (a.Unp) a.Unp[g] = pst;
       a.Fprev[g] = fst;
+      node_extrema(next, nv, nm2[0]);
     } else {
       if (a.f32sum)
         static_cast<float4 *>(a.partials)[psl] = make_float4((float)F[0], (float)F[1], (float)F[2], 0.0f);
@@ -573,29 +591,11 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
         static_cast<double4 *>(a.partials)[psl] = make_double4(F[0], F[1], F[2], 0.0);
     }
   }
+  block_diag_store<SB, 6>(nv, D_UMIN, a.eblk);
+  __syncthreads();
+  block_diag_store<SB, 1>(nm2, D_UMOD, a.eblk);
 }
 
-// Velocity extrema and max |u|^2 of the new state (get_solution_extrema
-// :604-655), one row per block of 256 nodes.
-__global__ __launch_bounds__(256) void supg_node_diag(int N, const double4 *U, double *nblk, const int *halt) {
-  if (halted(halt)) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  double v[6], m2 = 0.0;
-  if (n < N) {
-    const double4 u = U[n];
-    v[0] = v[1] = u.x;
-    v[2] = v[3] = u.y;
-    v[4] = v[5] = u.z;
-    m2 = u.x * u.x + u.y * u.y + u.z * u.z;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 6; i++) v[i] = d_identity(i);
-  }
-  block_diag_store<256, 6>(v, D_UMIN, nblk);
-  double w[1] = {m2};
-  __syncthreads();
-  block_diag_store<256, 1>(w, D_UMOD, nblk);
-}
 
 // Reduce the per-block rows into this step's ring row (one workgroup).
 // Reduction of the per-block rows to the step's log row, in two stages: 32
@@ -611,6 +611,8 @@ __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int
 #pragma unroll
   for (int c = 0; c < D_COLS; c++) m[c] = d_identity(c);
   const int stride = gridDim.x * blockDim.x;
+  // node columns: the fix-up's rows and the element blocks' rows; element
+  // columns: the element blocks' rows when the element diagnostics are on
   for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nn_blk; b += stride) {
 #pragma unroll
     for (int c = 0; c < D_COLS; c++)
@@ -619,15 +621,14 @@ __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int
         m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
       }
   }
-  if (with_elem)
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ne_blk; b += stride) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ne_blk; b += stride) {
 #pragma unroll
-      for (int c = 0; c < D_COLS; c++)
-        if (!(c < 6 || c == D_UMOD)) {
-          const double x = eblk[(long)b * D_COLS + c];
-          m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
-        }
-    }
+    for (int c = 0; c < D_COLS; c++)
+      if (with_elem || c < 6 || c == D_UMOD) {
+        const double x = eblk[(long)b * D_COLS + c];
+        m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
+      }
+  }
   __shared__ double red[4][D_COLS];
 #pragma unroll
   for (int c = 0; c < D_COLS; c++) {
@@ -673,15 +674,31 @@ struct SupgFix {
   const double4 *rows;
   int *err;
   const int *halt;
+  double *nblk;  // [workgroups][D_COLS]: node columns of each workgroup's row
   SupgConst k;
 };
+
+template <typename T>
+__device__ __forceinline__ void fixup_node(const SupgFix &a, int s, double nv[6], double &nm2);
 
 template <typename T>
 __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
   using T4 = typename Vec4<T>::type;
   if (halted(a.halt)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n_sh) return;
+  double nv[6], nm2[1] = {0.0};
+#pragma unroll
+  for (int i = 0; i < 6; i++) nv[i] = d_identity(D_UMIN + i);
+  if (s < a.n_sh) fixup_node<T>(a, s, nv, nm2[0]);
+  // this workgroup's row of velocity extrema and max |u|^2 (:604-655)
+  block_diag_store<256, 6>(nv, D_UMIN, a.nblk);
+  __syncthreads();
+  block_diag_store<256, 1>(nm2, D_UMOD, a.nblk);
+}
+
+template <typename T>
+__device__ __forceinline__ void fixup_node(const SupgFix &a, int s, double nv[6], double &nm2) {
+  using T4 = typename Vec4<T>::type;
   const T4 *partials = static_cast<const T4 *>(a.partials);
   // two rounds of independent loads: what s indexes (incl. its first 4
   // slots, the last repeated when it has fewer), then the node's state and
@@ -717,6 +734,7 @@ __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
   a.Un[g] = next;
   if (a.Unp) a.Unp[g] = pst;
   a.Fprev[g] = fst;
+  node_extrema(next, nv, nm2);
 }
 
 // form_lumped_mass (:507-598): element row sums of the Bubnov mass
@@ -946,6 +964,7 @@ femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   a.rows = s->bc.rows.p;
   a.err = s->bc.err.p;
   a.halt = s->halt.p;
+  a.nblk = s->nblk.p;
   a.k = supg_const(s, t, euler_step);
   if (f32p) supg_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   else supg_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
@@ -1077,8 +1096,9 @@ femx_status supg_enqueue(femx_supg *s, int nsteps) {
     if ((r = launch_fix(s, euler_step, s->time)) != FEMX_OK) return r;
     s->cur ^= 1;
     s->press_pending = true;
-    const int nnb = grid_for(s->mesh->N, 256);
-    supg_node_diag<<<nnb, 256, 0, s->st>>>(s->mesh->N, s->U[s->cur].p, s->nblk.p, s->halt.p);
+    // node extrema rows: one per fix-up workgroup (nblk) and one per element
+    // block (node columns of eblk), written where the new states are formed
+    const int nnb = s->mesh->blk.n_sh ? grid_for(s->mesh->blk.n_sh, 256) : 0;
     if (nnb + s->mesh->blk.nb <= DIAG_ONE_PASS) {
       // few rows: one workgroup folds them straight into the log row
       supg_diag_partial<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0,

@@ -64,6 +64,34 @@ __device__ __forceinline__ void node_extrema(const double4 u, double v[6], doubl
   m2 = fmax(m2, u.x * u.x + u.y * u.y + u.z * u.z);
 }
 
+// Block-wide velocity extrema and max |u|^2 (columns D_UMIN .. D_UMIN + 5
+// and D_UMOD) in one pass; thread 0 stores them to out[blockIdx.x * D_COLS].
+template <int NT>
+__device__ __forceinline__ void block_node_store(double v[6], double m2, double *out) {
+  __shared__ double wv[NT / 64][7];
+  double x[7] = {v[0], v[1], v[2], v[3], v[4], v[5], m2};
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    const bool mn = i < 6 && d_is_min(D_UMIN + i);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(x[i], off);
+      x[i] = mn ? fmin(x[i], o) : fmax(x[i], o);
+    }
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < 7; i++) wv[threadIdx.x >> 6][i] = x[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      const bool mn = i < 6 && d_is_min(D_UMIN + i);
+      double r = wv[0][i];
+      for (int w = 1; w < NT / 64; w++) r = mn ? fmin(r, wv[w][i]) : fmax(r, wv[w][i]);
+      out[(long)blockIdx.x * D_COLS + (i < 6 ? D_UMIN + i : D_UMOD)] = r;
+    }
+}
+
 // Block-wide min/max of the columns [c0, c0+K) of a diagnostic row; thread 0
 // stores the block's values to out[blockIdx.x * D_COLS + c]. NaN never wins
 // (like the reference's `if (x > max)` tests).
@@ -591,9 +619,7 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
         static_cast<double4 *>(a.partials)[psl] = make_double4(F[0], F[1], F[2], 0.0);
     }
   }
-  block_diag_store<SB, 6>(nv, D_UMIN, a.eblk);
-  __syncthreads();
-  block_diag_store<SB, 1>(nm2, D_UMOD, a.eblk);
+  block_node_store<SB>(nv, nm2[0], a.eblk);
 }
 
 
@@ -691,9 +717,7 @@ __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
   for (int i = 0; i < 6; i++) nv[i] = d_identity(D_UMIN + i);
   if (s < a.n_sh) fixup_node<T>(a, s, nv, nm2[0]);
   // this workgroup's row of velocity extrema and max |u|^2 (:604-655)
-  block_diag_store<256, 6>(nv, D_UMIN, a.nblk);
-  __syncthreads();
-  block_diag_store<256, 1>(nm2, D_UMOD, a.nblk);
+  block_node_store<256>(nv, nm2[0], a.nblk);
 }
 
 template <typename T>

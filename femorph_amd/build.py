@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemx.so")
+OBJ_DIR = os.path.join(HERE, "_build")
 
 SOURCES = ["femx_core.hip", "femx_vms.hip", "femx_supg.hip", "femx_server.hip", "femx_ffd.hip", "femx_ins.hip"]
 
@@ -36,9 +37,14 @@ def hipcc() -> str:
 
 
 HOST = os.path.join(CSRC, "host")
+DROPIN = os.path.join(CSRC, "dropin")
 MODEL_LIB = os.path.join(HERE, "libfemx_model.so")
 DRIVER = os.path.join(HERE, "femx_explicit")
-HOST_SOURCES = ["femModel.cpp", "femExplicitFluidSolvers.cpp", "femPointGrid.cpp", "femIncompressibleSolver.cpp"]
+# the standalone feMorph model (femModel, femSolver base, femPointGrid) ...
+HOST_SOURCES = ["femModel.cpp", "femSolver.cpp", "femPointGrid.cpp"]
+# ... and the drop-in solver classes, which compile against it here and
+# against the reference's own headers inside a feMorph build
+DROPIN_SOURCES = ["femVMSExplicitFluidSolver.cpp", "femSUPGExplicitFluidSolver.cpp", "femIncompressibleSolver.cpp"]
 GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-result"]
 
 
@@ -56,7 +62,7 @@ def build_host(verbose: bool = False) -> None:
     libfemx.so) and the femx_explicit driver, with the host compiler."""
     inc = ["-I", os.path.join(ROOT, "include"), "-I", HOST]
     rpath = "-Wl,-rpath,$ORIGIN"
-    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES]
+    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES] + [os.path.join(DROPIN, s) for s in DROPIN_SOURCES]
     tmp = MODEL_LIB + ".tmp"
     _run(["g++"] + GXX_FLAGS + ["-fPIC", "-shared"] + inc + ["-o", tmp] + srcs +
          ["-L", HERE, "-l:libfemx.so", rpath], verbose, "libfemx_model.so")
@@ -69,10 +75,21 @@ def build_host(verbose: bool = False) -> None:
 
 
 def build(verbose: bool = False) -> str:
+    """Each .hip source compiles to an object in parallel (hipcc, gfx950),
+    then one link makes libfemx.so; the host libraries follow."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    inc = ["-I", os.path.join(ROOT, "include")]
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in srcs]
+    jobs = int(os.environ.get("MAX_JOBS", "0") or 0) or min(len(srcs), os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=max(1, min(jobs, 16))) as ex:
+        list(ex.map(lambda so: _run([hipcc()] + flags + inc + ["-c", so[0], "-o", so[1]], verbose, so[0]),
+                    zip(srcs, objs)))
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
-    _run(cmd, verbose, "libfemx.so")
+    _run([hipcc()] + HIPCC_FLAGS + ["-o", tmp] + objs, verbose, "libfemx.so")
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

@@ -1,5 +1,6 @@
-// femIncompressibleSolver.cpp -- solve(femModel*) of the implicit solver
-// over the femx C ABI.  The control flow follows the reference's solve()
+// femIncompressibleSolver.cpp -- femIncompressibleSolver of feMorph
+// (declared as in src/femIncompressibleSolver.h:6-11): solve(femModel*) of
+// the implicit solver over the femx C ABI.  The control flow follows the reference's solve()
 // (src/femIncompressibleSolver.cpp:289-496):
 //   - initial conditions: zeros, the INI entries, velocities at t = 0 (:253-287);
 //   - the initial export;
@@ -14,39 +15,16 @@
 #include <string>
 #include <vector>
 
+#include "femException.h"
 #include "femx.h"
+#include "femx_dropin.h"
 
 namespace {
 
-void check(femx_status st) {
-  if (st != FEMX_OK) {
-    std::string m = std::string("ERROR: femx: ") + femx_last_error() + "\n";
-    throw femException(m.c_str());
-  }
-}
-
-struct Handles {
-  femx_mesh *mesh = nullptr;
-  femx_ins *ins = nullptr;
-  ~Handles() {
-    femx_ins_destroy(ins);
-    femx_mesh_free(mesh);
-  }
-};
-
-void clear_results(femModel *m) {
-  for (auto *r : m->resultList) delete r;
-  m->resultList.clear();
-}
+using femx_dropin::check;
 
 void add_result(femModel *m, const char *label, int nc, const double *v, long rows, int stride) {
-  femResult *r = new femResult();
-  r->label = label;
-  r->type = frNode;
-  r->numComponents = nc;
-  r->values.resize(rows);
-  for (long i = 0; i < rows; i++) r->values[i].assign(v + i * stride, v + i * stride + nc);
-  m->resultList.push_back(r);
+  femx_dropin::add_result(m, label, frNode, nc, v, rows, stride);
 }
 
 // The three results of the reference (:335-380): INSSolution (dof 4),
@@ -54,13 +32,15 @@ void add_result(femModel *m, const char *label, int nc, const double *v, long ro
 void store_results(femModel *m, const std::vector<double> &phi, const std::vector<double> &phid,
                    const std::vector<double> &vel) {
   const long N = (long)phi.size();
-  clear_results(m);
+  femx_dropin::clear_results(m);
   add_result(m, "INSSolution", 1, phi.data(), N, 1);
   add_result(m, "INSSolution_Dot", 1, phid.data(), N, 1);
   add_result(m, "velocity", 3, vel.data(), N, 3);
 }
 
 }  // namespace
+
+femIncompressibleSolver::femIncompressibleSolver() {}
 
 void femIncompressibleSolver::solve(femModel *model) {
   const long N = (long)model->nodeList.size(), E = (long)model->elementList.size();
@@ -118,7 +98,7 @@ void femIncompressibleSolver::solve(femModel *model) {
     }
   }
 
-  Handles h;
+  femx_dropin::InsHandle h;
   const femx_mesh_desc md = {(int)N, (int)E, npe, coords.data(), conn.data()};
   check(femx_mesh_upload(&md, &h.mesh));
   femx_ins_params p;
@@ -126,9 +106,7 @@ void femIncompressibleSolver::solve(femModel *model) {
   p.alpha_m = model->alphaM;
   p.alpha_f = model->alphaF;
   p.gamma = model->gamma;
-  p.tol = tolerance;
-  p.max_iter = max_iterations;
-  p.restart = krylov_space;
+  femx_default_linear_solve(&p.tol, &p.max_iter, &p.restart);
   check(femx_ins_create(h.mesh, diff.empty() ? nullptr : diff.data(), (int)dir_node.size(), dir_node.data(),
                         dir_val.data(), &p, &h.ins));
 

@@ -1,5 +1,9 @@
 // femx_core.hip -- errors, mesh upload, BC tables, kernel timing.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <mutex>
+#include <string>
 #include <cstring>
 #include <numeric>
 
@@ -378,6 +382,52 @@ extern "C" int femx_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+// ------------------------------------ defaults of the drop-in solver classes
+// The classes keep the reference's declarations (no option members), so
+// their precision and the implicit solver's linear-solve settings are
+// process-wide values, read at each solve().
+namespace {
+int precision_from_env() {
+  const char *e = getenv("FEMX_PRECISION");
+  if (!e) return FEMX_FP64;
+  const std::string v(e);
+  if (v == "mixed" || v == "1") return FEMX_MIXED;
+  if (v == "mixed_f32sum" || v == "2") return FEMX_MIXED_F32SUM;
+  return FEMX_FP64;
+}
+std::atomic<int> g_precision{precision_from_env()};
+std::mutex g_solve_mu;
+double g_tol = 1.0e-5;  // AztecOO Iterate(1000, 1.0E-5), src/femSolver.cpp:146-177
+int g_max_iter = 1000;
+int g_restart = 30;
+}  // namespace
+
+extern "C" femx_status femx_set_default_precision(int precision) {
+  if (precision < FEMX_FP64 || precision > FEMX_MIXED_F32SUM)
+    return set_error("precision must be FEMX_FP64, FEMX_MIXED or FEMX_MIXED_F32SUM"), FEMX_ERR_ARG;
+  g_precision.store(precision);
+  return FEMX_OK;
+}
+
+extern "C" int femx_default_precision(void) { return g_precision.load(); }
+
+extern "C" femx_status femx_set_default_linear_solve(double tol, int max_iter, int restart) {
+  if (!(tol > 0.0) || max_iter < 1 || restart < 1 || restart > 64)
+    return set_error("linear solve: tol > 0, max_iter >= 1, restart 1..64"), FEMX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_solve_mu);
+  g_tol = tol;
+  g_max_iter = max_iter;
+  g_restart = restart;
+  return FEMX_OK;
+}
+
+extern "C" void femx_default_linear_solve(double *tol, int *max_iter, int *restart) {
+  std::lock_guard<std::mutex> lk(g_solve_mu);
+  if (tol) *tol = g_tol;
+  if (max_iter) *max_iter = g_max_iter;
+  if (restart) *restart = g_restart;
 }
 
 // ------------------------------------------------ HBM streaming probe

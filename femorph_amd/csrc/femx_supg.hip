@@ -630,8 +630,13 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
 // of its loads in flight together.
 constexpr int DIAG_PARTS = 32;
 constexpr int DIAG_ONE_PASS = 2048;  // up to this many rows, one workgroup and one launch
+// The l-CFL column is a running maximum since the first step (the reference
+// sets l_CFL_max once before its time loop, :996, and resets only the v- and
+// a-CFL maxima per step, :1078-1079): the kernel that writes the step's row
+// folds in the previous step's row (prev, null on the first step).
 __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int ne_blk, const double *nblk,
-                                                         int nn_blk, int with_elem, double *part, const int *halt) {
+                                                         int nn_blk, int with_elem, double *part, const int *halt,
+                                                         const double *prev) {
   if (halted(halt)) return;
   double m[D_COLS];
 #pragma unroll
@@ -669,11 +674,13 @@ __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int
     const int c = threadIdx.x;
     double v = red[0][c];
     for (int w = 1; w < 4; w++) v = d_is_min(c) ? fmin(v, red[w][c]) : fmax(v, red[w][c]);
+    if (c == D_LCFL && prev) v = fmax(v, prev[D_LCFL]);
     part[(long)blockIdx.x * D_COLS + c] = v;
   }
 }
 
-__global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np, double *row, const int *halt) {
+__global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np, double *row, const int *halt,
+                                                      const double *prev) {
   if (halted(halt)) return;
   if (threadIdx.x < D_COLS) {
     const int c = threadIdx.x;
@@ -683,6 +690,7 @@ __global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np
     double v = d_identity(c);
 #pragma unroll
     for (int q = 0; q < DIAG_PARTS; q++) v = d_is_min(c) ? fmin(v, x[q]) : fmax(v, x[q]);
+    if (c == D_LCFL && prev) v = fmax(v, prev[D_LCFL]);
     row[c] = v;
   }
 }
@@ -1126,11 +1134,11 @@ femx_status supg_enqueue(femx_supg *s, int nsteps) {
     if (nnb + s->mesh->blk.nb <= DIAG_ONE_PASS) {
       // few rows: one workgroup folds them straight into the log row
       supg_diag_partial<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0,
-                                              row, s->halt.p);
+                                              row, s->halt.p, prev);
     } else {
       supg_diag_partial<<<DIAG_PARTS, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb,
-                                                       s->diag_enabled ? 1 : 0, s->dpart.p, s->halt.p);
-      supg_diag_final<<<1, 64, 0, s->st>>>(s->dpart.p, DIAG_PARTS, row, s->halt.p);
+                                                       s->diag_enabled ? 1 : 0, s->dpart.p, s->halt.p, nullptr);
+      supg_diag_final<<<1, 64, 0, s->st>>>(s->dpart.p, DIAG_PARTS, row, s->halt.p, prev);
     }
     FEMX_HIP(hipGetLastError());
     s->pend_n++;

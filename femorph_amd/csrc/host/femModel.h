@@ -5,9 +5,10 @@
 // femResult (src/femResult.h:10-18) and femException (src/femException.h).
 #pragma once
 
-#include <exception>
 #include <string>
 #include <vector>
+
+#include "femException.h"
 
 using std::string;
 using std::vector;
@@ -21,15 +22,6 @@ typedef vector<vector<int> > femIntMat;
 typedef vector<double> femDoubleVec;
 typedef vector<int> femIntVec;
 
-class femException : public std::exception {
- public:
-  femException(const char *m) : msg(m) {}
-  virtual ~femException() throw() {}
-  virtual const char *what() const throw() { return msg.c_str(); }
-
- protected:
-  std::string msg;
-};
 
 class femNode {
  public:

@@ -17,8 +17,9 @@
 #include <cstring>
 #include <string>
 
-#include "femExplicitFluidSolvers.h"
 #include "femIncompressibleSolver.h"
+#include "femSUPGExplicitFluidSolver.h"
+#include "femVMSExplicitFluidSolver.h"
 #include "femPointGrid.h"
 #include "femx.h"
 
@@ -147,16 +148,15 @@ int main(int argc, char **argv) {
       delete model;
       return 0;
     }
+    if (mixed && femx_set_default_precision(FEMX_MIXED) != FEMX_OK) throw femException("ERROR: precision\n");
     if (ins) {
       femIncompressibleSolver s;
       s.solve(model);
     } else if (vms) {
       femVMSExplicitFluidSolver s;
-      s.precision = mixed ? FEMX_MIXED : FEMX_FP64;
       s.solve(model);
     } else {
       femSUPGExplicitFluidSolver s;
-      s.precision = mixed ? FEMX_MIXED : FEMX_FP64;
       s.solve(model);
     }
     delete model;

@@ -87,6 +87,10 @@ class KernelTime(C.Structure):
 SIGNATURES = [
     ("femx_last_error", C.c_char_p, []),
     ("femx_device_count", C.c_int, []),
+    ("femx_set_default_precision", C.c_int, [C.c_int]),
+    ("femx_default_precision", C.c_int, []),
+    ("femx_set_default_linear_solve", C.c_int, [C.c_double, C.c_int, C.c_int]),
+    ("femx_default_linear_solve", None, [_dp, _ip, _ip]),
     ("femx_mesh_upload", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("femx_mesh_free", C.c_int, [C.c_void_p]),
     ("femx_mesh_block_stats", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_longlong)]),

@@ -9,7 +9,7 @@
  * The reference has no FFI of its own: its boundary is the C++ class API
  * above, called from solveExplicitNS (src/main.cpp:788-805).  These entry
  * points are what those classes bind to (see the drop-in classes in
- * femorph_amd/csrc/host/): POD structs, explicit sizes, caller-owned host
+ * femorph_amd/csrc/dropin/): POD structs, explicit sizes, caller-owned host
  * buffers, device memory owned by handles, int status codes, no exceptions.
  *
  * Threading: a handle is used by one host thread at a time; every handle
@@ -80,6 +80,20 @@ typedef struct femx_supg femx_supg;
 const char *femx_last_error(void);
 /* Number of visible HIP devices (0 on a host without GPU). */
 int femx_device_count(void);
+
+/* Process-wide defaults read by the drop-in solver classes at each solve().
+ * No counterpart in the reference: its classes (src/femVMSExplicitFluidSolver.h:9-14,
+ * src/femSUPGExplicitFluidSolver.h:9-14, src/femIncompressibleSolver.h:6-11)
+ * declare no options, and the drop-ins keep those declarations, so the
+ * choices the reference hardcodes (FP64 arithmetic; AztecOO's
+ * Iterate(1000, 1.0E-5), src/femSolver.cpp:146-177) are set here.
+ * Precision starts from the environment variable FEMX_PRECISION (fp64,
+ * mixed, mixed_f32sum, or 0, 1, 2), else FEMX_FP64; the linear solve starts
+ * at tol 1e-5, 1000 iterations, Krylov space 30. */
+femx_status femx_set_default_precision(int precision);
+int femx_default_precision(void);
+femx_status femx_set_default_linear_solve(double tol, int max_iter, int restart);
+void femx_default_linear_solve(double *tol, int *max_iter, int *restart);
 
 /* Upload a mesh to the current device: SoA/AoS device layout, node->element
  * gather lists, locality-ordered element blocks.  The descriptor's arrays

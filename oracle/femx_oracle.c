@@ -678,6 +678,8 @@ struct oxSUPG {
   long loop_time;
   double time;
   double logv[19];
+  double l_CFL_max;  /* running max since the first step (:996, never reset in the loop) */
+  double l_CFL_step; /* this step's own l-CFL maximum (test hook, not a log column) */
   int e0, e1;
 };
 
@@ -860,6 +862,7 @@ int ox_supg_set_ic(oxSUPG *s, const double *sol) {
   supg_pressure(s, s->sol_p);
   s->loop_time = 0;
   s->time = 0.0;
+  s->l_CFL_max = 0.0; /* :996 */
   return 0;
 }
 
@@ -878,7 +881,10 @@ int ox_supg_step(oxSUPG *s) {
     s->el_regime[e] = 0.0;
   }
   double min_tau = DBL_MAXV, max_tau = -DBL_MAXV, min_lam = DBL_MAXV, max_lam = -DBL_MAXV;
-  double v_CFL_max = 0.0, a_CFL_max = 0.0, l_CFL_max = 0.0;
+  /* v- and a-CFL maxima restart every step (:1078-1079); the l-CFL maximum
+   * is set once before the loop (:996) and keeps growing across steps */
+  double v_CFL_max = 0.0, a_CFL_max = 0.0, l_CFL_max = s->l_CFL_max;
+  double l_CFL_step = 0.0;
   double E_min = DBL_MAXV, E_max = -DBL_MAXV, nu_min = DBL_MAXV, nu_max = -DBL_MAXV;
   static double K1[24][24], K2[24][24], K3[24][24], K4[24][24], K[24][24];
   double xel[24], el_sol_p[24], N_[8], DN[24], G[9], detJ, u_gp[3];
@@ -922,6 +928,7 @@ int ox_supg_step(oxSUPG *s) {
       s->el_regime[e] += regime;
       double l_CFL = 2.0 * (lam) * (dt / (el_len * el_len));
       if (l_CFL > l_CFL_max) l_CFL_max = l_CFL;
+      if (l_CFL > l_CFL_step) l_CFL_step = l_CFL;
       /* eval_L_matrix :138-157 */
       for (int c = 0; c < 3; c++)
         for (int A = 0; A < nd; A++) L[c][A] = 0.0;
@@ -1061,6 +1068,8 @@ int ox_supg_step(oxSUPG *s) {
       if (u[k] < mn[k]) mn[k] = u[k];
     }
   }
+  s->l_CFL_max = l_CFL_max;
+  s->l_CFL_step = l_CFL_step;
   double lv[19] = {(double)s->loop_time, s->time, mn[0], mx[0], mn[1], mx[1], mn[2], mx[2], min_tau, max_tau,
                    min_lam, max_lam, v_CFL_max, a_CFL_max, l_CFL_max, E_min, E_max, nu_min, nu_max};
   memcpy(s->logv, lv, sizeof(lv));
@@ -1084,6 +1093,7 @@ int ox_supg_get(const oxSUPG *s, int which, double *out) {
       out[0] = s->time;
       out[1] = (double)s->loop_time;
       return 0;
+    case 10: out[0] = s->l_CFL_step; return 0;
   }
   return -1;
 }

@@ -161,7 +161,8 @@ class SUPGOracle:
     """FP64 restatement of femSUPGExplicitFluidSolver::solve, step by step."""
 
     GET = dict(sol_n=(0, "N3"), sol_p=(1, "N3"), l_M=(2, "N3"), el_lam=(3, "E"), el_regime=(4, "E"),
-               pres_n=(5, "E"), k_sol_p=(6, "E24"), k_sol_pp=(7, "E24"), log=(8, "S19"), scalars=(9, "S2"))
+               pres_n=(5, "E"), k_sol_p=(6, "E24"), k_sol_pp=(7, "E24"), log=(8, "S19"), scalars=(9, "S2"),
+               l_cfl_step=(10, "S1"))
 
     def __init__(self, mesh, bc, rho, mu, alpha, cI, dt, exopts=(0, 1, 1, 1, 1, 1), fixed_dofs="NONE",
                  exit_condition=1.0e5):
@@ -201,7 +202,7 @@ class SUPGOracle:
 
     def get(self, name):
         which, shape = self.GET[name]
-        dims = {"N3": (self.N, 3), "E": (self.E,), "E24": (24, self.E), "S19": (19,), "S2": (2,)}[shape]
+        dims = {"N3": (self.N, 3), "E": (self.E,), "E24": (24, self.E), "S19": (19,), "S2": (2,), "S1": (1,)}[shape]
         out = np.zeros(dims)
         lib().ox_supg_get(self._h, which, _p(out))
         return out

@@ -43,7 +43,7 @@ def _singles(femx, items, ics, steps):
     return out
 
 
-@pytest.mark.parametrize("precision,tot_dual", [(0, 0), (1, 0), (0, 1), (2, 0)])
+@pytest.mark.parametrize("precision,tot_dual", [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0)])
 def test_batch_equals_separate_solves(gpu, precision, tot_dual):
     items, ics = _items(1, precision, tot_dual)
     print("batch of", [m.n_elems for m, _, _ in items], "precision", precision, "dual", tot_dual, flush=True)
@@ -85,6 +85,49 @@ def test_batch_member_vs_oracle(gpu):
         sol, t = b.get_state(1)
         assert rel_l2(sol, o.get("sol_n")) <= 1e-12, step
         assert t == o.get("scalars")[0]
+
+
+def test_batch_mixed_dual_member_vs_oracle(gpu):
+    """Mixed precision with a second dual iteration per step, one member of
+    a batch against the FP64 oracle (velocity <= 1e-5)."""
+    items, ics = _items(5, precision=1, tot_dual=1, sizes=(5, 4, 6))
+    b = gpu.VMSBatch(items)
+    b.set_state(ics)
+    m, bc, p = items[2]
+    o = VMSOracle(m, bc, p["rho"], p["mu"], p["inv_eps"], p["c1"], p["c2"], p["dt"], tot_dual_steps=1)
+    o.set_ic(ics[2])
+    b.step(12)
+    o.step(12)
+    sol, t = b.get_state(2)
+    assert t == o.get("scalars")[0]
+    assert rel_l2(sol[:, :3], o.get("sol_n")[:, :3]) <= 1e-5
+
+
+def test_batch_c4_shaped(gpu):
+    """A C4-shaped batch at reduced count: 40 cavities, n log-uniform in
+    [8, 20] (3,072 .. 48,000 tets each, uneven), Re uniform in [50, 500],
+    mixed precision; every member bitwise equal to its own single-mesh solve."""
+    rng = np.random.default_rng(21)
+    ns = np.rint(np.exp(rng.uniform(np.log(8), np.log(20), 40))).astype(int)
+    items, ics, cache = [], [], {}
+    for i, n in enumerate(ns):
+        if n not in cache:
+            m = meshgen.tet_cube(int(n))
+            cache[n] = (m, meshgen.lid_cavity_bc(m), meshgen.min_tet_h(m))
+        m, bc, h = cache[n]
+        re = float(rng.uniform(50, 500))
+        items.append((m, bc, dict(rho=1.0, mu=1.0 / re, inv_eps=1.0, c1=4.0, c2=2.0, dt=0.1 * h,
+                                  tot_dual_steps=0, precision=1)))
+        ics.append(meshgen.random_velocity(m, 1e-3, 100 + i))
+    print("C4-shaped batch:", len(items), "meshes,", sum(m.n_elems for m, _, _ in items), "tets", flush=True)
+    b = gpu.VMSBatch(items, block_elems=256, xcd_order=1)
+    b.set_state(ics)
+    b.step(12)
+    ref = _singles(gpu, items, ics, 12)
+    for i, (sol_ref, t_ref) in enumerate(ref):
+        sol, t = b.get_state(i)
+        assert t == t_ref
+        assert np.array_equal(sol, sol_ref), (i, int(ns[i]), rel_l2(sol, sol_ref))
 
 
 def test_autotune_picks_a_candidate(gpu):

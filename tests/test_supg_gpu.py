@@ -90,15 +90,22 @@ def test_tet4_bhat_rejected(gpu):
         gpu.SUPGSolver(dm, bc, exopts=(0, 1, 1, 1, 1, 1), **params)
 
 
-def test_fixed_dofs_z(gpu):
-    """NODEDOF 3 Z: pressure uses the state before the zeroing (:1524 vs :1527)."""
+FIXED = {"X": (0,), "Y": (1,), "Z": (2,), "XY": (0, 1), "YZ": (1, 2), "XZ": (0, 2)}
+
+
+@pytest.mark.parametrize("fixed", list(FIXED))
+def test_fixed_dofs(gpu, fixed):
+    """NODEDOF 3 <mode> (:1527-1561): the zeroed components stay zero, and the
+    pressure uses the state before the zeroing (:1524 vs :1527)."""
     m, bc, params, ic = _case("hexa", 4, 0.15, seed=6)
-    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), fixed="Z")
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), fixed=fixed)
     for step in range(8):
         o.step()
         g.step()
         st = g.get_state()
-        assert not st["sol"][:, 2].any()
+        for c in FIXED[fixed]:
+            assert not st["sol"][:, c].any()
+        assert st["sol"][:, [c for c in range(3) if c not in FIXED[fixed]]].any()
         assert rel_l2(st["sol"], o.get("sol_n")) <= TOL64, step
         assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
 
@@ -112,6 +119,30 @@ def test_log_columns(gpu):
     lo, lg = o.get("log"), g.get_log()
     assert lg[0] == lo[0] and lg[1] == lo[1]
     np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-12, atol=0)
+
+
+def test_log_lcfl_running_max(gpu):
+    """Column 14 (l-CFL) is a running maximum since step 1: the reference sets
+    l_CFL_max once before the loop (femSUPGExplicitFluidSolver.cpp:996) and
+    resets only the v- and a-CFL maxima per step (:1078-1079).  A decaying
+    random field in a closed box makes the per-step maximum fall."""
+    m = meshgen.hex_cube(5, 0.2, 7)
+    bc = meshgen.lid_cavity_bc(m, u_lid=0.0)
+    params = dict(rho=1.0, mu=0.02, alpha=1.0, cI=4.0, dt=0.02)
+    ic = meshgen.random_velocity(m, 0.5, 11)[:, :3]
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
+    g.set_diagnostics(True)
+    per_step, col = [], []
+    for step in range(10):
+        o.step()
+        g.step()
+        lo, lg = o.get("log"), g.get_log()
+        np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-12, atol=0, err_msg=f"step {step}")
+        per_step.append(o.get("l_cfl_step")[0])
+        col.append(lg[14])
+    assert min(np.diff(per_step)) < 0, "case must have a step whose own l-CFL maximum falls"
+    assert np.all(np.diff(col) >= 0)
+    np.testing.assert_allclose(col, np.maximum.accumulate(per_step), rtol=1e-12, atol=0)
 
 
 def test_exit_condition(gpu):
@@ -146,5 +177,16 @@ def test_mixed_precision_error(gpu, precision):
     o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), precision=precision)
     o.step(60)
     g.step(60)
+    err = rel_l2(g.get_state()["sol"], o.get("sol_n"))
+    assert err <= 1e-5, err
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_tet4(gpu, precision):
+    """Mixed SUPG on tet4 (the FP32 kernel's 4-node variant) vs the FP64 oracle."""
+    m, bc, params, ic = _case("tet", 5, 0.1, seed=12)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 0, 1, 1, 1, 1), precision=precision)
+    o.step(40)
+    g.step(40)
     err = rel_l2(g.get_state()["sol"], o.get("sol_n"))
     assert err <= 1e-5, err

@@ -15,18 +15,28 @@
  *  - SUPG lumped mass       : the reference's golden file
  *                             examples/.../06_2D_hexa_lid_driven_cavity/
  *                             lumped_mass.txt (60,000 values);
- *  - VMS initial assembly   : V = detJ/6, nodal mass sums to the volume;
- *  - VMS residual           : zero state => 0; uniform u, constant p, zero
- *                             subscale => 0 at interior nodes; continuity row
- *                             of a linear field; brute-force pure-Python
- *                             evaluation of Appendix-C formulas on 1 element;
- *  - VMS projection / tau   : closed forms on one element;
- *  - SUPG K.u               : uniform velocity fixed point, rigid rotation
- *                             K2 u = 0 (no B-hat), matrix-free pure-Python
- *                             evaluation on one element;
+ *  - VMS initial assembly   : V = detJ/6, nodal mass = sum of V/4;
+ *  - VMS residual           : per term against its Galerkin integral,
+ *                             computed in numpy from exact tetrahedron
+ *                             moments (acceleration, nu diffusion,
+ *                             convection, pressure, subgrid convection,
+ *                             continuity), each isolated by differencing;
+ *                             the assembled update of one step;
+ *  - VMS projection / tau   : the subscale update, tau1 and tau2 of one
+ *                             step against their definitions per element;
+ *  - SUPG K.u               : K1..K4 one at a time against their PDE forms
+ *                             on affine hexa8 (independent 4x4x4 rule) and
+ *                             tet4 cells; lambda and its regime; the Euler
+ *                             update with the lumped mass; uniform-velocity
+ *                             fixed point, rigid rotation;
  *  - BC tables              : textbook linear interpolation and ordering.
- * The full-trajectory behaviour (quirk interplay) is pinned only by these
- * component checks: the reference needs Boost to compile and is not built.
+ * tests/test_oracle_mutations.py plants 35 single mistakes (signs, dropped
+ * coefficients and terms, transposed indices) in copies of this file; each
+ * makes one of those checks fail.  Not pinned (the reference needs Boost
+ * and is not built): the quirk-level operation order (pressure of global
+ * nodes 0..3 in the projection, pre- vs post-BC states, B-hat sign, the
+ * metric's index order), which follows the reference's code as it reads,
+ * and the centroid residual diagnostics.
  */
 #include "femx_oracle.h"
 

@@ -48,6 +48,9 @@ femx_status BCDev::build(const femx_bc_desc *bc, int N, bool with_pressure, hipS
     if (d < 0 || d > 6) return set_error("ERROR: Invalid direction for node velocities."), FEMX_ERR_BC;
     if (bc->vel_row_off[i + 1] <= bc->vel_row_off[i]) return set_error("empty velocity table"), FEMX_ERR_ARG;
     BCRecord &r = rec_of(nd);
+    // one table per node: the reference's node list is compacted (sorted,
+    // unique) before setNodeVelocity walks it (src/femModel.cpp:4558-4576)
+    if (r.vel >= 0) return set_error("velocity BC node listed twice (one table per node)"), FEMX_ERR_ARG;
     r.vel = i;
     r.vdir = d;
   }
@@ -431,45 +434,89 @@ extern "C" void femx_default_linear_solve(double *tol, int *max_iter, int *resta
 }
 
 // ------------------------------------------------ HBM streaming probe
-// Grid-stride float4 streams over buffers of `n4` float4 each; every access
-// index is < n4 (the mix reads 7 rows and writes 5 rows of m4 = n4 / 7).
+// float4 streams over buffers of `n4` float4 each.  Every thread keeps U
+// independent accesses in flight per pass (indices i + j * stride, j < U, with
+// stride = all threads of the grid); every access index is < n4 (the mix
+// reads 7 rows and writes 5 rows of m4 = n4 / 7).  NT: non-temporal loads and
+// stores (the streaming hint: the data is not kept in the caches).
 namespace {
-__global__ void probe_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long n4) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ f4v ld4(const f4v *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
-__global__ void probe_read(const float4 *__restrict__ a, float *__restrict__ out, long n4) {
-  float s = 0.f;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
-    const float4 v = a[i];
-    s += (v.x + v.y) + (v.z + v.w);
+template <bool NT>
+__device__ __forceinline__ void st4(f4v *p, f4v v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+constexpr int PU = 4;
+template <bool NT>
+__global__ __launch_bounds__(256) void probe_copy(const f4v *__restrict__ a, f4v *__restrict__ b, long n4) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += PU * stride) {
+    f4v v[PU];
+#pragma unroll
+    for (int j = 0; j < PU; j++)
+      if (i + j * stride < n4) v[j] = ld4<NT>(a + i + j * stride);
+#pragma unroll
+    for (int j = 0; j < PU; j++)
+      if (i + j * stride < n4) st4<NT>(b + i + j * stride, v[j]);
   }
-  if (s == 1234.5f) out[blockIdx.x] = s;  // never true for the zero-filled input; keeps the loads
 }
-__global__ void probe_write(float4 *__restrict__ b, long n4) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
-    b[i] = make_float4(1.f, 2.f, 3.f, 4.f);
+template <bool NT>
+__global__ __launch_bounds__(256) void probe_read(const f4v *__restrict__ a, float *__restrict__ out, long n4) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  f4v s = {0.f, 0.f, 0.f, 0.f};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += PU * stride) {
+    f4v v[PU];
+#pragma unroll
+    for (int j = 0; j < PU; j++) v[j] = i + j * stride < n4 ? ld4<NT>(a + i + j * stride) : f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < PU; j++) s += v[j];
+  }
+  const float t = (s.x + s.y) + (s.z + s.w);
+  if (t == 1234.5f) out[blockIdx.x] = t;  // never true for the zero-filled input; keeps the loads
 }
-__global__ void probe_mix(const float4 *__restrict__ a, float4 *__restrict__ b, long m4) {
+template <bool NT>
+__global__ __launch_bounds__(256) void probe_write(f4v *__restrict__ b, long n4) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  const f4v v = {1.f, 2.f, 3.f, 4.f};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += PU * stride) {
+#pragma unroll
+    for (int j = 0; j < PU; j++)
+      if (i + j * stride < n4) st4<NT>(b + i + j * stride, v);
+  }
+}
+template <bool NT>
+__global__ __launch_bounds__(256) void probe_mix(const f4v *__restrict__ a, f4v *__restrict__ b, long m4) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m4; i += (long)gridDim.x * blockDim.x) {
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    f4v v[7];
 #pragma unroll
-    for (int r = 0; r < 7; r++) {
-      const float4 v = a[r * m4 + i];
-      s.x += v.x;
-      s.y += v.y;
-      s.z += v.z;
-      s.w += v.w;
-    }
+    for (int r = 0; r < 7; r++) v[r] = ld4<NT>(a + r * m4 + i);
+    f4v s = v[0];
 #pragma unroll
-    for (int w = 0; w < 5; w++) b[w * m4 + i] = s;
+    for (int r = 1; r < 7; r++) s += v[r];
+#pragma unroll
+    for (int w = 0; w < 5; w++) st4<NT>(b + w * m4 + i, s);
   }
+}
+template <bool NT>
+void probe_launch(int pattern, int grid, hipStream_t st, const f4v *a, f4v *b, float *out, long n4, long m4) {
+  if (pattern == 0) probe_copy<NT><<<grid, 256, 0, st>>>(a, b, n4);
+  else if (pattern == 1) probe_read<NT><<<grid, 256, 0, st>>>(a, out, n4);
+  else if (pattern == 2) probe_write<NT><<<grid, 256, 0, st>>>(b, n4);
+  else probe_mix<NT><<<grid, 256, 0, st>>>(a, b, m4);
 }
 }  // namespace
 
 extern "C" femx_status femx_hbm_probe(int kind, long long bytes, int grid, int reps, double *tb_per_s) {
-  if (!tb_per_s || kind < 0 || kind > 3 || bytes < (1 << 20) || grid < 1 || grid > (1 << 20) || reps < 1)
+  if (!tb_per_s || kind < 0 || kind > 7 || bytes < (1 << 20) || grid < 1 || grid > (1 << 20) || reps < 1)
     return set_error("femx_hbm_probe: arguments"), FEMX_ERR_ARG;
   if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  const int pattern = kind & 3;
+  const bool nt = (kind & 4) != 0;
   const long n4 = (long)(bytes / 16), m4 = n4 / 7;
   DevBuf<float4> a, b;
   DevBuf<float> out;
@@ -477,11 +524,11 @@ extern "C" femx_status femx_hbm_probe(int kind, long long bytes, int grid, int r
   if ((r = a.alloc(n4)) != FEMX_OK || (r = b.alloc(n4)) != FEMX_OK || (r = out.alloc(grid)) != FEMX_OK) return r;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  const f4v *ap = reinterpret_cast<const f4v *>(a.p);
+  f4v *bp = reinterpret_cast<f4v *>(b.p);
   auto launch = [&]() {
-    if (kind == 0) probe_copy<<<grid, 256, 0, st>>>(a.p, b.p, n4);
-    else if (kind == 1) probe_read<<<grid, 256, 0, st>>>(a.p, out.p, n4);
-    else if (kind == 2) probe_write<<<grid, 256, 0, st>>>(b.p, n4);
-    else probe_mix<<<grid, 256, 0, st>>>(a.p, b.p, m4);
+    if (nt) probe_launch<true>(pattern, grid, st, ap, bp, out.p, n4, m4);
+    else probe_launch<false>(pattern, grid, st, ap, bp, out.p, n4, m4);
   };
   float ms = 0.f;
   bool ok = hipMemsetAsync(a.p, 0, n4 * sizeof(float4), st) == hipSuccess &&
@@ -496,7 +543,7 @@ extern "C" femx_status femx_hbm_probe(int kind, long long bytes, int grid, int r
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (!ok || ms <= 0.f) return set_error("femx_hbm_probe: HIP launch or timing failed"), FEMX_ERR_HIP;
-  const double moved = kind == 0 ? 2.0 * n4 * 16 : kind == 3 ? 12.0 * m4 * 16 : 1.0 * n4 * 16;
+  const double moved = pattern == 0 ? 2.0 * n4 * 16 : pattern == 3 ? 12.0 * m4 * 16 : 1.0 * n4 * 16;
   *tb_per_s = moved * reps / (ms * 1e-3) / 1e12;
   return FEMX_OK;
 }

@@ -214,12 +214,14 @@ def device_count() -> int:
     return load().femx_device_count()
 
 
-HBM_PROBE_KINDS = {"copy": 0, "read": 1, "write": 2, "mix_7r5w": 3}
+HBM_PROBE_KINDS = {"copy": 0, "read": 1, "write": 2, "mix_7r5w": 3,
+                   "copy_nt": 4, "read_nt": 5, "write_nt": 6, "mix_7r5w_nt": 7}
 
 
 def hbm_probe(kind: str, nbytes: int = 1 << 30, grid: int = 4096, reps: int = 20) -> float:
     """TB/s of a float4 streaming kernel (femx_hbm_probe): copy, read,
-    write, or a 7-read / 5-write mix, over buffers of nbytes."""
+    write, or a 7-read / 5-write mix, over buffers of nbytes; the "_nt"
+    kinds use non-temporal loads and stores."""
     out = C.c_double(0.0)
     _check(load().femx_hbm_probe(HBM_PROBE_KINDS[kind], int(nbytes), int(grid), int(reps), C.byref(out)))
     return out.value

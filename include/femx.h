@@ -381,11 +381,12 @@ int femx_ins_kernel_times(femx_ins *s, femx_kernel_time *out, int max_entries);
 
 /* HBM streaming probe (no counterpart in the reference: it measures the
  * practical bandwidth ceiling the kernels' roofline fractions are read
- * against).  kind: 0 copy (1 read + 1 write stream), 1 read, 2 write,
+ * against).  kind & 3: 0 copy (1 read + 1 write stream), 1 read, 2 write,
  * 3 mix of 7 read and 5 write streams (about the VMS block kernel's ratio);
- * float4 grid-stride kernels over `bytes` per buffer, `grid` workgroups of
- * 256 threads, `reps` timed launches after one warm-up.  Allocates two
- * buffers of `bytes` on the current device and frees them. */
+ * kind & 4: non-temporal loads and stores.  float4 streams over `bytes` per
+ * buffer, four accesses in flight per thread (seven reads for the mix),
+ * `grid` workgroups of 256 threads, `reps` timed launches after one warm-up.
+ * Allocates two buffers of `bytes` on the current device and frees them. */
 femx_status femx_hbm_probe(int kind, long long bytes, int grid, int reps, double *tb_per_s);
 
 #ifdef __cplusplus

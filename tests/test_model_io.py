@@ -97,6 +97,31 @@ def test_reference_example_model_both_readers(tmp_path):
     assert (det != 0).all()
 
 
+@pytest.mark.parametrize("name,counts", [
+    # nodes, elements, NODEDIRBC, NODEVEL rows, velocity-BC nodes, NODEPRES, (dt, steps, save), (dual, tot dual)
+    ("cylinder", (8595, 45754, 2565, 202, 101, 102, (0.01, 5000, 10), (1.0e-3, 0))),
+    ("ldc_medium", (7309, 36468, 2392, 1022, 511, 0, (1.0e-3, 2, 1), (1.0e-6, 100))),
+])
+def test_reference_examples_both_readers(tmp_path, name, counts):
+    """01_cylinder and 03_lid_driven_cavity_medium of the reference's
+    examples: the counts of their keyword lines (grep of the files), and the
+    C++ and Python readers agree exactly."""
+    path = os.path.join(str(tmp_path), "model.txt")
+    with gzip.open(os.path.join(GOLD, f"{name}_model.txt.gz"), "rb") as f:
+        open(path, "wb").write(f.read())
+    m = modelfile.read_model_txt(path)
+    nn, ne, ndir, nvel_rows, nvel, npres, ts, ds = counts
+    assert len(m.coords) == nn and len(m.conn) == ne and set(m.elem_types) == {"TET4"}
+    assert len(m.bc.dir_node) == ndir and int(m.bc.vel_row_off[-1]) == nvel_rows and len(m.bc.vel_node) == nvel
+    assert len(m.bc.pres_node) == npres
+    assert (m.time_step, m.total_steps, m.save_every) == ts and (m.dual_step, m.tot_dual_steps) == ds
+    assert m.vmsprops == [1.06, 0.04, 400.0, 4.0, 2.0]
+    _compare(_dump(path, tmp_path), m)
+    X = m.coords[np.array(m.conn)]
+    det = np.einsum("ij,ij->i", X[:, 1] - X[:, 0], np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))
+    assert (det > 0).all()
+
+
 @pytest.mark.parametrize("kind", ["tet", "hex"])
 def test_generated_model_round_trip(tmp_path, kind):
     if kind == "tet":

@@ -241,3 +241,24 @@ def test_node_of_no_element(gpu, precision, tol):
         ref = o.get("sol_n")
         assert np.isnan(ref[-1]).all() and np.isnan(sol[-1]).all(), (step, sol[-1])
         assert rel_l2(sol[:-1], ref[:-1]) <= tol, (step, rel_l2(sol[:-1], ref[:-1]))
+
+
+def test_duplicate_velocity_node_rejected(gpu):
+    """The reference walks a compacted (sorted, unique) velocity-node list,
+    one time table per node (src/femModel.cpp:4558-4576, :4581-4628); a C-ABI
+    caller that lists a node twice gets an argument error, for the VMS and
+    the SUPG solver alike."""
+    m, bc, params, _ = _case("cavity", 3)
+    k = int(bc.vel_node[0])
+    rows = bc.vel_rows[bc.vel_row_off[0]:bc.vel_row_off[1]]
+    dup = meshgen.BoundaryConditions(
+        vel_node=np.r_[bc.vel_node, k], vel_dir=np.r_[bc.vel_dir[:len(bc.vel_node)], 1],
+        vel_row_off=np.r_[bc.vel_row_off, bc.vel_row_off[-1] + len(rows)], vel_rows=np.vstack([bc.vel_rows, rows]),
+        dir_node=bc.dir_node, dir_val=bc.dir_val, pres_node=bc.pres_node, pres_val=bc.pres_val)
+    dm = gpu.DeviceMesh(m)
+    with pytest.raises(gpu.FemxError, match="listed twice"):
+        gpu.VMSSolver(dm, dup, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"])
+    with pytest.raises(gpu.FemxError, match="listed twice"):
+        gpu.SUPGSolver(dm, dup, rho=1.0, mu=0.01, alpha=1.0, cI=4.0, dt=params["dt"])
+    # the unmodified tables still build
+    gpu.VMSSolver(dm, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"])

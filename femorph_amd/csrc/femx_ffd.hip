@@ -9,6 +9,7 @@
 // :170-200).  Nodes outside the box get their box coordinates, as the
 // reference stores them (:193-197).
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "femx_common.h"
@@ -115,18 +116,11 @@ extern "C" femx_status femx_ffd_control_points(const femx_ffd_box *box, double *
   return FEMX_OK;
 }
 
-extern "C" femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, const double *coords, double *disp) {
-  if (!box || n_points < 0 || (n_points > 0 && (!coords || !disp))) return set_error("bad argument"), FEMX_ERR_ARG;
-  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+namespace {
+// The box data of the morph kernel (getBoxCoords :207-254 and the 1D
+// Bernstein coefficients :121-133), from a validated box.
+void ffd_setup(const femx_ffd_box *box, FfdDev &b) {
   const int *n = box->tot_points;
-  for (int d = 0; d < 3; d++)
-    if (n[d] < 2 || n[d] > FFD_MAX) return set_error("control points per axis must be in 2..13"), FEMX_ERR_ARG;
-  const long tot = (long)n[0] * n[1] * n[2];
-  std::vector<double> d(3 * tot);
-  femx_status r = femx_ffd_control_points(box, nullptr, d.data());
-  if (r != FEMX_OK) return r;
-  if (n_points == 0) return FEMX_OK;
-  FfdDev b;
   double S[3], T[3], U[3];
   for (int c = 0; c < 3; c++) {
     b.minp[c] = box->min_point[c];
@@ -153,6 +147,28 @@ extern "C" femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, con
     for (int i = 0; i < FFD_MAX; i++)
       b.binom[a][i] = i < n[a] ? (double)(fact(n[a] - 1) / (fact(n[a] - 1 - i) * fact(i))) : 0.0;
   }
+}
+
+femx_status ffd_check_points(const femx_ffd_box *box) {
+  const int *n = box->tot_points;
+  for (int d = 0; d < 3; d++)
+    if (n[d] < 2 || n[d] > FFD_MAX) return set_error("control points per axis must be in 2..13"), FEMX_ERR_ARG;
+  return FEMX_OK;
+}
+}  // namespace
+
+extern "C" femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, const double *coords, double *disp) {
+  if (!box || n_points < 0 || (n_points > 0 && (!coords || !disp))) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  femx_status r = ffd_check_points(box);
+  if (r != FEMX_OK) return r;
+  const int *n = box->tot_points;
+  const long tot = (long)n[0] * n[1] * n[2];
+  std::vector<double> d(3 * tot);
+  if ((r = femx_ffd_control_points(box, nullptr, d.data())) != FEMX_OK) return r;
+  if (n_points == 0) return FEMX_OK;
+  FfdDev b;
+  ffd_setup(box, b);
   DevBuf<double> dc, dd, dout;
   if ((r = dc.upload(coords, 3L * n_points, nullptr)) != FEMX_OK || (r = dd.upload(d.data(), d.size(), nullptr)) != FEMX_OK ||
       (r = dout.alloc(3L * n_points)) != FEMX_OK)
@@ -160,5 +176,99 @@ extern "C" femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, con
   ffd_morph<<<grid_for(n_points, 256), 256>>>(n_points, dc.p, b, dd.p, dout.p);
   FEMX_HIP(hipGetLastError());
   FEMX_HIP(hipMemcpy(disp, dout.p, 3L * n_points * sizeof(double), hipMemcpyDeviceToHost));
+  return FEMX_OK;
+}
+
+// ------------------------------------------- morph without allocations
+// The control-point displacement lattice of femx_ffd_morph_noalloc: at most
+// 13^3 points x 3 doubles (52,728 bytes), one copy per GPU, held until the
+// process ends.  Host code never uses this name as an address: it uses the
+// per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[3 * FFD_MAX * FFD_MAX * FFD_MAX];
+
+namespace {
+constexpr int FFD_NOALLOC_MAX_DISP = 64;  // displaced control points carried in the kernel arguments
+constexpr int FFD_MAX_DEVICES = 64;
+
+struct FfdDisp {
+  int n;
+  int id[FFD_NOALLOC_MAX_DISP];
+  double v[FFD_NOALLOC_MAX_DISP][3];
+  double nS[3], nT[3], nU[3];  // normalised box axes
+};
+
+// Control-point displacements (femPointGrid :81-113) into the lattice, one
+// thread per control point: every entry starts at zero and adds the
+// displaced points that name it in list order, component by component, in
+// the order of femx_ffd_control_points (no contraction into FMAs, so the
+// lattice is bitwise the host one).
+__global__ __launch_bounds__(256) void ffd_lattice(int tot, FfdDisp D, double *lat) {
+#pragma clang fp contract(off)
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= tot) return;
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < D.n; q++) {
+    if (D.id[q] != id) continue;
+    for (int c = 0; c < 3; c++) d[c] += D.v[q][0] * D.nS[c];
+    for (int c = 0; c < 3; c++) d[c] += D.v[q][1] * D.nT[c];
+    for (int c = 0; c < 3; c++) d[c] += D.v[q][2] * D.nU[c];
+  }
+  for (int c = 0; c < 3; c++) lat[3L * id + c] = d[c];
+}
+
+std::mutex g_scratch_mu;
+double *g_scratch_dev[FFD_MAX_DEVICES] = {};
+
+// Device address of the lattice array on the current device, looked up
+// once per device.
+femx_status scratch_address(double **out) {
+  int dev = -1;
+  FEMX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= FFD_MAX_DEVICES) return set_error("device ordinal out of range"), FEMX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return FEMX_OK;
+}
+}  // namespace
+
+extern "C" femx_status femx_ffd_morph_noalloc(const femx_ffd_box *box, int n_points, const double *coords,
+                                              double *disp, void *stream) {
+  if (!box || n_points < 0 || (n_points > 0 && (!coords || !disp))) return set_error("bad argument"), FEMX_ERR_ARG;
+  femx_status r = ffd_check_points(box);
+  if (r != FEMX_OK) return r;
+  if (box->n_disp < 0 || box->n_disp > FFD_NOALLOC_MAX_DISP)
+    return set_error("femx_ffd_morph_noalloc: at most 64 displaced control points"), FEMX_ERR_ARG;
+  if (box->n_disp > 0 && (!box->disp_node || !box->disp_val)) return set_error("bad argument"), FEMX_ERR_ARG;
+  const int *n = box->tot_points;
+  const int tot = n[0] * n[1] * n[2];
+  FfdDisp D;
+  D.n = box->n_disp;
+  for (int q = 0; q < D.n; q++) {
+    D.id[q] = box->disp_node[q];
+    if (D.id[q] < 0 || D.id[q] >= tot) return set_error("displaced control point out of range"), FEMX_ERR_ARG;
+    for (int c = 0; c < 3; c++) D.v[q][c] = box->disp_val[3L * q + c];
+  }
+  const double *S = box->axis_s, *T = box->axis_t, *U = box->axis_u;
+  const double ls = std::sqrt(dot3(S, S)), lt = std::sqrt(dot3(T, T)), lu = std::sqrt(dot3(U, U));
+  for (int c = 0; c < 3; c++) {  // femUtils::Normalize3DVector leaves a zero vector as is
+    D.nS[c] = ls > 0.0 ? S[c] / ls : S[c];
+    D.nT[c] = lt > 0.0 ? T[c] / lt : T[c];
+    D.nU[c] = lu > 0.0 ? U[c] / lu : U[c];
+  }
+  if (n_points == 0) return FEMX_OK;
+  FfdDev b;
+  ffd_setup(box, b);
+  double *lat = nullptr;
+  if ((r = scratch_address(&lat)) != FEMX_OK) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  ffd_lattice<<<grid_for(tot, 256), 256, 0, st>>>(tot, D, lat);
+  ffd_morph<<<grid_for(n_points, 256), 256, 0, st>>>(n_points, coords, b, lat, disp);
+  FEMX_HIP(hipGetLastError());
   return FEMX_OK;
 }

@@ -118,6 +118,7 @@ SIGNATURES = [
                                           C.c_int, _ip, _ip, _dp]),
     ("femx_ffd_control_points", C.c_int, [C.POINTER(FFDBox), _dp, _dp]),
     ("femx_ffd_morph", C.c_int, [C.POINTER(FFDBox), C.c_int, _dp, _dp]),
+    ("femx_ffd_morph_noalloc", C.c_int, [C.POINTER(FFDBox), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("femx_supg_step_async", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_finish", C.c_int, [C.c_void_p, _ip]),
     ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
@@ -643,6 +644,16 @@ def ffd_morph(min_point, n, S, T, U, disp_node, disp_val, coords):
     out = np.zeros_like(c)
     _check(load().femx_ffd_morph(C.byref(b), len(c), _p(c), _p(out)))
     return out
+
+
+def ffd_morph_noalloc(min_point, n, S, T, U, disp_node, disp_val, n_points, coords_dev, disp_dev, stream=None):
+    """femx_ffd_morph_noalloc: the morph of n_points device-resident points
+    (coords_dev, disp_dev: device addresses of [n][3] doubles), enqueued on
+    `stream` (a hipStream_t address, None = the null stream) without any
+    device allocation; the caller synchronises the stream."""
+    b, keep = _ffd_box(min_point, n, S, T, U, disp_node, disp_val)
+    _check(load().femx_ffd_morph_noalloc(C.byref(b), int(n_points), C.c_void_p(coords_dev), C.c_void_p(disp_dev),
+                                         C.c_void_p(stream)))
 
 
 def ffd_control_points(min_point, n, S, T, U, disp_node, disp_val):

@@ -304,6 +304,19 @@ femx_status femx_ffd_control_points(const femx_ffd_box *box, double *coords, dou
  * B_i(s) B_j(t) B_k(u) d_ijk inside the box; points outside get their box
  * coordinates (s, t, u), as the reference stores them. */
 femx_status femx_ffd_morph(const femx_ffd_box *box, int n_points, const double *coords, double *disp);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): femx_ffd_morph of
+ * `n_points` DEVICE-resident points, `coords` [n][3] in and `disp` [n][3]
+ * out (device pointers), enqueued on `stream` (a hipStream_t; NULL = the
+ * null stream) without allocating, copying or synchronising.  The box and
+ * its displacement list are host data, at most 64 displaced control points
+ * (13 control points per axis, as femx_ffd_morph).  The control-point
+ * lattice goes to a fixed device array of 52,728 bytes per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU (other streams, other threads) race on it.  The result equals
+ * femx_ffd_morph's bit for bit. */
+femx_status femx_ffd_morph_noalloc(const femx_ffd_box *box, int n_points, const double *coords, double *disp,
+                                   void *stream);
 
 /* ------------------------------- implicit advection-diffusion (gen-alpha) */
 /* femIncompressibleSolver (src/femIncompressibleSolver.h:6-11,

@@ -72,24 +72,52 @@ def make_workload(n=96, jitter=0.15, seed=1):
     return m, bc, params, ic
 
 
+SETTLE_MAX_S = 4.0
+
+
+def settle(g, chunk=50, tol=0.01, min_s=0.5, max_s=None):
+    """Steps in chunks until two consecutive chunks take the same wall time
+    within tol (the GPU clocks and caches have settled; a short window
+    straight after start-up runs about 10% slow), for at least min_s and at
+    most max_s.  Returns (steps, seconds)."""
+    max_s = SETTLE_MAX_S if max_s is None else max_s
+    if max_s <= 0:
+        return 0, 0.0
+    min_s = min(min_s, max_s)
+    t_start = time.perf_counter()
+    prev, steps = None, 0
+    while True:
+        t0 = time.perf_counter()
+        g.step(chunk)
+        g.sync()
+        dt = time.perf_counter() - t0
+        steps += chunk
+        el = time.perf_counter() - t_start
+        if (prev is not None and abs(dt - prev) <= tol * prev and el >= min_s) or el >= max_s:
+            return steps, el
+        prev = dt
+
+
 def time_solver(dm, bc, params, ic, precision, steps, warmup):
     g = femx.VMSSolver(dm, bc, precision=precision, **params)
     g.set_state(ic)
     g.step(warmup)
     g.sync()
+    settled = settle(g)
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
     t1 = time.perf_counter()
-    # per-kernel HIP events on the solver's stream, separate pass
+    # per-kernel HIP events on the solver's stream, a second pass of at
+    # least 200 steps in the same settled state
     g.kernel_timing(True)
-    g.step(steps)
+    g.step(max(steps, 200))
     g.sync()
     kt = g.kernel_times()
     g.kernel_timing(False)
     sol, _, _ = g.get_state()
     finite = bool(np.isfinite(sol).all())
-    return (t1 - t0), kt, finite
+    return (t1 - t0), kt, finite, settled
 
 
 def cpu_baseline(m, bc, params, ic, budget_s=20.0):
@@ -362,11 +390,15 @@ def workload_c2(args):
     ic = meshgen.random_velocity(m, 1e-3, args.seed)[:, :3]
     E, N = m.n_elems, m.n_nodes
     res = {}
+    wanted = args.precisions.split(",")
     for name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64), ("mixed_f32sum", femx.FEMX_MIXED_F32SUM)):
+        if name not in wanted:
+            continue
         g = femx.SUPGSolver(dm, bc, precision=prec, **p)
         g.set_state(ic)
         g.step(args.warmup)
         g.sync()
+        settle(g)
         t0 = time.perf_counter()
         g.step(args.steps)
         g.sync()
@@ -398,11 +430,13 @@ def workload_c2(args):
                                 "VALU issue ~55% busy); the HBM fraction is reported for reference",
                         "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4))
                                     for k, v in kt.items()}},
-           other: {"value": round(E * args.steps / res[other][0], 1),
-                   "ms_per_step": round(res[other][0] / args.steps * 1e3, 4)},
-           "mixed_f32sum": {"value": round(E * args.steps / res["mixed_f32sum"][0], 1),
-                            "ms_per_step": round(res["mixed_f32sum"][0] / args.steps * 1e3, 4),
-                            "note": F32SUM_NOTE}}
+           }
+    for name in (other, "mixed_f32sum"):
+        if name in res:
+            out[name] = {"value": round(E * args.steps / res[name][0], 1),
+                         "ms_per_step": round(res[name][0] / args.steps * 1e3, 4)}
+    if "mixed_f32sum" in out:
+        out["mixed_f32sum"]["note"] = F32SUM_NOTE
     print(json.dumps(out), flush=True)
 
 
@@ -511,6 +545,81 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def workload_dropin(args):
+    """femVMSExplicitFluidSolver::solve as shipped: femx_explicit --vms
+    --mixed on a model.txt of the C3 workload (zero initial state, no saves
+    inside the window), against the C-ABI step on the same mesh.  The
+    drop-in prints one line per dual iteration (per-iteration norms and CFL,
+    from the device ring of the last 64 steps); the per-step rate comes from
+    the arrival times of those lines on its stdout pipe, between the 200th
+    and the last line, so the model read and set-up are not in the window."""
+    import subprocess
+    import tempfile
+    from femorph_amd import modelfile
+    from femorph_amd.build import DRIVER
+
+    m, bc, params, ic = make_workload(args.n)
+    E = m.n_elems
+    steps = max(args.steps, 1000)
+    d = tempfile.mkdtemp(prefix="femx_dropin_")
+    path = os.path.join(d, "model.txt")
+    t0 = time.perf_counter()
+    modelfile.write_model_txt(path, m, bc, time_step=params["dt"], total_steps=steps + 1, save_every=10 ** 9,
+                              vmsprops=(params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"]))
+    print(f"dropin: model.txt written in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    t_start = time.perf_counter()
+    pr = subprocess.Popen([DRIVER, "--vms", "--mixed", "model.txt"], cwd=d, stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE)
+    marks, lines, tail = [], 0, b""
+    while True:
+        chunk = pr.stdout.read1(1 << 16)
+        if not chunk:
+            break
+        data = tail + chunk
+        lines += data.count(b"\n")
+        tail = data[data.rfind(b"\n") + 1:]
+        marks.append((time.perf_counter(), lines))
+    rc = pr.wait()
+    wall = time.perf_counter() - t_start
+    err = pr.stderr.read().decode(errors="replace")
+    for f in os.listdir(d):
+        os.remove(os.path.join(d, f))
+    os.rmdir(d)
+    if rc != 0 or lines < steps:
+        raise RuntimeError(f"femx_explicit failed (rc {rc}, {lines} lines): {err[-2000:]}")
+    a = next(mk for mk in marks if mk[1] >= 200)
+    b = marks[-1]
+    dropin_ms = (b[0] - a[0]) / (b[1] - a[1]) * 1e3
+    print(f"dropin: {lines} lines, {dropin_ms:.4f} ms/step, whole run {wall:.1f} s", file=sys.stderr, flush=True)
+    # the C-ABI step on the same mesh: no diagnostics, and the drop-in's
+    # norms-only diagnostics, both on the handle's stream
+    dm = femx.DeviceMesh(m)
+    res = {}
+    for name, diag in (("c_abi", 0), ("c_abi_norms", 2)):
+        g = femx.VMSSolver(dm, bc, precision=femx.FEMX_MIXED, **params)
+        g.set_state(np.zeros((m.n_nodes, 4)))
+        g.set_diagnostics(diag)
+        g.step(args.warmup)
+        g.sync()
+        settle(g)
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.sync()
+        res[name] = (time.perf_counter() - t0) / steps * 1e3
+        del g
+    out = {"metric": "ms per VMS tet4 step through femVMSExplicitFluidSolver::solve (drop-in) vs the C ABI",
+           "value": round(dropin_ms, 4), "unit": "ms/step", "higher_is_better": False,
+           "dtype": "f32-element/f64-state", "data": "synthetic (seeded jittered Kuhn-split cube, channel BCs)",
+           "config": {"workload": f"drop-in solve() on VMS tet4 {args.n}^3x6 channel Re200 ({E} tets), "
+                                  f"{steps} steps, zero initial state", "elements": E, "steps": steps},
+           "element_steps_per_s": round(E / (dropin_ms * 1e-3), 1),
+           "c_abi_ms_per_step": round(res["c_abi"], 4),
+           "c_abi_with_norms_ms_per_step": round(res["c_abi_norms"], 4),
+           "dropin_over_c_abi": round(dropin_ms / res["c_abi"], 3),
+           "whole_run_s": round(wall, 2)}
+    print(json.dumps(out), flush=True)
+
+
 def workload_accuracy(args):
     """Error of the FP32-element / FP64-state path after `steps` steps:
     C1 against the FP64 oracle itself, C3 (VMS) and C2 (SUPG) against the
@@ -599,10 +708,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins", "hbm"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins", "hbm", "dropin"],
+                    default="c3",
                     help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request "
                          "stream; hbm: streaming-bandwidth probe")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precisions", default="mixed,fp64,mixed_f32sum",
+                    help="c3/c2: the precisions timed (the headline one must be among them)")
+    ap.add_argument("--settle-max", type=float, default=4.0,
+                    help="seconds of settling steps at most before a timed window (0: none, for profiler runs)")
     ap.add_argument("--ins-elem", choices=["tet4", "hexa8"], default="tet4", help="ins: element type")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
     ap.add_argument("--requests", type=int, default=120, help="requests per rate (serve)")
@@ -614,6 +728,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="serve: hardware queues of the process (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()
+    global SETTLE_MAX_S
+    SETTLE_MAX_S = args.settle_max
     if args.steps is None:
         args.steps = {"c3": 500, "c2": 300, "c4": 200, "accuracy": 1000}.get(args.workload, 50)
     if args.workload == "serve" and args.hw_queues:
@@ -647,16 +763,22 @@ def main():
     if args.workload == "hbm":
         workload_hbm(args)
         return
+    if args.workload == "dropin":
+        workload_dropin(args)
+        return
 
     m, bc, params, ic = make_workload(args.n)
     E, N = m.n_elems, m.n_nodes
     r = N / E
     dm = femx.DeviceMesh(m)
     results = {}
+    wanted = args.precisions.split(",")
     for prec_name, prec in (("mixed", femx.FEMX_MIXED), ("fp64", femx.FEMX_FP64),
                             ("mixed_f32sum", femx.FEMX_MIXED_F32SUM)):
-        secs, kt, finite = time_solver(dm, bc, params, ic, prec, args.steps, args.warmup)
-        results[prec_name] = dict(secs=secs, kt=kt, finite=finite)
+        if prec_name not in wanted:
+            continue
+        secs, kt, finite, settled = time_solver(dm, bc, params, ic, prec, args.steps, args.warmup)
+        results[prec_name] = dict(secs=secs, kt=kt, finite=finite, settled=settled)
     main_res = results[args.precision]
     mixed = args.precision == "mixed"
     ms_per_step = main_res["secs"] / args.steps * 1e3
@@ -670,7 +792,8 @@ def main():
     dom_bytes = dom["bytes_per_launch"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     step_bytes = algorithmic_bytes_per_element_step(r, mixed) * E
-    step_ms_events = sum(v["total_ms"] for v in kt.values()) / args.steps
+    event_steps = max(args.steps, 200)
+    step_ms_events = sum(v["total_ms"] for v in kt.values()) / event_steps
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.precision, args.n),
             "kernel": dom_name,
@@ -691,14 +814,19 @@ def main():
                    "elements": E, "nodes": N, "dual_iterations_per_step": 1,
                    "precision": args.precision},
         "roofline": roof,
-        other: {"value": round(E * args.steps / results[other]["secs"], 1),
-                "ms_per_step": round(results[other]["secs"] / args.steps * 1e3, 4)},
-        "mixed_f32sum": {"value": round(E * args.steps / results["mixed_f32sum"]["secs"], 1),
-                         "ms_per_step": round(results["mixed_f32sum"]["secs"] / args.steps * 1e3, 4),
-                         "note": F32SUM_NOTE},
         "finite_state": all(v["finite"] for v in results.values()),
         "event_step_ms": round(step_ms_events, 4),
+        "settle": {"steps": main_res["settled"][0], "seconds": round(main_res["settled"][1], 3),
+                   "rule": "after the warmup steps, 50-step chunks until two consecutive chunks agree within 1% "
+                           f"(at least 0.5 s, at most {SETTLE_MAX_S:g} s); the kernel events come from a second pass of "
+                           f"{event_steps} steps"},
     }
+    for name in (other, "mixed_f32sum"):
+        if name in results:
+            out[name] = {"value": round(E * args.steps / results[name]["secs"], 1),
+                         "ms_per_step": round(results[name]["secs"] / args.steps * 1e3, 4)}
+    if "mixed_f32sum" in out:
+        out["mixed_f32sum"]["note"] = F32SUM_NOTE
     ceil_gbs = stream_ceiling_gbs()
     roof["stream_ceiling"] = {"gbs": round(ceil_gbs, 1), "probe": CEILING_PROBE,
                               "achieved_frac": round(achieved / ceil_gbs, 4)}

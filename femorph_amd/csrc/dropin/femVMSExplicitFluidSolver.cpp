@@ -53,21 +53,40 @@ void femVMSExplicitFluidSolver::solve(femModel *model) {
   add_result(model, "pressure", frNode, 1, sol.data() + 3, N, 4);
   model->ExportToVTKLegacy("out_Step_0.vtk", false);
 
-  check(femx_vms_set_diagnostics(s.h, 1));
+  // The per-iteration norms stay on the device (a ring of 64 steps) and are
+  // printed in batches: at a save, when the ring is full, and at the end,
+  // so the steps in between run without host synchronisation.  Only the
+  // steps that save results compute the element fields.
   const int nd = model->tot_dual_steps + 1;
-  std::vector<double> norms(3 * nd), ru(3 * E), rp(E), su(3 * E), sp(E);
+  const int ring = 64;
+  std::vector<double> norms(3L * ring * nd), ru(3 * E), rp(E), su(3 * E), sp(E);
+  std::vector<double> times;
   double currentTime = 0.0;
   long saveCounter = 0;
+  int mode = 0;
+  long loop_first = 1;  // loopTime of the first step whose lines are pending
+  auto flush = [&]() {
+    const int n = (int)times.size();
+    if (n == 0) return;
+    check(femx_vms_get_norms(s.h, n, norms.data()));
+    for (int k = 0; k < n; k++)
+      for (int d = 0; d < nd; d++) {
+        const double *v = norms.data() + 3L * ((long)k * nd + d);
+        printf("%4d.%-4d %20.8f %20.3e %20.3e %10.3f\n", (int)(loop_first + k), d + 1, times[k], v[0], v[1], v[2]);
+      }
+    loop_first += n;
+    times.clear();
+  };
   for (int loopTime = 1; loopTime < model->totalSteps; loopTime++) {
     currentTime += model->timeStep;
-    check(femx_vms_step(s.h, 1));
     const bool save = saveCounter == model->saveEvery;
-    check(femx_vms_get_diagnostics(s.h, norms.data(), save ? ru.data() : nullptr, save ? rp.data() : nullptr,
-                                   save ? su.data() : nullptr, save ? sp.data() : nullptr));
-    for (int d = 0; d < nd; d++)
-      printf("%4d.%-4d %20.8f %20.3e %20.3e %10.3f\n", loopTime, d + 1, currentTime, norms[3 * d], norms[3 * d + 1],
-             norms[3 * d + 2]);
+    const int want = save ? 1 : 2;
+    if (want != mode) check(femx_vms_set_diagnostics(s.h, mode = want));
+    check(femx_vms_step(s.h, 1));
+    times.push_back(currentTime);
+    if (save || (int)times.size() == ring || loopTime + 1 == model->totalSteps) flush();
     if (save) {
+      check(femx_vms_get_diagnostics(s.h, nullptr, ru.data(), rp.data(), su.data(), sp.data()));
       saveCounter = 0;
       clear_results(model);
       check(femx_vms_get_state(s.h, sol.data(), nullptr, nullptr));

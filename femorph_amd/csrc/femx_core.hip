@@ -548,6 +548,55 @@ extern "C" femx_status femx_hbm_probe(int kind, long long bytes, int grid, int r
   return FEMX_OK;
 }
 
+// ------------------------------------------------- VALU issue-rate probe
+// Eight independent FMA chains per thread (x = x * a + b, which converges to
+// b / (1 - a) = 1 and never reaches a denormal), `iters` rounds: 16 flops per
+// lane per round in T.  The sum keeps every chain live.
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void probe_valu(T *out, int iters, T a, T b) {
+  T x[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) x[j] = (T)(threadIdx.x & 7) + (T)j * (T)0.125;
+  for (int i = 0; i < iters; i++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = fma(x[j], a, b);
+  }
+  T s = (T)0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) s += x[j];
+  if (s == (T)1234.5) out[blockIdx.x] = s;  // never true; keeps the chains
+}
+}  // namespace
+
+extern "C" femx_status femx_valu_probe(int kind, int grid, int iters, int reps, double *tflops) {
+  if (!tflops || kind < 0 || kind > 1 || grid < 1 || grid > (1 << 20) || iters < 1 || reps < 1)
+    return set_error("femx_valu_probe: arguments"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  DevBuf<double> out;
+  femx_status r;
+  if ((r = out.alloc(grid)) != FEMX_OK) return r;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto launch = [&]() {
+    if (kind == 0) probe_valu<float><<<grid, 256>>>(reinterpret_cast<float *>(out.p), iters, 0.999f, 0.001f);
+    else probe_valu<double><<<grid, 256>>>(out.p, iters, 0.999, 0.001);
+  };
+  float ms = 0.f;
+  bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  if (ok) {
+    launch();  // warm-up
+    ok = hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int i = 0; ok && i < reps; i++) launch();
+    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, nullptr) == hipSuccess &&
+         hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!ok || ms <= 0.f) return set_error("femx_valu_probe: HIP launch or timing failed"), FEMX_ERR_HIP;
+  *tflops = 16.0 * 256.0 * grid * (double)iters * reps / (ms * 1e-3) / 1e12;
+  return FEMX_OK;
+}
+
 femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
                              const std::vector<int> *elem_group) {
   m.N = (int)N;

@@ -103,6 +103,7 @@ SIGNATURES = [
     ("femx_vms_get_cfl", C.c_int, [C.c_void_p, _dp, _dp]),
     ("femx_vms_set_diagnostics", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_vms_get_diagnostics", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    ("femx_vms_get_norms", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("femx_vms_sync", C.c_int, [C.c_void_p]),
     ("femx_vms_stream", C.c_void_p, [C.c_void_p]),
     ("femx_vms_batch_create", C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(BCDesc), C.POINTER(VMSParams),
@@ -155,6 +156,7 @@ SIGNATURES = [
     ("femx_ins_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_ins_kernel_times", C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int]),
     ("femx_hbm_probe", C.c_int, [C.c_int, C.c_longlong, C.c_int, C.c_int, _dp]),
+    ("femx_valu_probe", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
 ]
 
 _LIB = None
@@ -225,6 +227,14 @@ def hbm_probe(kind: str, nbytes: int = 1 << 30, grid: int = 4096, reps: int = 20
     kinds use non-temporal loads and stores."""
     out = C.c_double(0.0)
     _check(load().femx_hbm_probe(HBM_PROBE_KINDS[kind], int(nbytes), int(grid), int(reps), C.byref(out)))
+    return out.value
+
+
+def valu_probe(kind: str, grid: int = 8192, iters: int = 4096, reps: int = 10) -> float:
+    """TFLOP/s of eight independent FMA chains per thread (femx_valu_probe):
+    kind "f32" (v_fma_f32, no packed FP32) or "f64" (v_fma_f64)."""
+    out = C.c_double(0.0)
+    _check(load().femx_valu_probe({"f32": 0, "f64": 1}[kind], int(grid), int(iters), int(reps), C.byref(out)))
     return out.value
 
 
@@ -395,7 +405,15 @@ class VMSSolver:
         return float(a[0]), float(b[0])
 
     def set_diagnostics(self, enable=True):
+        """0/False off, 1/True norms and element fields, 2 norms only."""
         _check(load().femx_vms_set_diagnostics(self._h, int(enable)))
+
+    def get_norms(self, nsteps):
+        """Per-iteration norms of the last nsteps (<= 64) steps run with
+        diagnostics, oldest first: [nsteps, tot_dual_steps + 1, 3]."""
+        out = np.zeros((nsteps, self.tot_dual_steps + 1, 3))
+        _check(load().femx_vms_get_norms(self._h, int(nsteps), _p(out)))
+        return out
 
     def get_diagnostics(self):
         """Element diagnostics of the last step: dict with iter_norms

@@ -141,21 +141,30 @@ femx_status femx_vms_get_mass(femx_vms *s, double *nd_mass);
  * the CFL number printed by the reference (:1202-1215). */
 femx_status femx_vms_get_cfl(femx_vms *s, double *max_u, double *cfl);
 /* Element diagnostics of every dual iteration (off by default: they cost
- * an extra pass over E x 16 doubles per iteration and 200 B per element of
- * device memory).  They replace eval_centroid_residuals /
+ * an extra pass over E x 16 doubles per iteration and up to 200 B per
+ * element of device memory).  They replace eval_centroid_residuals /
  * eval_centroid_subgrid / eval_residual_norms
  * (src/femVMSExplicitFluidSolver.cpp:699-724,774-793,1087-1093,1171-1173)
  * with the reference's semantics: the centroid residual of element e
  * averages the global residual as assembled up to element e (ascending
- * element order). */
+ * element order).  enable: 0 off; 1 the per-iteration norms and the element
+ * fields of femx_vms_get_diagnostics; 2 the norms only (no field writes).
+ * The norms of the last 64 steps run with diagnostics stay on the device
+ * (femx_vms_get_norms), so stepping needs no host synchronisation.  Not
+ * offered on batches. */
 femx_status femx_vms_set_diagnostics(femx_vms *s, int enable);
 /* Diagnostics of the last step: iter_norms [(tot_dual_steps+1)][3] = max
  * centroid |res_u|, max centroid |res_p|, CFL of each dual iteration (the
  * values the reference prints at :1212); el_u_res [E][3], el_p_res [E],
  * el_u_sdus [E][3], el_p_sdus [E] of the last dual iteration (the res_v,
- * res_p, subgrid_u, subgrid_p results, :1269-1325).  Any pointer may be NULL. */
+ * res_p, subgrid_u, subgrid_p results, :1269-1325; they need mode 1 in that
+ * step).  Any pointer may be NULL. */
 femx_status femx_vms_get_diagnostics(femx_vms *s, double *iter_norms, double *el_u_res, double *el_p_res,
                                      double *el_u_sdus, double *el_p_sdus);
+/* The per-iteration norms (as iter_norms above) of the last nsteps steps run
+ * with diagnostics, oldest first: norms [nsteps][tot_dual_steps+1][3],
+ * nsteps <= 64.  One host synchronisation for all of them. */
+femx_status femx_vms_get_norms(femx_vms *s, int nsteps, double *norms);
 femx_status femx_vms_sync(femx_vms *s);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 void *femx_vms_stream(femx_vms *s);
@@ -401,6 +410,13 @@ int femx_ins_kernel_times(femx_ins *s, femx_kernel_time *out, int max_entries);
  * `grid` workgroups of 256 threads, `reps` timed launches after one warm-up.
  * Allocates two buffers of `bytes` on the current device and frees them. */
 femx_status femx_hbm_probe(int kind, long long bytes, int grid, int reps, double *tb_per_s);
+/* VALU issue-rate probe (no counterpart in the reference: it measures the
+ * practical vector FMA ceiling the SUPG kernels' flop rates are read
+ * against).  kind 0: FP32 v_fma_f32 (this build emits no packed FP32), 1:
+ * FP64 v_fma_f64; eight independent FMA chains per thread, `iters` rounds,
+ * `grid` workgroups of 256 threads, `reps` timed launches after one warm-up.
+ * *tflops counts an FMA as two flops. */
+femx_status femx_valu_probe(int kind, int grid, int iters, int reps, double *tflops);
 
 #ifdef __cplusplus
 }

@@ -202,6 +202,33 @@ def test_element_diagnostics(gpu, tot_dual):
     assert rel_l2(sol, o.get("sol_n")) <= TOL64
 
 
+def test_norms_ring_without_fields(gpu):
+    """Diagnostics mode 2 (norms only): the per-iteration norms of 70 steps
+    stay on the device and femx_vms_get_norms returns the last 64, oldest
+    first, equal to the oracle's printed values of each step; a mode 1 step
+    after them still yields the element fields."""
+    m, bc, params, ic = _case("channel", 4, 0.15, seed=6)
+    o, g, _ = _pair(gpu, m, bc, params, ic, tot_dual_steps=1)
+    g.set_diagnostics(2)
+    ref = []
+    for step in range(70):
+        o.step()
+        g.step()
+        ref.append(o.get("scalars")[2:5].copy())
+    got = g.get_norms(64)
+    assert got.shape == (64, 2, 3)
+    np.testing.assert_allclose(got[:, -1, :], np.array(ref[6:]), rtol=1e-11, atol=0)
+    with pytest.raises(gpu.FemxError, match="at most the last 64"):
+        g.get_norms(65)
+    g.set_diagnostics(1)
+    o.step()
+    g.step()
+    d = g.get_diagnostics()
+    assert rel_l2(d["res_v"], o.get("el_u_res")) <= 1e-11
+    np.testing.assert_allclose(d["iter_norms"][-1], o.get("scalars")[2:5], rtol=1e-11, atol=0)
+    np.testing.assert_allclose(g.get_norms(1)[0], d["iter_norms"], rtol=0, atol=0)
+
+
 @pytest.mark.parametrize("precision,tol", [(0, TOL64), (1, 1e-5)])
 def test_high_valence_node(gpu, precision, tol):
     """A node with 512 incident tets: the block builder splits its corners
@@ -259,6 +286,6 @@ def test_duplicate_velocity_node_rejected(gpu):
     with pytest.raises(gpu.FemxError, match="listed twice"):
         gpu.VMSSolver(dm, dup, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"])
     with pytest.raises(gpu.FemxError, match="listed twice"):
-        gpu.SUPGSolver(dm, dup, rho=1.0, mu=0.01, alpha=1.0, cI=4.0, dt=params["dt"])
+        gpu.SUPGSolver(dm, dup, rho=1.0, mu=0.01, alpha=1.0, cI=4.0, dt=params["dt"], exopts=(0, 0, 1, 1, 1, 1))
     # the unmodified tables still build
     gpu.VMSSolver(dm, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"])

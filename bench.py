@@ -51,6 +51,25 @@ def profiled_traffic(precision: str, n: int):
     k = t["kernels"].get(PROFILED_KERNEL[precision])
     return k and k["traffic_bytes"]
 METRIC = "element-steps/sec (VMS tet4) + % HBM roofline; p50/p99 solve latency"
+FLOPS_FILE = os.path.join(ROOT, "profiles", "supg_block_c2_flops.json")
+# vector FMA peak of one MI355X without packed FP32 instructions (this build
+# emits none): FP32 then issues at the FP64 rate, 78.6 TFLOP/s for both
+VALU_PEAK_TFLOPS = 78.6
+
+
+def profiled_flops(precision: str, n: int):
+    """Flops per launch of the SUPG block kernel at the C2 size, from the
+    committed rocprofv3 SQ_INSTS_VALU_* counter runs, or None."""
+    try:
+        t = json.load(open(FLOPS_FILE))
+    except OSError:
+        return None
+    if n != 128:
+        return None
+    for k in t["kernels"].values():
+        if k["precision"] == precision and k["timing_kind"] == "supg_block_press":
+            return k["flops"]
+    return None
 
 
 def algorithmic_bytes_per_element_step(r: float, mixed: bool) -> float:
@@ -415,6 +434,9 @@ def workload_c2(args):
     dom = kt[dom_name]
     dom_ms = dom["total_ms"] / max(dom["launches"], 1)
     achieved = dom["bytes_per_launch"] / (dom_ms * 1e-3) / 1e9
+    flops = profiled_flops(args.precision, n)
+    tflops = flops / (dom_ms * 1e-3) / 1e12 if flops else None
+    valu_probe = femx.valu_probe("f32" if args.precision == "mixed" else "f64", 32768, 2048, 5)
     other = "fp64" if args.precision == "mixed" else "mixed"
     out = {"metric": "element-steps/sec (SUPG hexa8)", "value": round(E * args.steps / secs, 1),
            "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -423,11 +445,16 @@ def workload_c2(args):
            "data": "synthetic (seeded jittered cube)",
            "config": {"workload": f"C2: SUPG hexa8 {n}^3 jitter0.2 lid cavity Re400, AB2+Bhat+K1-K4 "
                                   f"({E} hexa, {N} nodes)", "elements": E, "nodes": N},
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
-                        "kernel_ms": round(dom_ms, 4), "kernel_bytes": dom["bytes_per_launch"],
-                        "note": "compute-heavy kernel (about 6.4k VALU instructions per element-step, "
-                                "VALU issue ~55% busy); the HBM fraction is reported for reference",
+           "roofline": {"bound": "valu", "achieved": round(tflops, 2) if tflops else None,
+                        "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tflops / VALU_PEAK_TFLOPS, 4) if tflops else None, "traffic": None,
+                        "kernel": dom_name, "kernel_ms": round(dom_ms, 4), "flops_per_launch": flops,
+                        "flops_source": "profiles/supg_block_c2_flops.json (rocprofv3 SQ_INSTS_VALU_* counters)",
+                        "peak_note": "vector FMA peak without packed FP32 (FP32 then issues at the FP64 rate)",
+                        "valu_probe_tflops": round(valu_probe, 2),
+                        "valu_probe_frac": round(tflops / valu_probe, 4) if tflops else None,
+                        "hbm": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel_bytes": dom["bytes_per_launch"]},
                         "kernels": {k: dict(launches=v["launches"], mean_ms=round(v["total_ms"] / max(v["launches"], 1), 4))
                                     for k, v in kt.items()}},
            }

@@ -6,6 +6,7 @@
 #include <string>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "femx_common.h"
 
@@ -102,6 +103,58 @@ static inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
   return v;
 }
 
+namespace {
+// f(begin, end) over [0, n) in contiguous ranges on up to 8 host threads.
+// Every range's work is independent of the others, so the results do not
+// depend on the thread count.
+template <class F>
+void parallel_ranges(long n, const F &f) {
+  const long T = std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  if (n < (1L << 16) || T == 1) {
+    f(0L, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (long t = 0; t < T; t++) th.emplace_back([&f, n, t, T] { f(n * t / T, n * (t + 1) / T); });
+  for (auto &x : th) x.join();
+}
+
+// Sorts idx by (key, idx): sorted runs on up to 8 host threads, then
+// pairwise merges.  (key, idx) pairs are unique, so the order does not
+// depend on the thread count.
+void sort_by_key(std::vector<uint64_t> &key, std::vector<int> &idx) {
+  const long n = (long)key.size();
+  struct KI {
+    uint64_t k;
+    int i;
+    bool operator<(const KI &o) const { return k != o.k ? k < o.k : i < o.i; }
+  };
+  std::vector<KI> a(n), t(n);
+  for (long x = 0; x < n; x++) a[x] = {key[x], idx[x]};
+  const long T = n < (1L << 16) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  std::vector<long> cut(T + 1);
+  for (long r = 0; r <= T; r++) cut[r] = n * r / T;
+  {
+    std::vector<std::thread> th;
+    for (long r = 0; r < T; r++) th.emplace_back([&, r] { std::sort(a.begin() + cut[r], a.begin() + cut[r + 1]); });
+    for (auto &x : th) x.join();
+  }
+  for (long w = 1; w < T; w *= 2) {
+    std::vector<std::thread> th;
+    for (long r = 0; r < T; r += 2 * w) {
+      const long l = cut[r], m = cut[std::min(r + w, T)], h = cut[std::min(r + 2 * w, T)];
+      th.emplace_back([&, l, m, h] { std::merge(a.begin() + l, a.begin() + m, a.begin() + m, a.begin() + h, t.begin() + l); });
+    }
+    for (auto &x : th) x.join();
+    a.swap(t);
+  }
+  for (long x = 0; x < n; x++) {
+    key[x] = a[x].k;
+    idx[x] = a[x].i;
+  }
+}
+}  // namespace
+
 void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
   Blocks &b = m.blk;
   const long E = m.E, npe = m.npe;
@@ -110,43 +163,61 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   // of meshes: blocks never mix meshes, each mesh gets its own bounding box)
   const int ng = elem_group ? 1 + *std::max_element(elem_group->begin(), elem_group->end()) : 1;
   std::vector<double> lo(3 * ng, 1e300), hi(3 * ng, -1e300);
-  for (long e = 0; e < E; e++) {
-    const int gi = elem_group ? (*elem_group)[e] : 0;
-    for (int a = 0; a < npe; a++)
+  std::mutex box_mu;
+  parallel_ranges(E, [&](long e0, long e1) {
+    std::vector<double> l(3 * ng, 1e300), h(3 * ng, -1e300);
+    for (long e = e0; e < e1; e++) {
+      const int gi = elem_group ? (*elem_group)[e] : 0;
+      for (int a = 0; a < npe; a++)
+        for (int k = 0; k < 3; k++) {
+          const double x = m.h_coords[3L * m.h_conn[e * npe + a] + k];
+          l[3 * gi + k] = std::min(l[3 * gi + k], x);
+          h[3 * gi + k] = std::max(h[3 * gi + k], x);
+        }
+    }
+    std::lock_guard<std::mutex> lk(box_mu);
+    for (int i = 0; i < 3 * ng; i++) {
+      lo[i] = std::min(lo[i], l[i]);
+      hi[i] = std::max(hi[i], h[i]);
+    }
+  });
+  std::vector<uint64_t> code(E);
+  std::vector<int> order(E);
+  parallel_ranges(E, [&](long e0, long e1) {
+    for (long e = e0; e < e1; e++) {
+      const int gi = elem_group ? (*elem_group)[e] : 0;
+      double c[3] = {0, 0, 0};
+      for (int a = 0; a < npe; a++)
+        for (int k = 0; k < 3; k++) c[k] += m.h_coords[3L * m.h_conn[e * npe + a] + k];
+      uint64_t cd = 0;
       for (int k = 0; k < 3; k++) {
-        const double x = m.h_coords[3L * m.h_conn[e * npe + a] + k];
-        lo[3 * gi + k] = std::min(lo[3 * gi + k], x);
-        hi[3 * gi + k] = std::max(hi[3 * gi + k], x);
+        double span = hi[3 * gi + k] - lo[3 * gi + k];
+        double f = span > 0 ? (c[k] / npe - lo[3 * gi + k]) / span : 0.0;
+        uint64_t q = (uint64_t)std::min(std::max(f, 0.0) * 2097151.0, 2097151.0);
+        cd |= spread3(q) << k;
       }
-  }
-  struct Key {
-    int group;
-    uint64_t code;
-    int e;
-    bool operator<(const Key &o) const {
-      return group != o.group ? group < o.group : (code != o.code ? code < o.code : e < o.e);
+      code[e] = cd;
+      order[e] = (int)e;
     }
-  };
-  std::vector<Key> key(E);
-  for (long e = 0; e < E; e++) {
-    const int gi = elem_group ? (*elem_group)[e] : 0;
-    double c[3] = {0, 0, 0};
-    for (int a = 0; a < npe; a++)
-      for (int k = 0; k < 3; k++) c[k] += m.h_coords[3L * m.h_conn[e * npe + a] + k];
-    uint64_t code = 0;
-    for (int k = 0; k < 3; k++) {
-      double span = hi[3 * gi + k] - lo[3 * gi + k];
-      double f = span > 0 ? (c[k] / npe - lo[3 * gi + k]) / span : 0.0;
-      uint64_t q = (uint64_t)std::min(std::max(f, 0.0) * 2097151.0, 2097151.0);
-      code |= spread3(q) << k;
-    }
-    key[e] = {gi, code, (int)e};
+  });
+  // by (group, code, element): by (code, element), then stable by group
+  sort_by_key(code, order);
+  code = std::vector<uint64_t>();
+  if (elem_group) {
+    std::vector<long> cnt(ng + 1, 0);
+    for (long i = 0; i < E; i++) cnt[(*elem_group)[order[i]] + 1]++;
+    for (int g = 0; g < ng; g++) cnt[g + 1] += cnt[g];
+    std::vector<int> o2(E);
+    for (long i = 0; i < E; i++) o2[cnt[(*elem_group)[order[i]]]++] = order[i];
+    order.swap(o2);
   }
-  std::sort(key.begin(), key.end());
-  b.perm.resize(E);
-  for (long i = 0; i < E; i++) b.perm[i] = key[i].e;
-  key.clear();
-  key.shrink_to_fit();
+  b.perm.swap(order);
+  // the connectivity in block order, read sequentially below
+  std::vector<int> sconn(E * npe);
+  parallel_ranges(E, [&](long i0, long i1) {
+    for (long i = i0; i < i1; i++)
+      for (int a = 0; a < npe; a++) sconn[i * npe + a] = m.h_conn[(long)b.perm[i] * npe + a];
+  });
   // greedy blocks: <= B elements, <= 256 local nodes, and at most
   // MAX_NODE_CORNERS corners of one node (so a corner's rank among its
   // node's corners in the block fits 7 bits)
@@ -162,10 +233,8 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   auto close_block = [&](long e_end) {
     std::sort(cur_nodes.begin(), cur_nodes.end());
     for (size_t j = 0; j < cur_nodes.size(); j++) local_id[cur_nodes[j]] = (int)j;
-    for (long i = e_begin; i < e_end; i++) {
-      const int eo = b.perm[i];
-      for (int a = 0; a < npe; a++) b.lconn[i * npe + a] = (uint8_t)local_id[m.h_conn[(long)eo * npe + a]];
-    }
+    for (long i = e_begin; i < e_end; i++)
+      for (int a = 0; a < npe; a++) b.lconn[i * npe + a] = (uint8_t)local_id[sconn[i * npe + a]];
     b.loc_node.insert(b.loc_node.end(), cur_nodes.begin(), cur_nodes.end());
     b.max_local = std::max(b.max_local, (int)cur_nodes.size());
     b.elem_off.push_back((int)e_end);
@@ -178,7 +247,7 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
     const int eo = b.perm[i];
     int fresh = 0, most = 0;
     for (int a = 0; a < npe; a++) {
-      const int n = m.h_conn[(long)eo * npe + a];
+      const int n = sconn[i * npe + a];
       fresh += stamp[n] != blk;
       if (stamp[n] == blk) most = std::max(most, corners[n]);
     }
@@ -187,7 +256,7 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
                         most + npe > MAX_NODE_CORNERS))
       close_block(i);
     for (int a = 0; a < npe; a++) {
-      const int n = m.h_conn[(long)eo * npe + a];
+      const int n = sconn[i * npe + a];
       if (stamp[n] != blk) {
         stamp[n] = blk;
         corners[n] = 0;

@@ -18,6 +18,14 @@
 //                 request, reads finished states back and frees their
 //                 handles and budget.
 //
+//   union thread  continuous batching of the VMS requests in the mixed
+//                 precision without dual iterations: after its first step on
+//                 its own stream a request joins a union of handles
+//                 (femx_vms_union_*), which advances all of its members by
+//                 one launch of each phase per step in chunks of `chunk`
+//                 steps, two chunks in flight; members join between chunks
+//                 and leave when their own step counts are done.
+//
 // Each request's arithmetic is that of its own handle, so a result does not
 // depend on what else is being served (bitwise).
 #include <hip/hip_runtime.h>
@@ -72,6 +80,7 @@ struct Request {
   int done = 0, issued = 0;
   femx_request_stats stats{};
   std::vector<double> result;
+  bool joins_union = false;  // runs in the union of the VMS handles
 
   femx_bc_desc bc() const {
     femx_bc_desc b;
@@ -121,7 +130,7 @@ struct femx_server {
   int device = 0;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<std::shared_ptr<Request>> pending, ready;
+  std::deque<std::shared_ptr<Request>> pending, ready, union_ready;
   std::vector<std::vector<std::shared_ptr<Request>>> active;  // per GPU thread
   std::map<long long, std::shared_ptr<Request>> finished;
   std::vector<femx_request_stats> unpolled;
@@ -131,11 +140,43 @@ struct femx_server {
   bool stop = false;
   std::string first_error;
   std::vector<std::thread> preps, gpus;
+  std::thread uni;
 
   void prep_loop();
   void gpu_loop(int t);
+  void union_loop();
   void fail(const std::shared_ptr<Request> &r, femx_status st);
+  void finish(const std::shared_ptr<Request> &r, femx_status st);
 };
+
+// read the final state back, free the handles and the budget (the work of
+// the request has completed)
+void femx_server::finish(const std::shared_ptr<Request> &r, femx_status st) {
+  const femx_status end_st = st;
+  if (st == FEMX_OK || st == FEMX_EXIT_CONDITION) {
+    if (r->kind == FEMX_REQ_VMS) {
+      r->result.resize(4L * r->N);
+      st = femx_vms_get_state(r->vh, r->result.data(), nullptr, nullptr);
+    } else {
+      r->result.resize(3L * r->N);
+      st = femx_supg_get_state(r->sh, r->result.data(), nullptr, nullptr, nullptr, nullptr);
+    }
+    if (st == FEMX_OK) st = end_st;
+  }
+  r->release();
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    r->stats.status = st;
+    r->stats.steps_done = r->done;
+    r->stats.t_done = now_s();
+    if (st != FEMX_OK && st != FEMX_EXIT_CONDITION && first_error.empty()) first_error = femx_last_error();
+    reserved -= r->bytes;
+    n_resident--;
+    finished[r->id] = r;
+    unpolled.push_back(r->stats);
+  }
+  cv.notify_all();
+}
 
 void femx_server::fail(const std::shared_ptr<Request> &r, femx_status st) {
   r->release();
@@ -180,6 +221,12 @@ void femx_server::prep_loop() {
           st = femx_vms_set_state(r->vh, zero.data(), nullptr);
           r->st = (hipStream_t)femx_vms_stream(r->vh);
         }
+        // a union member runs its first step on its own stream (the union
+        // joins after it; the first step has no projection to apply)
+        if (st == FEMX_OK && r->joins_union) {
+          st = femx_vms_step(r->vh, 1);
+          r->issued = r->done = 1;
+        }
       } else {
         st = femx_supg_create(r->mesh, &bc, &r->supg, &r->sh);
         if (st == FEMX_OK) {
@@ -201,10 +248,110 @@ void femx_server::prep_loop() {
     {
       std::lock_guard<std::mutex> lk(mu);
       r->stats.t_start = now_s();
-      ready.push_back(r);
+      (r->joins_union ? union_ready : ready).push_back(r);
     }
     cv.notify_all();
   }
+}
+
+void femx_server::union_loop() {
+  (void)hipSetDevice(device);
+  femx_vms_union *u = nullptr;
+  if (femx_vms_union_create(&u) != FEMX_OK) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (first_error.empty()) first_error = femx_last_error();
+    return;
+  }
+  hipStream_t ust = (hipStream_t)femx_vms_union_stream(u);
+  std::vector<std::shared_ptr<Request>> members;
+  // enqueued chunks, oldest first: the event after the chunk and the members
+  // whose steps end in it (removed from the union right after enqueueing it)
+  struct Chunk {
+    hipEvent_t ev;
+    std::vector<std::shared_ptr<Request>> done;
+  };
+  std::deque<Chunk> inflight;
+  std::vector<hipEvent_t> spare;
+  for (;;) {
+    std::vector<std::shared_ptr<Request>> fresh;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait_for(lk, std::chrono::microseconds(members.empty() && inflight.empty() ? 2000 : 50),
+                  [&] { return stop || !union_ready.empty() || !members.empty() || !inflight.empty(); });
+      if (stop) break;
+      while (!union_ready.empty()) {
+        fresh.push_back(union_ready.front());
+        union_ready.pop_front();
+      }
+    }
+    bool changed = false;
+    for (auto &r : fresh) {
+      const femx_status st = femx_vms_union_add(u, r->vh, r->steps - 1);
+      if (st != FEMX_OK) {
+        fail(r, st);
+        continue;
+      }
+      members.push_back(r);
+      changed = true;
+    }
+    // retire the members of completed chunks, oldest first
+    while (!inflight.empty() && hipEventQuery(inflight.front().ev) != hipErrorNotReady) {
+      for (auto &r : inflight.front().done) {
+        r->done = r->steps;
+        finish(r, FEMX_OK);
+      }
+      spare.push_back(inflight.front().ev);
+      inflight.pop_front();
+      changed = true;
+    }
+    // keep two chunks in flight while members have steps left
+    if (!members.empty() && inflight.size() < 2) {
+      femx_status st = femx_vms_union_step(u, chunk);
+      Chunk c{nullptr, {}};
+      if (st == FEMX_OK) {
+        if (!spare.empty()) {
+          c.ev = spare.back();
+          spare.pop_back();
+        } else if (hipEventCreateWithFlags(&c.ev, hipEventDisableTiming) != hipSuccess) {
+          st = FEMX_ERR_HIP;
+        }
+      }
+      if (st == FEMX_OK && hipEventRecord(c.ev, ust) != hipSuccess) st = FEMX_ERR_HIP;
+      if (st != FEMX_OK) {
+        for (auto &r : members) {
+          (void)femx_vms_union_remove(u, r->vh);
+          fail(r, st);
+        }
+        members.clear();
+      } else {
+        // members whose steps end in this chunk leave the union now; their
+        // streams wait for it, and they are read back once it has completed
+        for (size_t i = 0; i < members.size();) {
+          if (femx_vms_union_left(u, members[i]->vh) == 0) {
+            (void)femx_vms_union_remove(u, members[i]->vh);
+            c.done.push_back(members[i]);
+            members.erase(members.begin() + i);
+          } else {
+            i++;
+          }
+        }
+        inflight.push_back(std::move(c));
+      }
+      changed = true;
+    }
+    if (!changed) std::this_thread::yield();
+  }
+  (void)hipStreamSynchronize(ust);
+  for (auto &c : inflight) {
+    for (auto &r : c.done) r->release();
+    (void)hipEventDestroy(c.ev);
+  }
+  for (auto &r : members) {
+    (void)femx_vms_union_remove(u, r->vh);
+    r->release();
+  }
+  for (auto e : spare) (void)hipEventDestroy(e);
+  femx_vms_union_destroy(u);
 }
 
 void femx_server::gpu_loop(int t) {
@@ -265,31 +412,7 @@ void femx_server::gpu_loop(int t) {
         continue;
       }
       if (over && st == FEMX_OK) st = FEMX_ERR_HIP;
-      // finished (or failed): read the final state back, free the handles and the budget
-      const femx_status end_st = st;
-      if (st == FEMX_OK || st == FEMX_EXIT_CONDITION) {
-        if (r->kind == FEMX_REQ_VMS) {
-          r->result.resize(4L * r->N);
-          st = femx_vms_get_state(r->vh, r->result.data(), nullptr, nullptr);
-        } else {
-          r->result.resize(3L * r->N);
-          st = femx_supg_get_state(r->sh, r->result.data(), nullptr, nullptr, nullptr, nullptr);
-        }
-        if (st == FEMX_OK) st = end_st;
-      }
-      r->release();
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        r->stats.status = st;
-        r->stats.steps_done = r->done;
-        r->stats.t_done = now_s();
-        if (st != FEMX_OK && st != FEMX_EXIT_CONDITION && first_error.empty()) first_error = femx_last_error();
-        reserved -= r->bytes;
-        n_resident--;
-        finished[r->id] = r;
-        unpolled.push_back(r->stats);
-      }
-      cv.notify_all();
+      finish(r, st);
       active.erase(active.begin() + i);
       changed = true;
     }
@@ -311,6 +434,7 @@ extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_activ
   s->active.resize(gpu_threads);
   for (int i = 0; i < prep_threads; i++) s->preps.emplace_back([s] { s->prep_loop(); });
   for (int i = 0; i < gpu_threads; i++) s->gpus.emplace_back([s, i] { s->gpu_loop(i); });
+  s->uni = std::thread([s] { s->union_loop(); });
   *out = s;
   return FEMX_OK;
 }
@@ -344,6 +468,8 @@ extern "C" femx_status femx_server_submit(femx_server *s, const femx_request *q,
   r->pres_val.assign(b.pres_val, b.pres_val + b.n_pres);
   r->vms = q->vms;
   r->supg = q->supg;
+  r->joins_union = q->kind == FEMX_REQ_VMS && q->vms.precision == FEMX_MIXED && q->vms.tot_dual_steps == 0 &&
+                   q->steps >= 2;
   const bool vms = q->kind == FEMX_REQ_VMS;
   r->bytes = footprint(d.n_nodes, d.n_elems, d.npe, vms,
                        f32_elem(vms ? q->vms.precision : q->supg.precision));
@@ -389,7 +515,7 @@ extern "C" femx_status femx_server_drain(femx_server *s, double timeout_s) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
   const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
-    return s->pending.empty() && s->ready.empty() && s->n_resident == 0;
+    return s->pending.empty() && s->ready.empty() && s->union_ready.empty() && s->n_resident == 0;
   });
   if (!ok) return set_error("drain timed out"), FEMX_ERR_ARG;
   if (!s->first_error.empty()) return set_error(s->first_error), FEMX_ERR_HIP;
@@ -418,12 +544,14 @@ extern "C" femx_status femx_server_destroy(femx_server *s) {
   s->cv.notify_all();
   for (auto &t : s->preps) t.join();
   for (auto &t : s->gpus) t.join();
+  if (s->uni.joinable()) s->uni.join();
   for (auto &list : s->active)
     for (auto &r : list) {
       if (r->st) (void)hipStreamSynchronize(r->st);
       r->release();
     }
   for (auto &r : s->ready) r->release();
+  for (auto &r : s->union_ready) r->release();
   delete s;
   return FEMX_OK;
 }

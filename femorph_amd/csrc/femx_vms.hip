@@ -42,6 +42,14 @@
 
 #include "femx_common.h"
 
+// Multiply-adds are fused only where the source writes one expression
+// a * b + c (not across statements, as the compiler's default allows): the
+// single-handle kernels and the union kernels inline the same block and
+// fix-up code into different kernels, and with the fusion fixed by the
+// source both perform the same roundings, so a union member's result is bit
+// for bit that of its handle run on its own.
+#pragma clang fp contract(on)
+
 namespace femx {
 namespace {
 
@@ -141,6 +149,15 @@ __device__ __forceinline__ void tet_geometry_t(const X_t X[4][3], T DN[3][4], T 
 
 struct VmsConst {
   double nu, inv_eps, c1, c2, inv_dt, dual_step, t;
+};
+
+// A handle's moving part in a union of handles: the time of its last step,
+// which of its ping-pong states holds the current state, and the steps it
+// still has to run.
+struct VmsDyn {
+  double t;
+  int par;
+  int left;
 };
 
 // The reference's Gauss weights are lN[g][a] = GB + GD * (g == a) with
@@ -408,10 +425,10 @@ struct BlockArgs {
   VmsConst k;
 };
 
-// Block maximum of v (>= 0) -> out[blockIdx.x]; one plain store per block
-// (the CFL diagnostic is reduced over blocks only when it is read back).
+// Block maximum of v (>= 0) -> out[row]; one plain store per block (the
+// CFL diagnostic is reduced over blocks only when it is read back).
 template <int NT>
-__device__ __forceinline__ void block_max_store(double v, double *out) {
+__device__ __forceinline__ void block_max_store(double v, double *out, int row) {
   __shared__ double wm[NT / 64];
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(v, off);
@@ -423,7 +440,7 @@ __device__ __forceinline__ void block_max_store(double v, double *out) {
     double m = wm[0];
 #pragma unroll
     for (int i = 1; i < NT / 64; i++) m = wm[i] > m ? wm[i] : m;
-    out[blockIdx.x] = m;
+    out[row] = m;
   }
 }
 
@@ -467,8 +484,10 @@ __device__ __forceinline__ double uni(double v) {
 }
 __device__ __forceinline__ double4 uni(double4 v) { return make_double4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
 
+// One element block `blk` of one mesh; its per-block maximum goes to
+// a.bmax[row].
 template <typename T, typename S, bool PROJ, bool RES, bool ACC>
-__global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 4 : 2)) void vms_block(BlockArgs<T> a) {
+__device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int blk, const int row) {
   using T4 = typename Vec4<T>::type;
   // FP64: staged node records as 16-byte pairs {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}.
   // FP32 element math: coordinates and velocities relative to the block's
@@ -505,7 +524,6 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
   if constexpr (F32) sres = L.sres;
   else sres = sres64;
   const int tid = threadIdx.x;
-  const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
   VmsConst k = a.k;
   int node0 = 0;
   if (a.kt) {
@@ -803,7 +821,27 @@ __global__ __launch_bounds__(VB, sizeof(T) == 4 && !ACC ? 5 : (sizeof(T) == 4 ? 
       static_cast<typename Vec4<S>::type *>(a.partials)[slot_base + ((meta_own >> LN_RANK_SHIFT) & LN_RANK)] = v;
     }
   }
-  block_max_store<VB>(m2, a.bmax);
+  block_max_store<VB>(m2, a.bmax, row);
+}
+
+// occupancy (waves per SIMD): FP32 element math 5 (4 with dual iterations), FP64 2
+#define VMS_BLOCK_WAVES(T, ACC) (sizeof(T) == 4 && !(ACC) ? 5 : (sizeof(T) == 4 ? 4 : 2))
+template <typename T, typename S, bool PROJ, bool RES, bool ACC>
+__global__ __launch_bounds__(VB, VMS_BLOCK_WAVES(T, ACC)) void vms_block(BlockArgs<T> a) {
+  const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
+  vms_block_run<T, S, PROJ, RES, ACC>(a, blk, blockIdx.x);
+}
+
+__device__ __forceinline__ VmsConst ld_const(const VmsConst *p) {
+  VmsConst v;
+  v.nu = p->nu;
+  v.inv_eps = p->inv_eps;
+  v.c1 = p->c1;
+  v.c2 = p->c2;
+  v.inv_dt = p->inv_dt;
+  v.dual_step = p->dual_step;
+  v.t = p->t;
+  return v;
 }
 
 struct FixArgs {
@@ -826,11 +864,12 @@ struct FixArgs {
   VmsConst k;
 };
 
+// Shared node s of one mesh; the workgroup's maximum goes to a.bmax[row].
 template <typename T>
-__global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
+__device__ __forceinline__ void vms_fixup_run(const FixArgs &a, const int s, const int row) {
   using T4 = typename Vec4<T>::type;
   const T4 *partials = static_cast<const T4 *>(a.partials);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double *const bmax_row = a.bmax + row;
   double m2 = 0.0;
   if (s < a.n_sh) {
     // two rounds of independent loads: what s indexes (incl. its first 4
@@ -873,7 +912,17 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
       R[2] += (double)q.z;
       R[3] += (double)q.w;
     }
-    const VmsConst k = a.kt ? a.kt[a.node_mesh[g]] : a.k;
+    VmsConst k = a.k;
+    if (a.kt) {
+      const VmsConst *q = a.kt + a.node_mesh[g];
+      k.nu = q->nu;
+      k.inv_eps = q->inv_eps;
+      k.c1 = q->c1;
+      k.c2 = q->c2;
+      k.inv_dt = q->inv_dt;
+      k.dual_step = q->dual_step;
+      k.t = q->t;
+    }
     double4 pre, post;
     if (unscaled) {
       m2 = node_update(base, im, R, k, pre, post, rec, a.rec, a.row_off, a.rows, a.err);
@@ -884,7 +933,12 @@ __global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
     if (rec >= 0) a.Prn[rec] = pre;
     a.Qn[g] = post;
   }
-  block_max_store<256>(m2, a.bmax);
+  block_max_store<256>(m2, bmax_row, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void vms_fixup(FixArgs a) {
+  vms_fixup_run<T>(a, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x);
 }
 
 // Nodal lumped mass nd_mass[n] = sum over incident elements of V/4 in
@@ -1103,6 +1157,11 @@ struct femx_vms {
   DevBuf<double> kdt;        // per mesh dt
   DevBuf<int> blk_mesh, mesh_node0, node_mesh;
   std::vector<int> h_node0, h_nnodes;
+  // membership of a union (femx_vms_union_*): its device-side time, state
+  // parity and steps left, and the host's view of the steps left
+  femx_vms_union *in_union = nullptr;
+  DevBuf<VmsDyn> dyn;
+  int u_left = 0;
   std::vector<double> h_time, h_dt;
   int xcd = 1;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps, by state parity
@@ -1203,15 +1262,8 @@ femx_status launch_block(femx_vms *s, bool proj, bool res, bool acc, bool first_
   return FEMX_OK;
 }
 
-femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
+FixArgs fix_args(femx_vms *s, bool first_dual, double t) {
   Blocks &b = s->mesh->blk;
-  if (b.n_sh == 0) return FEMX_OK;
-  const bool f32p = f32_sum(s->p.precision);
-  const double rec_bytes = f32p ? 16.0 : 32.0;  // one partial slot record
-  // partials written and read; per shared node its index, slot range, first
-  // slots, 1/m and BC record, the base state read and the post-BC state written
-  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 16.0 + 8.0 + 4.0 + 32.0 + 32.0);
-  TimedLaunch tl(s->timer, s->timer.kind("vms_fixup", bytes), s->st);
   FixArgs a;
   a.n_sh = b.n_sh;
   a.sh_node = b.d_sh_node.p;
@@ -1234,6 +1286,19 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   a.node_mesh = s->batch ? s->node_mesh.p : nullptr;
   a.kt = s->batch ? s->kt.p : nullptr;
   a.k = vms_const(s, t);
+  return a;
+}
+
+femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
+  Blocks &b = s->mesh->blk;
+  if (b.n_sh == 0) return FEMX_OK;
+  const bool f32p = f32_sum(s->p.precision);
+  const double rec_bytes = f32p ? 16.0 : 32.0;  // one partial slot record
+  // partials written and read; per shared node its index, slot range, first
+  // slots, 1/m and BC record, the base state read and the post-BC state written
+  const double bytes = b.sh_off[b.n_sh] * rec_bytes * 2 + b.n_sh * (4.0 + 4.0 + 16.0 + 8.0 + 4.0 + 32.0 + 32.0);
+  TimedLaunch tl(s->timer, s->timer.kind("vms_fixup", bytes), s->st);
+  const FixArgs a = fix_args(s, first_dual, t);
   if (f32p) vms_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   else vms_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   FEMX_HIP(hipGetLastError());
@@ -1384,6 +1449,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
 
 extern "C" femx_status femx_vms_destroy(femx_vms *s) {
   if (!s) return FEMX_OK;
+  if (s->in_union) (void)femx_vms_union_remove(s->in_union, s);
   hipStream_t st = s->st;
   if (st) (void)hipStreamSynchronize(st);
   femx_mesh *own = s->owned;
@@ -1395,6 +1461,7 @@ extern "C" femx_status femx_vms_destroy(femx_vms *s) {
 
 extern "C" femx_status femx_vms_set_state(femx_vms *s, const double *sol, const double *sdus) {
   if (!s || !sol) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   Mesh &m = *s->mesh;
   s->cur = 0;
   s->iter = 0;
@@ -1434,6 +1501,7 @@ extern "C" femx_status femx_vms_set_state(femx_vms *s, const double *sol, const 
 
 extern "C" femx_status femx_vms_step(femx_vms *s, int nsteps) {
   if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   for (int k = 0; k < nsteps; k++) {
     s->time += s->p.dt;  // currentTime += timeStep (:1028)
     for (int d = 0; d <= s->p.tot_dual_steps; d++) {
@@ -1452,6 +1520,7 @@ extern "C" femx_status femx_vms_step(femx_vms *s, int nsteps) {
 
 extern "C" femx_status femx_vms_get_state(femx_vms *s, double *sol, double *sdus, double *time) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   Mesh &m = *s->mesh;
   femx_status r = flush_projection(s);
   if (r != FEMX_OK) return r;
@@ -1897,3 +1966,274 @@ extern "C" femx_status femx_vms_batch_autotune(int n, const femx_mesh_desc *mesh
     }
   return FEMX_OK;
 }
+
+// ------------------------------------------------------- unions of handles
+// Continuous batching of resident solves: several VMS handles, each with its
+// own mesh, state and arrays, advanced together by ONE launch of the block
+// kernel, one of the fix-up and one tick per step, whatever their sizes.  A
+// device table holds every member's kernel arguments; each workgroup finds
+// its member by a binary search over the members' block offsets, takes the
+// member's time, state parity and steps left from the member's device
+// record, and runs the member's block with the same code as the member's own
+// launch, so a member's result is bit for bit that of the handle stepped on
+// its own.  Members join and leave between enqueued steps (only the small
+// table is uploaded again); a member whose steps are done exits its blocks
+// at once until it is removed.
+namespace femx {
+namespace {
+
+struct VmsMember {
+  BlockArgs<float> b;
+  FixArgs f;
+  double4 *Qb[2];
+  double4 *Prb[2];
+  VmsDyn *dyn;
+  double dt;
+};
+
+// member h with off[h] <= g < off[h + 1] (off ascending, nm + 1 entries)
+__device__ __forceinline__ int member_of(const int *off, int nm, int g) {
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uni(off[mid]) <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(VB, VMS_BLOCK_WAVES(float, false)) void vms_block_union(const VmsMember *mem,
+                                                                                      const int *blk_off, int nm,
+                                                                                      int ntot) {
+  const int g = xcd_block(blockIdx.x, ntot);
+  const int h = member_of(blk_off, nm, g);
+  const VmsMember &me = mem[h];
+  const int left = uni(me.dyn->left);
+  if (left <= 0) return;
+  const int par = uni(me.dyn->par);
+  BlockArgs<float> a = me.b;
+  a.Q = me.Qb[par];
+  a.Qn = me.Qb[par ^ 1];
+  a.Pr = me.Prb[par];
+  a.Prn = me.Prb[par ^ 1];
+  a.k.t = uni(me.dyn->t) + me.dt;  // currentTime += timeStep (:1028)
+  const int blk = g - uni(blk_off[h]);
+  vms_block_run<float, double, true, true, false>(a, blk, blk);
+}
+
+__global__ __launch_bounds__(256) void vms_fixup_union(const VmsMember *mem, const int *fx_off, int nm) {
+  const int g = blockIdx.x;
+  const int h = member_of(fx_off, nm, g);
+  const VmsMember &me = mem[h];
+  const int left = uni(me.dyn->left);
+  if (left <= 0) return;
+  const int par = uni(me.dyn->par);
+  FixArgs a = me.f;
+  a.Q = me.Qb[par];
+  a.Qn = me.Qb[par ^ 1];
+  a.Pr = me.Prb[par];
+  a.Prn = me.Prb[par ^ 1];
+  a.k.t = uni(me.dyn->t) + me.dt;
+  const int wg = g - uni(fx_off[h]);
+  vms_fixup_run<double>(a, wg * 256 + (int)threadIdx.x, wg);
+}
+
+// end of a union step: every member with steps left moves on by one
+__global__ void vms_union_tick(const VmsMember *mem, int nm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  VmsDyn *d = mem[i].dyn;
+  if (d->left > 0) {
+    d->t = d->t + mem[i].dt;
+    d->par ^= 1;
+    d->left -= 1;
+  }
+}
+
+__global__ void vms_union_join(VmsDyn *d, double t, int par, int left) {
+  d->t = t;
+  d->par = par;
+  d->left = left;
+}
+
+}  // namespace
+}  // namespace femx
+
+struct femx_vms_union {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<femx_vms *> mem;
+  bool dirty = true;
+  int nblk = 0, nfx = 0;
+  DevBuf<VmsMember> tab;
+  DevBuf<int> off;  // blk_off [nm + 1] then fx_off [nm + 1]
+  size_t cap = 0;   // members the device table holds
+  // pinned staging of the table uploads, a ring of slots, each reused once
+  // the copy that read it has run
+  static constexpr int NSLOT = 3;
+  unsigned char *pin[NSLOT] = {nullptr, nullptr, nullptr};
+  size_t pin_cap[NSLOT] = {0, 0, 0};
+  hipEvent_t pin_ev[NSLOT] = {nullptr, nullptr, nullptr};
+  int pin_next = 0;
+};
+
+namespace {
+
+femx_status union_upload(femx_vms_union *u) {
+  const int nm = (int)u->mem.size();
+  std::vector<VmsMember> t(nm);
+  std::vector<int> off(2L * (nm + 1), 0);
+  for (int i = 0; i < nm; i++) {
+    femx_vms *h = u->mem[i];
+    VmsMember &e = t[i];
+    e.b = block_args<float>(h, true, 0.0, h->sdus32.p);
+    e.b.diag_res = nullptr;
+    e.b.diag_sdu = nullptr;
+    e.f = fix_args(h, true, 0.0);
+    for (int k = 0; k < 2; k++) {
+      e.Qb[k] = h->Q[k].p;
+      e.Prb[k] = h->Pr[k].p;
+    }
+    e.dyn = h->dyn.p;
+    e.dt = h->p.dt;
+    const Blocks &b = h->mesh->blk;
+    off[i + 1] = off[i] + b.nb;
+    off[nm + 1 + i + 1] = off[nm + 1 + i] + (b.n_sh ? grid_for(b.n_sh, 256) : 0);
+  }
+  u->nblk = off[nm];
+  u->nfx = off[2 * nm + 1];
+  femx_status r;
+  if ((size_t)nm > u->cap) {
+    // grow (the old table may still be read by enqueued steps: wait for them)
+    FEMX_HIP(hipStreamSynchronize(u->st));
+    const size_t cap = std::max<size_t>(2 * (size_t)nm, 16);
+    if ((r = u->tab.alloc(cap)) != FEMX_OK || (r = u->off.alloc(2 * (cap + 1))) != FEMX_OK) return r;
+    u->cap = cap;
+  }
+  const size_t bytes_t = nm * sizeof(VmsMember), bytes_o = off.size() * sizeof(int);
+  const int k = u->pin_next;
+  u->pin_next = (k + 1) % femx_vms_union::NSLOT;
+  if (u->pin_ev[k]) FEMX_HIP(hipEventSynchronize(u->pin_ev[k]));
+  if (u->pin_cap[k] < bytes_t + bytes_o) {
+    if (u->pin[k]) FEMX_HIP(hipHostFree(u->pin[k]));
+    u->pin[k] = nullptr;
+    const size_t want = 2 * (bytes_t + bytes_o) + 4096;
+    FEMX_HIP(hipHostMalloc((void **)&u->pin[k], want, hipHostMallocDefault));
+    u->pin_cap[k] = want;
+  }
+  if (nm > 0) {
+    std::memcpy(u->pin[k], t.data(), bytes_t);
+    FEMX_HIP(hipMemcpyAsync(u->tab.p, u->pin[k], bytes_t, hipMemcpyHostToDevice, u->st));
+  }
+  std::memcpy(u->pin[k] + bytes_t, off.data(), bytes_o);
+  FEMX_HIP(hipMemcpyAsync(u->off.p, u->pin[k] + bytes_t, bytes_o, hipMemcpyHostToDevice, u->st));
+  if (!u->pin_ev[k]) FEMX_HIP(hipEventCreateWithFlags(&u->pin_ev[k], hipEventDisableTiming));
+  FEMX_HIP(hipEventRecord(u->pin_ev[k], u->st));
+  u->dirty = false;
+  return FEMX_OK;
+}
+
+}  // namespace
+
+extern "C" femx_status femx_vms_union_create(femx_vms_union **out) {
+  if (!out) return set_error("null argument"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  femx_vms_union *u = new femx_vms_union();
+  (void)hipGetDevice(&u->device);
+  if (hipStreamCreateWithFlags(&u->st, hipStreamNonBlocking) != hipSuccess) {
+    delete u;
+    return set_error("hipStreamCreate failed"), FEMX_ERR_HIP;
+  }
+  *out = u;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_union_destroy(femx_vms_union *u) {
+  if (!u) return FEMX_OK;
+  (void)hipStreamSynchronize(u->st);
+  for (femx_vms *h : u->mem) h->in_union = nullptr;
+  for (int k = 0; k < femx_vms_union::NSLOT; k++) {
+    if (u->pin[k]) (void)hipHostFree(u->pin[k]);
+    if (u->pin_ev[k]) (void)hipEventDestroy(u->pin_ev[k]);
+  }
+  hipStream_t st = u->st;
+  delete u;
+  (void)hipStreamDestroy(st);
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_union_add(femx_vms_union *u, femx_vms *h, int steps) {
+  if (!u || !h || steps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (h->in_union) return set_error("handle already in a union"), FEMX_ERR_ARG;
+  if (h->p.precision != FEMX_MIXED || h->p.tot_dual_steps != 0 || h->batch || h->diag)
+    return set_error("union members: FEMX_MIXED, no dual iterations, no diagnostics, not a batch"),
+           FEMX_ERR_UNSUPPORTED;
+  if (!h->proj_pending) return set_error("union members run their first step on their own"), FEMX_ERR_ARG;
+  femx_status r;
+  if (!h->dyn.p && (r = h->dyn.alloc(1)) != FEMX_OK) return r;
+  // the union's stream waits for the work already queued on the handle's
+  hipEvent_t ev = nullptr;
+  FEMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const bool ok = hipEventRecord(ev, h->st) == hipSuccess && hipStreamWaitEvent(u->st, ev, 0) == hipSuccess;
+  (void)hipEventDestroy(ev);
+  if (!ok) return set_error("hipStreamWaitEvent failed"), FEMX_ERR_HIP;
+  vms_union_join<<<1, 1, 0, u->st>>>(h->dyn.p, h->time, h->cur, steps);
+  FEMX_HIP(hipGetLastError());
+  h->in_union = u;
+  h->u_left = steps;
+  u->mem.push_back(h);
+  u->dirty = true;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_union_step(femx_vms_union *u, int k) {
+  if (!u || k < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  femx_status r;
+  if (u->dirty && (r = union_upload(u)) != FEMX_OK) return r;
+  const int nm = (int)u->mem.size();
+  if (nm == 0 || k == 0) return FEMX_OK;
+  const int *blk_off = u->off.p, *fx_off = u->off.p + nm + 1;
+  for (int i = 0; i < k; i++) {
+    if (u->nblk > 0) vms_block_union<<<u->nblk, VB, 0, u->st>>>(u->tab.p, blk_off, nm, u->nblk);
+    if (u->nfx > 0) vms_fixup_union<<<u->nfx, 256, 0, u->st>>>(u->tab.p, fx_off, nm);
+    vms_union_tick<<<grid_for(nm, 64), 64, 0, u->st>>>(u->tab.p, nm);
+  }
+  FEMX_HIP(hipGetLastError());
+  // the host's view of every member: the same time sequence as its own steps
+  for (femx_vms *h : u->mem) {
+    const int n = std::min(k, h->u_left);
+    for (int i = 0; i < n; i++) h->time += h->p.dt;
+    h->cur ^= n & 1;
+    h->iter += n;
+    h->u_left -= n;
+  }
+  return FEMX_OK;
+}
+
+extern "C" int femx_vms_union_left(femx_vms_union *u, femx_vms *h) {
+  if (!u || !h || h->in_union != u) return -1;
+  return h->u_left;
+}
+
+extern "C" femx_status femx_vms_union_remove(femx_vms_union *u, femx_vms *h) {
+  if (!u || !h || h->in_union != u) return set_error("handle not in this union"), FEMX_ERR_ARG;
+  // the handle's stream waits for the union's work on it
+  hipEvent_t ev = nullptr;
+  FEMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const bool ok = hipEventRecord(ev, u->st) == hipSuccess && hipStreamWaitEvent(h->st, ev, 0) == hipSuccess;
+  (void)hipEventDestroy(ev);
+  if (!ok) return set_error("hipStreamWaitEvent failed"), FEMX_ERR_HIP;
+  u->mem.erase(std::find(u->mem.begin(), u->mem.end(), h));
+  h->in_union = nullptr;
+  h->u_left = 0;
+  u->dirty = true;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_vms_union_sync(femx_vms_union *u) {
+  if (!u) return set_error("null argument"), FEMX_ERR_ARG;
+  FEMX_HIP(hipStreamSynchronize(u->st));
+  return FEMX_OK;
+}
+
+extern "C" void *femx_vms_union_stream(femx_vms_union *u) { return u ? (void *)u->st : nullptr; }

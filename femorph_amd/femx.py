@@ -106,6 +106,14 @@ SIGNATURES = [
     ("femx_vms_get_norms", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("femx_vms_sync", C.c_int, [C.c_void_p]),
     ("femx_vms_stream", C.c_void_p, [C.c_void_p]),
+    ("femx_vms_union_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("femx_vms_union_destroy", C.c_int, [C.c_void_p]),
+    ("femx_vms_union_add", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("femx_vms_union_step", C.c_int, [C.c_void_p, C.c_int]),
+    ("femx_vms_union_left", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("femx_vms_union_remove", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("femx_vms_union_sync", C.c_int, [C.c_void_p]),
+    ("femx_vms_union_stream", C.c_void_p, [C.c_void_p]),
     ("femx_vms_batch_create", C.c_int, [C.c_int, C.POINTER(MeshDesc), C.POINTER(BCDesc), C.POINTER(VMSParams),
                                         C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("femx_vms_batch_destroy", C.c_int, [C.c_void_p]),
@@ -305,6 +313,40 @@ def vms_batch_autotune(items, steps=6):
     _check(load().femx_vms_batch_autotune(len(items), d.meshes, d.bcs, d.params, steps, _p(b, _ip), _p(x, _ip),
                                           _p(ms)))
     return int(b[0]), int(x[0]), ms
+
+
+class VMSUnion:
+    """Continuous batching of VMS handles (femx_vms_union_*): members of any
+    sizes advanced together, one launch per phase per step, each by its own
+    number of steps; bit for bit the handles stepped on their own."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(load().femx_vms_union_create(C.byref(h)))
+        self._h = h
+        self._members = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.femx_vms_union_destroy(self._h)
+            self._h = None
+
+    def add(self, solver, steps):
+        _check(load().femx_vms_union_add(self._h, solver._h, int(steps)))
+        self._members[id(solver)] = solver
+
+    def step(self, k):
+        _check(load().femx_vms_union_step(self._h, int(k)))
+
+    def left(self, solver):
+        return load().femx_vms_union_left(self._h, solver._h)
+
+    def remove(self, solver):
+        _check(load().femx_vms_union_remove(self._h, solver._h))
+        self._members.pop(id(solver), None)
+
+    def sync(self):
+        _check(load().femx_vms_union_sync(self._h))
 
 
 class VMSBatch:

@@ -74,6 +74,7 @@ typedef struct {
 
 typedef struct femx_mesh femx_mesh;
 typedef struct femx_vms femx_vms;
+typedef struct femx_vms_union femx_vms_union;
 typedef struct femx_supg femx_supg;
 
 /* Text of the last error on this thread ("" if none). */
@@ -168,6 +169,31 @@ femx_status femx_vms_get_norms(femx_vms *s, int nsteps, double *norms);
 femx_status femx_vms_sync(femx_vms *s);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 void *femx_vms_stream(femx_vms *s);
+
+/* ----------------------------------------------- unions of VMS handles */
+/* Continuous batching of resident solves (no counterpart in the reference,
+ * which runs one solve per process): handles of any sizes join a union and
+ * are advanced together, one launch of each phase per step for all of
+ * them, each by its own number of steps; they join and leave between
+ * enqueued steps.  A member's result is bit for bit that of the handle
+ * stepped on its own.  Members: FEMX_MIXED, no dual iterations, no
+ * diagnostics, not a batch, at least one step run on their own.  While in a
+ * union a handle is not stepped, set or read on its own. */
+femx_status femx_vms_union_create(femx_vms_union **out);
+femx_status femx_vms_union_destroy(femx_vms_union *u);
+/* h joins with `steps` steps to run (the union's stream first waits for the
+ * work queued on h's stream). */
+femx_status femx_vms_union_add(femx_vms_union *u, femx_vms *h, int steps);
+/* Enqueues k union steps on the union's stream (asynchronous): every member
+ * with steps left runs one per union step. */
+femx_status femx_vms_union_step(femx_vms_union *u, int k);
+/* Steps h has left after the enqueued union steps (-1: not a member). */
+int femx_vms_union_left(femx_vms_union *u, femx_vms *h);
+/* h leaves; its own stream waits for the union's work, so femx_vms_get_state
+ * then returns its state after its steps. */
+femx_status femx_vms_union_remove(femx_vms_union *u, femx_vms *h);
+femx_status femx_vms_union_sync(femx_vms_union *u);
+void *femx_vms_union_stream(femx_vms_union *u);
 
 /* ------------------------------------------------------- VMS batches */
 /* Many independent VMS tet4 solves advanced together (the reference runs

@@ -1,5 +1,7 @@
 """The solve server (femx_server_*): requests admitted by device footprint
-while others run, every resident request advanced on its own stream.
+while others run; mixed-precision VMS requests advanced together in a union
+of resident handles (continuous batching), the others each on its own
+stream.
 
 Tolerance: bitwise.  A served request runs the arithmetic of its own
 handle, so its result must equal the same request run alone (and the
@@ -74,7 +76,8 @@ def test_results_alone_equal_in_the_mix(gpu):
     assert peak <= 4
     results = {i: srv.result(i) for i in ids}
     srv.close()
-    for req, i in ((probe_v, ids[3]), (probe_s, ids[6])):
+    # every request, VMS ones served through the union of resident handles
+    for req, i in zip(mix, ids):
         assert stats[i]["steps_done"] == req[3]
         assert np.array_equal(results[i], _alone(gpu, req)), req[0]
 

@@ -1,0 +1,93 @@
+"""Unions of VMS handles (continuous batching of resident solves,
+femx_vms_union_*): members of different sizes and Reynolds numbers join at
+different times and leave when their own step counts are done; every
+member's final state and subgrid velocities equal, bit for bit, those of the
+same handle stepped on its own (same blocks, same sums, same order)."""
+import numpy as np
+import pytest
+
+from femorph_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _member(gpu, n, kind, jitter, seed, re):
+    m = meshgen.tet_cube(n, jitter, seed)
+    bc = meshgen.lid_cavity_bc(m) if kind == "cavity" else meshgen.channel_bc(m)
+    p = dict(rho=1.0, mu=1.0 / re, inv_eps=1.0, c1=4.0, c2=2.0, dt=0.1 * meshgen.min_tet_h(m))
+    ic = meshgen.random_velocity(m, 1e-3, seed + 11)
+    return m, bc, p, ic
+
+
+def _solver(gpu, case):
+    m, bc, p, ic = case
+    dm = gpu.DeviceMesh(m)
+    g = gpu.VMSSolver(dm, bc, precision=gpu.FEMX_MIXED, **p)
+    g.set_state(ic)
+    return dm, g
+
+
+def test_union_members_equal_their_own_runs(gpu):
+    cases = [_member(gpu, 7, "cavity", 0.1, 1, 100.0), _member(gpu, 4, "channel", 0.15, 2, 200.0),
+             _member(gpu, 11, "cavity", 0.0, 3, 400.0), _member(gpu, 3, "channel", 0.2, 4, 50.0),
+             _member(gpu, 9, "channel", 0.1, 5, 300.0)]
+    steps = [9, 14, 5, 11, 7]
+    refs = []
+    for case, n in zip(cases, steps):
+        dm, g = _solver(gpu, case)
+        g.step(n)
+        sol, sdus, t = g.get_state()
+        refs.append((sol, sdus, t))
+        del g, dm
+    u = gpu.VMSUnion()
+    live = {}
+    order = [0, 1, 2, 3, 4]
+    joined = 0
+    rounds = 0
+    while joined < len(order) or live:
+        # one new member per round: each runs its first step on its own
+        if joined < len(order):
+            i = order[joined]
+            dm, g = _solver(gpu, cases[i])
+            g.step(1)
+            u.add(g, steps[i] - 1)
+            live[i] = (dm, g)
+            joined += 1
+        print(f"round {rounds}: members {sorted(live)}", flush=True)
+        u.step(2)
+        rounds += 1
+        for i in list(live):
+            dm, g = live[i]
+            if u.left(g) == 0:
+                u.remove(g)
+                sol, sdus, t = g.get_state()
+                rsol, rsdus, rt = refs[i]
+                assert t == rt, (i, t, rt)
+                assert np.array_equal(sol, rsol), (i, np.abs(sol - rsol).max())
+                assert np.array_equal(sdus, rsdus), (i, np.abs(sdus - rsdus).max())
+                del live[i]
+    assert rounds >= 5
+
+
+def test_union_rejects_non_members(gpu):
+    case = _member(gpu, 3, "cavity", 0.0, 1, 100.0)
+    dm, g = _solver(gpu, case)
+    u = gpu.VMSUnion()
+    with pytest.raises(gpu.FemxError, match="first step"):
+        u.add(g, 3)  # no step run on its own yet
+    g.step(1)
+    u.add(g, 3)
+    with pytest.raises(gpu.FemxError, match="in a union"):
+        g.step(1)
+    with pytest.raises(gpu.FemxError, match="in a union"):
+        g.get_state()
+    u.step(3)
+    u.remove(g)
+    sol, _, _ = g.get_state()
+    assert np.isfinite(sol).all()
+    m, bc, p, ic = case
+    g64 = gpu.VMSSolver(dm, bc, precision=gpu.FEMX_FP64, **p)
+    g64.set_state(ic)
+    g64.step(1)
+    with pytest.raises(gpu.FemxError, match="FEMX_MIXED"):
+        u.add(g64, 2)

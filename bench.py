@@ -336,6 +336,8 @@ def workload_serve(args):
         reqs = []
         for _ in range(args.requests):
             kind = "vms" if rng.random() < 0.5 else "supg"
+            if args.serve_kinds != "both":
+                kind = args.serve_kinds
             n = int(np.rint(np.exp(rng.uniform(np.log(8), np.log(64)))))
             steps = int(rng.integers(50, 501))
             re = float(rng.uniform(50.0, 500.0))
@@ -383,6 +385,17 @@ def workload_serve(args):
                                  "mean_ms": {"queue": round(float(wait.mean()) * 1e3, 2),
                                              "prep": round(float(prep.mean()) * 1e3, 2),
                                              "resident": round(float(run.mean()) * 1e3, 2)}}
+        by_kind = {}
+        for kname, kid in (("vms", 0), ("supg", 1)):
+            sel = np.array([s_["kind"] == kid for s_ in st])
+            if sel.any():
+                by_kind[kname] = {"requests": int(sel.sum()),
+                                  "p50_ms": round(float(np.percentile(lat[sel], 50)) * 1e3, 2),
+                                  "p99_ms": round(float(np.percentile(lat[sel], 99)) * 1e3, 2),
+                                  "max_prep_ms": round(float(prep[sel].max()) * 1e3, 2),
+                                  "mean_prep_ms": round(float(prep[sel].mean()) * 1e3, 2),
+                                  "mean_resident_ms": round(float(run[sel].mean()) * 1e3, 2)}
+        table[str(int(rate))]["by_kind"] = by_kind
         print(json.dumps({"rate": rate, **table[str(int(rate))]}), file=sys.stderr, flush=True)
     top = table[str(int(rates[-1]))]
     out = {"metric": "p50/p99 solve latency (served VMS tet4 / SUPG hexa8 requests)",
@@ -746,6 +759,8 @@ def main():
                     help="seconds of settling steps at most before a timed window (0: none, for profiler runs)")
     ap.add_argument("--ins-elem", choices=["tet4", "hexa8"], default="tet4", help="ins: element type")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
+    ap.add_argument("--serve-kinds", choices=["both", "vms", "supg"], default="both",
+                    help="serve: request kinds (both: half VMS tet4, half SUPG hexa8)")
     ap.add_argument("--requests", type=int, default=120, help="requests per rate (serve)")
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--max-active", type=int, default=64)

@@ -21,22 +21,44 @@ femx_status hip_fail(hipError_t err, const char *what);
     if (_e != hipSuccess) return femx::hip_fail(_e, #call); \
   } while (0)
 
+// ------------------------------------------------------- device memory
+// hipFree waits for every stream of the device, so freeing a finished
+// solve's buffers while other solves run stalls all of them.  A handle that
+// has synchronised its own stream releases its buffers inside an IdleRelease
+// scope: they go to a per-device cache of idle blocks (sizes rounded up to
+// size classes within 1/8 of the request), from which later allocations of
+// the same class are served before hipMalloc is called.  Frees outside such
+// a scope (a buffer re-allocated, temporaries) call hipFree as before.  The
+// cache holds at most femx_set_device_cache bytes per device (environment
+// FEMX_DEVICE_CACHE_BYTES, default 8 GiB) and is emptied when hipMalloc
+// runs out of memory.  Handle streams are cached the same way.
+void *dev_alloc(size_t bytes, int *dev, hipError_t *err);
+void dev_free(void *p, size_t bytes, int dev);
+hipStream_t stream_get();         // a non-blocking stream of the current device
+void stream_put(hipStream_t st);  // a stream with no work left, for reuse
+struct IdleRelease {
+  IdleRelease();
+  ~IdleRelease();
+  IdleRelease(const IdleRelease &) = delete;
+  IdleRelease &operator=(const IdleRelease &) = delete;
+};
+
 // --------------------------------------------------------- device buffers
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  int dev = 0;
   femx_status alloc(size_t count) {
     free();
-    n = count;
     if (count == 0) return FEMX_OK;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      n = 0;
+    hipError_t e = hipSuccess;
+    p = static_cast<T *>(dev_alloc(count * sizeof(T), &dev, &e));
+    if (!p) {
       set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
       return FEMX_ERR_NOMEM;
     }
+    n = count;
     return FEMX_OK;
   }
   femx_status upload(const T *h, size_t count, hipStream_t st) {
@@ -47,7 +69,7 @@ struct DevBuf {
     return FEMX_OK;
   }
   void free() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, n * sizeof(T), dev);
     p = nullptr;
     n = 0;
   }

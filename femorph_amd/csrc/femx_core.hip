@@ -5,6 +5,7 @@
 #include <mutex>
 #include <string>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <thread>
 
@@ -22,6 +23,140 @@ femx_status hip_fail(hipError_t err, const char *what) {
 }
 
 int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
+
+// ------------------------------------------------ device memory cache
+namespace {
+constexpr int CACHE_DEVICES = 64;
+constexpr size_t CACHE_STREAMS = 64;  // idle streams kept per device
+
+size_t cache_cap_from_env() {
+  const char *e = getenv("FEMX_DEVICE_CACHE_BYTES");
+  if (!e || !*e) return size_t(8) << 30;
+  return (size_t)strtoull(e, nullptr, 10);
+}
+
+struct DeviceCache {
+  std::mutex mu;
+  std::map<size_t, std::vector<void *>> blocks[CACHE_DEVICES];  // size class -> idle blocks
+  size_t held[CACHE_DEVICES] = {};
+  std::vector<hipStream_t> streams[CACHE_DEVICES];
+  size_t cap = cache_cap_from_env();
+};
+
+// Never destroyed: the blocks it holds are released with the process, not by
+// hipFree calls from static destructors after the runtime has shut down.
+DeviceCache &cache() {
+  static DeviceCache *c = new DeviceCache();
+  return *c;
+}
+
+thread_local int t_idle = 0;
+
+// Up to 4 KiB: multiples of 256 B; above: multiples of 1/8 of the power of
+// two below the size (at most 12.5% more than asked).
+size_t size_class(size_t b) {
+  if (b <= 4096) return (b + 255) & ~size_t(255);
+  const int lg = 63 - __builtin_clzll(b - 1);
+  const size_t step = size_t(1) << (lg - 3);
+  return (b + step - 1) / step * step;
+}
+
+// Takes the device's blocks out of the cache (with the lock held) and
+// returns them for hipFree without the lock.
+std::vector<void *> take_blocks(DeviceCache &c, int d, size_t keep) {
+  std::vector<void *> out;
+  for (auto it = c.blocks[d].begin(); it != c.blocks[d].end() && c.held[d] > keep; ++it)
+    while (!it->second.empty() && c.held[d] > keep) {
+      out.push_back(it->second.back());
+      it->second.pop_back();
+      c.held[d] -= it->first;
+    }
+  return out;
+}
+}  // namespace
+
+IdleRelease::IdleRelease() { t_idle++; }
+IdleRelease::~IdleRelease() { t_idle--; }
+
+void *dev_alloc(size_t bytes, int *dev, hipError_t *err) {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  *dev = d;
+  const size_t c = size_class(bytes);
+  DeviceCache &dc = cache();
+  if (d >= 0 && d < CACHE_DEVICES) {
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.blocks[d].find(c);
+    if (it != dc.blocks[d].end() && !it->second.empty()) {
+      void *p = it->second.back();
+      it->second.pop_back();
+      dc.held[d] -= c;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e == hipErrorOutOfMemory && d >= 0 && d < CACHE_DEVICES) {
+    std::vector<void *> idle;
+    {
+      std::lock_guard<std::mutex> lk(dc.mu);
+      idle = take_blocks(dc, d, 0);
+    }
+    if (!idle.empty()) {
+      (void)hipGetLastError();
+      for (void *q : idle) (void)hipFree(q);
+      e = hipMalloc(&p, c);
+    }
+  }
+  *err = e;
+  return e == hipSuccess ? p : nullptr;
+}
+
+void dev_free(void *p, size_t bytes, int dev) {
+  if (!p) return;
+  if (t_idle > 0 && dev >= 0 && dev < CACHE_DEVICES) {
+    const size_t c = size_class(bytes);
+    DeviceCache &dc = cache();
+    std::lock_guard<std::mutex> lk(dc.mu);
+    if (dc.held[dev] + c <= dc.cap) {
+      dc.blocks[dev][c].push_back(p);
+      dc.held[dev] += c;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+hipStream_t stream_get() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  DeviceCache &dc = cache();
+  if (d >= 0 && d < CACHE_DEVICES) {
+    std::lock_guard<std::mutex> lk(dc.mu);
+    if (!dc.streams[d].empty()) {
+      hipStream_t st = dc.streams[d].back();
+      dc.streams[d].pop_back();
+      return st;
+    }
+  }
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return st;
+}
+
+void stream_put(hipStream_t st) {
+  if (!st) return;
+  int d = -1;
+  if (hipStreamGetDevice(st, &d) == hipSuccess && d >= 0 && d < CACHE_DEVICES) {
+    DeviceCache &dc = cache();
+    std::lock_guard<std::mutex> lk(dc.mu);
+    if (dc.cap > 0 && dc.streams[d].size() < CACHE_STREAMS) {
+      dc.streams[d].push_back(st);
+      return;
+    }
+  }
+  (void)hipStreamDestroy(st);
+}
 
 femx_status BCDev::build(const femx_bc_desc *bc, int N, bool with_pressure, hipStream_t st) {
   std::vector<int> node_rec(N, -1);
@@ -500,6 +635,34 @@ extern "C" void femx_default_linear_solve(double *tol, int *max_iter, int *resta
   if (tol) *tol = g_tol;
   if (max_iter) *max_iter = g_max_iter;
   if (restart) *restart = g_restart;
+}
+
+extern "C" femx_status femx_set_device_cache(size_t max_bytes_per_device) {
+  DeviceCache &dc = cache();
+  std::vector<void *> idle;
+  std::vector<hipStream_t> streams;
+  {
+    std::lock_guard<std::mutex> lk(dc.mu);
+    dc.cap = max_bytes_per_device;
+    for (int d = 0; d < CACHE_DEVICES; d++) {
+      std::vector<void *> b = take_blocks(dc, d, max_bytes_per_device);
+      idle.insert(idle.end(), b.begin(), b.end());
+      if (max_bytes_per_device == 0) {
+        streams.insert(streams.end(), dc.streams[d].begin(), dc.streams[d].end());
+        dc.streams[d].clear();
+      }
+    }
+  }
+  for (void *p : idle) (void)hipFree(p);
+  for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+  return FEMX_OK;
+}
+
+extern "C" size_t femx_device_cache_bytes(int device) {
+  if (device < 0 || device >= CACHE_DEVICES) return 0;
+  DeviceCache &dc = cache();
+  std::lock_guard<std::mutex> lk(dc.mu);
+  return dc.held[device];
 }
 
 // ------------------------------------------------ HBM streaming probe

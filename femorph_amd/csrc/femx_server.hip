@@ -104,6 +104,10 @@ struct Request {
     }
     femx_vms_destroy(vh);
     femx_supg_destroy(sh);
+    // the request's only handle has completed its work (destroy synchronises
+    // its stream), so the mesh's buffers are idle too and go to the device
+    // cache instead of hipFree, which would wait for every other request
+    IdleRelease idle;
     femx_mesh_free(mesh);
     vh = nullptr;
     sh = nullptr;

@@ -1043,7 +1043,7 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
     femx_supg_destroy(s);
     return st;
   };
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(FEMX_ERR_HIP);
+  if (!(s->st = stream_get())) return fail(FEMX_ERR_HIP);
   if ((r = s->bc.build(bc, m.N, false, s->st)) != FEMX_OK) return fail(r);
   if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
   if ((r = s->xyzm.alloc(m.N)) != FEMX_OK || (r = s->Fprev.alloc(m.N)) != FEMX_OK) return fail(r);
@@ -1073,8 +1073,11 @@ extern "C" femx_status femx_supg_destroy(femx_supg *s) {
   if (!s) return FEMX_OK;
   hipStream_t st = s->st;
   if (st) (void)hipStreamSynchronize(st);
-  delete s;
-  if (st) (void)hipStreamDestroy(st);
+  {
+    IdleRelease idle;  // the handle's work has completed
+    delete s;
+  }
+  stream_put(st);
   return FEMX_OK;
 }
 

@@ -1356,7 +1356,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
     femx_vms_destroy(s);
     return st;
   };
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(FEMX_ERR_HIP);
+  if (!(s->st = stream_get())) return fail(FEMX_ERR_HIP);
   if ((r = s->bc.build(bc, m.N, true, s->st)) != FEMX_OK) return fail(r);
   if ((r = s->rec_node.upload(s->bc.h_rec_node.data(), s->bc.h_rec_node.size(), s->st)) != FEMX_OK) return fail(r);
   {
@@ -1453,9 +1453,14 @@ extern "C" femx_status femx_vms_destroy(femx_vms *s) {
   hipStream_t st = s->st;
   if (st) (void)hipStreamSynchronize(st);
   femx_mesh *own = s->owned;
-  delete s;
-  delete own;
-  if (st) (void)hipStreamDestroy(st);
+  {
+    // the handle's work (its stream, and a union's, which its stream waited
+    // for on leaving) has completed: its buffers go to the device cache
+    IdleRelease idle;
+    delete s;
+    delete own;
+  }
+  stream_put(st);
   return FEMX_OK;
 }
 

@@ -91,6 +91,8 @@ SIGNATURES = [
     ("femx_default_precision", C.c_int, []),
     ("femx_set_default_linear_solve", C.c_int, [C.c_double, C.c_int, C.c_int]),
     ("femx_default_linear_solve", None, [_dp, _ip, _ip]),
+    ("femx_set_device_cache", C.c_int, [C.c_size_t]),
+    ("femx_device_cache_bytes", C.c_size_t, [C.c_int]),
     ("femx_mesh_upload", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("femx_mesh_free", C.c_int, [C.c_void_p]),
     ("femx_mesh_block_stats", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_longlong)]),
@@ -223,6 +225,18 @@ def block_stats(mesh: Mesh, block_elems: int) -> dict:
 
 def device_count() -> int:
     return load().femx_device_count()
+
+
+def set_device_cache(max_bytes_per_device: int) -> None:
+    """Cap of the per-device cache that destroyed solver handles return
+    their device buffers and streams to (femx_set_device_cache); 0 frees
+    what it holds and turns it off."""
+    _check(load().femx_set_device_cache(int(max_bytes_per_device)))
+
+
+def device_cache_bytes(device: int = 0) -> int:
+    """Bytes of idle device memory the cache holds on a device."""
+    return int(load().femx_device_cache_bytes(int(device)))
 
 
 HBM_PROBE_KINDS = {"copy": 0, "read": 1, "write": 2, "mix_7r5w": 3,

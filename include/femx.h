@@ -96,6 +96,20 @@ int femx_default_precision(void);
 femx_status femx_set_default_linear_solve(double tol, int max_iter, int restart);
 void femx_default_linear_solve(double *tol, int *max_iter, int *restart);
 
+/* Device memory cache.  No counterpart in the reference (one solve per
+ * process).  hipFree waits for every stream of the device, so a solver
+ * handle destroyed while other handles run would stall them all; instead
+ * femx_vms_destroy and femx_supg_destroy, after synchronising the handle's
+ * stream, keep its device buffers (and its stream) in a per-device cache of
+ * idle blocks, from which later handles of the same sizes are served.  The
+ * cache holds at most max_bytes_per_device per device (initially the
+ * environment variable FEMX_DEVICE_CACHE_BYTES, else 8 GiB); setting a
+ * smaller value frees the excess now, 0 frees everything and turns the cache
+ * off.  It is also emptied when an allocation runs out of device memory.
+ * femx_device_cache_bytes returns what the cache holds on a device. */
+femx_status femx_set_device_cache(size_t max_bytes_per_device);
+size_t femx_device_cache_bytes(int device);
+
 /* Upload a mesh to the current device: SoA/AoS device layout, node->element
  * gather lists, locality-ordered element blocks.  The descriptor's arrays
  * are copied; the caller keeps ownership. Replaces the per-solve pointer

@@ -3,8 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/femx.h"
@@ -20,6 +23,41 @@ femx_status hip_fail(hipError_t err, const char *what);
     hipError_t _e = (call);                            \
     if (_e != hipSuccess) return femx::hip_fail(_e, #call); \
   } while (0)
+
+// ------------------------------------------------------ host threads
+// f(begin, end) over [0, n) in contiguous ranges on up to 8 host threads.
+// Every range's work is independent of the others, so the results do not
+// depend on the thread count.
+template <class F>
+void parallel_ranges(long n, const F &f) {
+  const long T = std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  if (n < (1L << 16) || T == 1) {
+    f(0L, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (long t = 0; t < T; t++) th.emplace_back([&f, n, t, T] { f(n * t / T, n * (t + 1) / T); });
+  for (auto &x : th) x.join();
+}
+
+// f(k) for every k in [0, n) on up to 8 host threads, items taken in order
+// from a shared counter.  Every item writes only its own outputs, so the
+// results do not depend on the thread count or the schedule.
+template <class F>
+void parallel_items(long n, const F &f) {
+  const long T = std::min<long>(std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8), n);
+  if (T <= 1) {
+    for (long k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::atomic<long> next{0};
+  std::vector<std::thread> th;
+  for (long t = 0; t < T; t++)
+    th.emplace_back([&] {
+      for (long k; (k = next.fetch_add(1)) < n;) f(k);
+    });
+  for (auto &x : th) x.join();
+}
 
 // ------------------------------------------------------- device memory
 // hipFree waits for every stream of the device, so freeing a finished
@@ -90,7 +128,8 @@ struct DevBuf {
 //
 // Element blocks (the unit of one workgroup of the element kernels):
 // elements are renumbered along a Morton curve of their centroids and cut
-// greedily into blocks of at most B elements touching at most 256 nodes.
+// greedily into blocks of at most B elements touching at most 256 nodes (the
+// curve in ranges of 2^17 elements, each range cut on its own).
 //   perm      int[E]        block order -> original element id
 //   elem_off  int[nb+1]     element range of each block (block order)
 //   node_off  int[nb+1]     range of each block's local nodes in loc_node
@@ -133,6 +172,10 @@ struct Mesh {
   int device = 0;
 };
 
+// Corners (e * npe + a) of every node in ascending order, from the flat
+// connectivity: off [N+1] and, if slot is given, slot [npe*E].  Returns the
+// largest node degree.
+int node_csr(const std::vector<int> &conn, long N, std::vector<int> &off, std::vector<int> *slot);
 // Builds m.blk on the host (needs h_coords, h_conn, the global CSR) and uploads it.
 // elem_group (optional): group id per element; blocks never mix groups and
 // each group is Morton-ordered in its own bounding box (batches of meshes).

@@ -239,56 +239,88 @@ static inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
 }
 
 namespace {
-// f(begin, end) over [0, n) in contiguous ranges on up to 8 host threads.
-// Every range's work is independent of the others, so the results do not
-// depend on the thread count.
-template <class F>
-void parallel_ranges(long n, const F &f) {
-  const long T = std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
-  if (n < (1L << 16) || T == 1) {
-    f(0L, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (long t = 0; t < T; t++) th.emplace_back([&f, n, t, T] { f(n * t / T, n * (t + 1) / T); });
-  for (auto &x : th) x.join();
-}
-
-// Sorts idx by (key, idx): sorted runs on up to 8 host threads, then
-// pairwise merges.  (key, idx) pairs are unique, so the order does not
-// depend on the thread count.
-void sort_by_key(std::vector<uint64_t> &key, std::vector<int> &idx) {
-  const long n = (long)key.size();
-  struct KI {
-    uint64_t k;
-    int i;
-    bool operator<(const KI &o) const { return k != o.k ? k < o.k : i < o.i; }
+// order = element ids sorted by (code, id).  Elements go to 2^16 buckets by
+// the top 16 bits of their 63-bit code, stably in id order (per-thread
+// histograms over contiguous id ranges), then each bucket is sorted by
+// (code, id).
+void morton_order(const std::vector<uint64_t> &code, std::vector<int> &order) {
+  const long E = (long)code.size();
+  constexpr int NBK = 1 << 16, SHIFT = 63 - 16;
+  const long T = E < (1L << 16) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  std::vector<int> pos((size_t)T * NBK, 0);
+  auto each_range = [&](const auto &fn) {
+    std::vector<std::thread> th;
+    for (long t = 0; t < T; t++) th.emplace_back([&, t] { fn(t, E * t / T, E * (t + 1) / T); });
+    for (auto &x : th) x.join();
   };
-  std::vector<KI> a(n), t(n);
-  for (long x = 0; x < n; x++) a[x] = {key[x], idx[x]};
-  const long T = n < (1L << 16) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
-  std::vector<long> cut(T + 1);
-  for (long r = 0; r <= T; r++) cut[r] = n * r / T;
-  {
-    std::vector<std::thread> th;
-    for (long r = 0; r < T; r++) th.emplace_back([&, r] { std::sort(a.begin() + cut[r], a.begin() + cut[r + 1]); });
-    for (auto &x : th) x.join();
-  }
-  for (long w = 1; w < T; w *= 2) {
-    std::vector<std::thread> th;
-    for (long r = 0; r < T; r += 2 * w) {
-      const long l = cut[r], m = cut[std::min(r + w, T)], h = cut[std::min(r + 2 * w, T)];
-      th.emplace_back([&, l, m, h] { std::merge(a.begin() + l, a.begin() + m, a.begin() + m, a.begin() + h, t.begin() + l); });
+  each_range([&](long t, long e0, long e1) {
+    int *h = pos.data() + t * NBK;
+    for (long e = e0; e < e1; e++) h[code[e] >> SHIFT]++;
+  });
+  // bucket-major, then thread: equal buckets keep ascending ids
+  std::vector<int> bstart(NBK + 1);
+  int s = 0;
+  for (int k = 0; k < NBK; k++) {
+    bstart[k] = s;
+    for (long t = 0; t < T; t++) {
+      const int c = pos[t * NBK + k];
+      pos[t * NBK + k] = s;
+      s += c;
     }
-    for (auto &x : th) x.join();
-    a.swap(t);
   }
-  for (long x = 0; x < n; x++) {
-    key[x] = a[x].k;
-    idx[x] = a[x].i;
-  }
+  bstart[NBK] = s;
+  order.resize(E);
+  each_range([&](long t, long e0, long e1) {
+    int *h = pos.data() + t * NBK;
+    for (long e = e0; e < e1; e++) order[h[code[e] >> SHIFT]++] = (int)e;
+  });
+  constexpr int CHUNK = 256;
+  parallel_items(NBK / CHUNK, [&](long c) {
+    for (int k = (int)c * CHUNK; k < (int)(c + 1) * CHUNK; k++)
+      if (bstart[k + 1] - bstart[k] > 1)
+        std::sort(order.begin() + bstart[k], order.begin() + bstart[k + 1],
+                  [&](int x, int y) { return code[x] != code[y] ? code[x] < code[y] : x < y; });
+  });
 }
 }  // namespace
+
+int node_csr(const std::vector<int> &conn, long N, std::vector<int> &off, std::vector<int> *slot) {
+  const long C = (long)conn.size();
+  const long T = C < (1L << 18) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  // per thread (contiguous ranges of corners) and node: the count, then the
+  // first position of the thread's corners of the node
+  std::vector<int> pos((size_t)T * N, 0);
+  auto each_range = [&](const auto &fn) {
+    if (T == 1) return fn(0L, 0L, C);
+    std::vector<std::thread> th;
+    for (long t = 0; t < T; t++) th.emplace_back([&, t] { fn(t, C * t / T, C * (t + 1) / T); });
+    for (auto &x : th) x.join();
+  };
+  each_range([&](long t, long i0, long i1) {
+    int *p = pos.data() + t * N;
+    for (long i = i0; i < i1; i++) p[conn[i]]++;
+  });
+  off.assign(N + 1, 0);
+  int s = 0, max_deg = 0;
+  for (long n = 0; n < N; n++) {
+    off[n] = s;
+    for (long t = 0; t < T; t++) {
+      const int c = pos[t * N + n];
+      pos[t * N + n] = s;
+      s += c;
+    }
+    max_deg = std::max(max_deg, s - off[n]);
+  }
+  off[N] = s;
+  if (slot) {
+    slot->resize(C);
+    each_range([&](long t, long i0, long i1) {
+      int *p = pos.data() + t * N;
+      for (long i = i0; i < i1; i++) (*slot)[p[conn[i]]++] = (int)i;
+    });
+  }
+  return max_deg;
+}
 
 void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
   Blocks &b = m.blk;
@@ -298,26 +330,35 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   // of meshes: blocks never mix meshes, each mesh gets its own bounding box)
   const int ng = elem_group ? 1 + *std::max_element(elem_group->begin(), elem_group->end()) : 1;
   std::vector<double> lo(3 * ng, 1e300), hi(3 * ng, -1e300);
-  std::mutex box_mu;
-  parallel_ranges(E, [&](long e0, long e1) {
-    std::vector<double> l(3 * ng, 1e300), h(3 * ng, -1e300);
-    for (long e = e0; e < e1; e++) {
-      const int gi = elem_group ? (*elem_group)[e] : 0;
-      for (int a = 0; a < npe; a++)
+  if (!elem_group) {
+    // the box of the nodes of some element
+    for (long n = 0; n < m.N; n++)
+      if (g_off[n + 1] > g_off[n])
         for (int k = 0; k < 3; k++) {
-          const double x = m.h_coords[3L * m.h_conn[e * npe + a] + k];
-          l[3 * gi + k] = std::min(l[3 * gi + k], x);
-          h[3 * gi + k] = std::max(h[3 * gi + k], x);
+          lo[k] = std::min(lo[k], m.h_coords[3 * n + k]);
+          hi[k] = std::max(hi[k], m.h_coords[3 * n + k]);
         }
-    }
-    std::lock_guard<std::mutex> lk(box_mu);
-    for (int i = 0; i < 3 * ng; i++) {
-      lo[i] = std::min(lo[i], l[i]);
-      hi[i] = std::max(hi[i], h[i]);
-    }
-  });
+  } else {
+    std::mutex box_mu;
+    parallel_ranges(E, [&](long e0, long e1) {
+      std::vector<double> l(3 * ng, 1e300), h(3 * ng, -1e300);
+      for (long e = e0; e < e1; e++) {
+        const int gi = (*elem_group)[e];
+        for (int a = 0; a < npe; a++)
+          for (int k = 0; k < 3; k++) {
+            const double x = m.h_coords[3L * m.h_conn[e * npe + a] + k];
+            l[3 * gi + k] = std::min(l[3 * gi + k], x);
+            h[3 * gi + k] = std::max(h[3 * gi + k], x);
+          }
+      }
+      std::lock_guard<std::mutex> lk(box_mu);
+      for (int i = 0; i < 3 * ng; i++) {
+        lo[i] = std::min(lo[i], l[i]);
+        hi[i] = std::max(hi[i], h[i]);
+      }
+    });
+  }
   std::vector<uint64_t> code(E);
-  std::vector<int> order(E);
   parallel_ranges(E, [&](long e0, long e1) {
     for (long e = e0; e < e1; e++) {
       const int gi = elem_group ? (*elem_group)[e] : 0;
@@ -332,11 +373,11 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
         cd |= spread3(q) << k;
       }
       code[e] = cd;
-      order[e] = (int)e;
     }
   });
   // by (group, code, element): by (code, element), then stable by group
-  sort_by_key(code, order);
+  std::vector<int> order;
+  morton_order(code, order);
   code = std::vector<uint64_t>();
   if (elem_group) {
     std::vector<long> cnt(ng + 1, 0);
@@ -347,89 +388,134 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
     order.swap(o2);
   }
   b.perm.swap(order);
-  // the connectivity in block order, read sequentially below
-  std::vector<int> sconn(E * npe);
-  parallel_ranges(E, [&](long i0, long i1) {
-    for (long i = i0; i < i1; i++)
-      for (int a = 0; a < npe; a++) sconn[i * npe + a] = m.h_conn[(long)b.perm[i] * npe + a];
-  });
   // greedy blocks: <= B elements, <= 256 local nodes, and at most
   // MAX_NODE_CORNERS corners of one node (so a corner's rank among its
-  // node's corners in the block fits 7 bits)
+  // node's corners in the block fits 7 bits).  The Morton order is cut into
+  // ranges of RANGE elements, each cut greedily on its own (a range starts a
+  // new block), with the current block's nodes in a small hash set.
   constexpr int MAX_NODE_CORNERS = 128;
-  std::vector<int> stamp(m.N, -1), local_id(m.N, 0), corners(m.N, 0);
+  constexpr long RANGE = 1L << 17;
+  // ranges start at every group's first element and every RANGE elements
+  // after it, so a mesh of a batch is cut as it is on its own
+  std::vector<long> rstart;
+  for (long g0 = 0; g0 < E;) {
+    long g1 = g0 + 1;
+    if (elem_group) {
+      const int gi = (*elem_group)[b.perm[g0]];
+      while (g1 < E && (*elem_group)[b.perm[g1]] == gi) g1++;
+    } else {
+      g1 = E;
+    }
+    for (long r0 = g0; r0 < g1; r0 += RANGE) rstart.push_back(r0);
+    g0 = g1;
+  }
+  rstart.push_back(E);
+  const long nr = (long)rstart.size() - 1;
+  struct RangeBlocks {
+    std::vector<int> elem_end, n_local, loc_node;
+    int max_local = 0;
+  };
+  std::vector<RangeBlocks> rb(nr);
+  b.lconn.assign(E * npe, 0);
+  parallel_items(nr, [&](long r) {
+    const long i0 = rstart[r], i1 = std::min(rstart[r + 1], i0 + RANGE);
+    RangeBlocks &o = rb[r];
+    constexpr int HS = 1024;  // > 4 x the 256 nodes of a block
+    std::vector<int> hkey(HS, -1), hval(HS, 0), hcnt(HS, 0), used, cur;
+    auto find = [&](int n) {
+      unsigned h = ((unsigned)n * 2654435761u) >> 22;
+      while (hkey[h] != -1 && hkey[h] != n) h = (h + 1) & (HS - 1);
+      return (int)h;
+    };
+    long e_begin = i0;
+    auto close_block = [&](long e_end) {
+      std::sort(cur.begin(), cur.end());
+      for (size_t j = 0; j < cur.size(); j++) hval[find(cur[j])] = (int)j;
+      for (long i = e_begin; i < e_end; i++) {
+        const int *c = m.h_conn.data() + (long)b.perm[i] * npe;
+        for (int a = 0; a < npe; a++) b.lconn[i * npe + a] = (uint8_t)hval[find(c[a])];
+      }
+      o.loc_node.insert(o.loc_node.end(), cur.begin(), cur.end());
+      o.max_local = std::max(o.max_local, (int)cur.size());
+      o.elem_end.push_back((int)e_end);
+      o.n_local.push_back((int)cur.size());
+      for (int h : used) hkey[h] = -1;
+      used.clear();
+      cur.clear();
+      e_begin = e_end;
+    };
+    for (long i = i0; i < i1; i++) {
+      const int eo = b.perm[i];
+      const int *c = m.h_conn.data() + (long)eo * npe;
+      int fresh = 0, most = 0;
+      for (int a = 0; a < npe; a++) {
+        const int h = find(c[a]);
+        if (hkey[h] == -1) fresh++;
+        else most = std::max(most, hcnt[h]);
+      }
+      const bool new_group = elem_group && (*elem_group)[eo] != (*elem_group)[b.perm[e_begin]];
+      if (i > e_begin && (i - e_begin >= B || (int)cur.size() + fresh > 256 || new_group ||
+                          most + npe > MAX_NODE_CORNERS))
+        close_block(i);
+      for (int a = 0; a < npe; a++) {
+        const int n = c[a], h = find(n);
+        if (hkey[h] == -1) {
+          hkey[h] = n;
+          hcnt[h] = 0;
+          used.push_back(h);
+          cur.push_back(n);
+        }
+        hcnt[h]++;
+      }
+    }
+    close_block(i1);
+  });
   b.elem_off.assign(1, 0);
   b.node_off.assign(1, 0);
   b.loc_node.clear();
-  b.lconn.assign(E * npe, 0);
-  std::vector<int> cur_nodes;
-  int blk = 0;
-  long e_begin = 0;
-  auto close_block = [&](long e_end) {
-    std::sort(cur_nodes.begin(), cur_nodes.end());
-    for (size_t j = 0; j < cur_nodes.size(); j++) local_id[cur_nodes[j]] = (int)j;
-    for (long i = e_begin; i < e_end; i++)
-      for (int a = 0; a < npe; a++) b.lconn[i * npe + a] = (uint8_t)local_id[sconn[i * npe + a]];
-    b.loc_node.insert(b.loc_node.end(), cur_nodes.begin(), cur_nodes.end());
-    b.max_local = std::max(b.max_local, (int)cur_nodes.size());
-    b.elem_off.push_back((int)e_end);
-    b.node_off.push_back((int)b.loc_node.size());
-    cur_nodes.clear();
-    blk++;
-    e_begin = e_end;
-  };
-  for (long i = 0; i < E; i++) {
-    const int eo = b.perm[i];
-    int fresh = 0, most = 0;
-    for (int a = 0; a < npe; a++) {
-      const int n = sconn[i * npe + a];
-      fresh += stamp[n] != blk;
-      if (stamp[n] == blk) most = std::max(most, corners[n]);
+  b.max_local = 0;
+  for (const RangeBlocks &o : rb) {
+    for (size_t k = 0; k < o.elem_end.size(); k++) {
+      b.elem_off.push_back(o.elem_end[k]);
+      b.node_off.push_back(b.node_off.back() + o.n_local[k]);
     }
-    const bool new_group = elem_group && (*elem_group)[eo] != (*elem_group)[b.perm[e_begin]];
-    if (i > e_begin && (i - e_begin >= B || (int)cur_nodes.size() + fresh > 256 || new_group ||
-                        most + npe > MAX_NODE_CORNERS))
-      close_block(i);
-    for (int a = 0; a < npe; a++) {
-      const int n = sconn[i * npe + a];
-      if (stamp[n] != blk) {
-        stamp[n] = blk;
-        corners[n] = 0;
-        cur_nodes.push_back(n);
-      }
-      corners[n]++;
-    }
+    b.loc_node.insert(b.loc_node.end(), o.loc_node.begin(), o.loc_node.end());
+    b.max_local = std::max(b.max_local, o.max_local);
   }
-  close_block(E);
-  b.nb = blk;
+  rb = std::vector<RangeBlocks>();
+  b.nb = (int)b.elem_off.size() - 1;
   const long L = (long)b.loc_node.size();
   b.total_local = (int)L;
-  // local gather lists and completeness
+  // local gather lists and completeness, 64 blocks per work item
   b.lcsr_off.assign(L + 1, 0);
   b.lcsr.assign(E * npe, 0);
   b.loc_pslot.assign(L, -1);
-  std::vector<int> cnt;
-  for (int k = 0; k < b.nb; k++) {
-    const int n0 = b.node_off[k], nn = b.node_off[k + 1] - n0, e0 = b.elem_off[k], e1 = b.elem_off[k + 1];
-    cnt.assign(nn + 1, 0);
-    for (long i = e0; i < e1; i++)
-      for (int a = 0; a < npe; a++) cnt[b.lconn[i * npe + a] + 1]++;
-    for (int j = 0; j < nn; j++) cnt[j + 1] += cnt[j];
-    for (int j = 0; j <= nn; j++) b.lcsr_off[n0 + j] = (int)(e0 * npe) + cnt[j];
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (long i = e0; i < e1; i++)
-      for (int a = 0; a < npe; a++) {
-        const int j = b.lconn[i * npe + a];
-        // position of this corner's contribution in the block's node-sorted order
-        b.lcsr[i * npe + a] = (uint16_t)(fill[j]++);
+  constexpr int BCHUNK = 64;
+  parallel_items((b.nb + BCHUNK - 1) / BCHUNK, [&](long c) {
+    std::vector<int> cnt, fill;
+    for (int k = (int)c * BCHUNK; k < std::min(b.nb, (int)(c + 1) * BCHUNK); k++) {
+      const int n0 = b.node_off[k], nn = b.node_off[k + 1] - n0, e0 = b.elem_off[k], e1 = b.elem_off[k + 1];
+      cnt.assign(nn + 1, 0);
+      for (long i = e0; i < e1; i++)
+        for (int a = 0; a < npe; a++) cnt[b.lconn[i * npe + a] + 1]++;
+      for (int j = 0; j < nn; j++) cnt[j + 1] += cnt[j];
+      for (int j = 0; j < nn; j++) b.lcsr_off[n0 + j] = (int)(e0 * npe) + cnt[j];
+      fill.assign(cnt.begin(), cnt.end() - 1);
+      for (long i = e0; i < e1; i++)
+        for (int a = 0; a < npe; a++) {
+          const int j = b.lconn[i * npe + a];
+          // position of this corner's contribution in the block's node-sorted order
+          b.lcsr[i * npe + a] = (uint16_t)(fill[j]++);
+        }
+      for (int j = 0; j < nn; j++) {
+        const int g = b.loc_node[n0 + j];
+        const int local_deg = cnt[j + 1] - cnt[j];
+        const int global_deg = g_off[g + 1] - g_off[g];
+        b.loc_pslot[n0 + j] = local_deg == global_deg ? -1 : -2;  // -2: shared, slot assigned below
       }
-    for (int j = 0; j < nn; j++) {
-      const int g = b.loc_node[n0 + j];
-      const int local_deg = cnt[j + 1] - cnt[j];
-      const int global_deg = g_off[g + 1] - g_off[g];
-      b.loc_pslot[n0 + j] = local_deg == global_deg ? -1 : -2;  // -2: shared, slot assigned below
     }
-  }
+  });
+  b.lcsr_off[L] = (int)(E * npe);
   // shared nodes: partial slots grouped per node, in block order
   std::vector<int> nparts(m.N, 0);
   for (long l = 0; l < L; l++)
@@ -838,14 +924,8 @@ femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, lon
   m.h_coords.assign(coords, coords + 3 * N);
   m.h_conn.assign(conn, conn + npe * E);
   // node -> (element, local node) gather lists in ascending element order
-  std::vector<int> off(N + 1, 0), slot(npe * E);
-  for (long i = 0; i < npe * E; i++) off[m.h_conn[i] + 1]++;
-  for (long n = 0; n < N; n++) {
-    m.max_deg = std::max(m.max_deg, off[n + 1]);
-    off[n + 1] += off[n];
-  }
-  std::vector<int> fill(off.begin(), off.end() - 1);
-  for (long i = 0; i < npe * E; i++) slot[fill[m.h_conn[i]]++] = (int)i;  // i = e*npe + a, ascending
+  std::vector<int> off, slot;
+  m.max_deg = node_csr(m.h_conn, N, off, &slot);
   std::vector<double4> xyz(N);
   for (long n = 0; n < N; n++) xyz[n] = make_double4(coords[3 * n], coords[3 * n + 1], coords[3 * n + 2], 0.0);
   hipStream_t st = nullptr;
@@ -895,9 +975,8 @@ extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_
   m.npe = (int)npe;
   m.h_coords.assign(d->coords, d->coords + 3 * N);
   m.h_conn.assign(d->conn, d->conn + npe * E);
-  std::vector<int> off(N + 1, 0);
-  for (long i = 0; i < npe * E; i++) off[m.h_conn[i] + 1]++;
-  for (long n = 0; n < N; n++) off[n + 1] += off[n];
+  std::vector<int> off;
+  node_csr(m.h_conn, N, off, nullptr);
   build_blocks_host(m, off, block_elems);
   const Blocks &b = m.blk;
   long long complete = 0, maxe = 0;

@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "femx_common.h"
 
@@ -1373,34 +1374,50 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
     std::vector<int> slot0(b.nb + 1, 0), first_slot(m.N, -1);
     for (int si = 0; si < b.n_sh; si++)
       if (b.sh_off[si + 1] > b.sh_off[si]) first_slot[b.sh_node[si]] = b.sh_slot[b.sh_off[si]];
-    int slot = 0;
+    // the block's first partial slot: slots are numbered block by block
     for (int blk = 0; blk < b.nb; blk++) {
-      const int n0 = b.node_off[blk], cbase = 4 * b.elem_off[blk];
-      slot0[blk] = slot;
-      for (int l = n0; l < b.node_off[blk + 1]; l++) {
-        const int g = b.loc_node[l], ps = b.loc_pslot[l], c0 = b.lcsr_off[l] - cbase;
-        if (c0 < 0 || c0 > LN_BEGIN) return fail((set_error("block contribution runs overflow 10 bits"), FEMX_ERR_ARG));
-        int meta = c0 | (has_rec[g] ? LN_BC : 0);
-        if (ps >= 0) {
-          if (ps - slot0[blk] > LN_RANK || ps != slot) return fail((set_error("block partial slots out of order"), FEMX_ERR_ARG));
-          meta |= LN_SHARED | ((ps - slot0[blk]) << LN_RANK_SHIFT) | (ps == first_slot[g] ? LN_FIRST : 0);
-          slot++;
-        }
-        ln[l] = make_int2(g, meta);
-      }
-      for (int e = b.elem_off[blk]; e < b.elem_off[blk + 1]; e++) {
-        unsigned short c[4];
-        for (int a = 0; a < 4; a++) {
-          const int j = b.lconn[4L * e + a], l = n0 + j, g = b.loc_node[l];
-          const int rank = (int)b.lcsr[4L * e + a] - (b.lcsr_off[l] - cbase);
-          if (j > EC_ID || rank < 0 || rank > EC_RANK)
-            return fail((set_error("a node has more than 128 elements in one block"), FEMX_ERR_ARG));
-          c[a] = (unsigned short)(j | (rank << EC_RANK_SHIFT) | (has_rec[g] ? BC_BIT : 0));
-        }
-        ecr[e] = make_ushort4(c[0], c[1], c[2], c[3]);
-      }
+      int c = 0;
+      for (int l = b.node_off[blk]; l < b.node_off[blk + 1]; l++) c += b.loc_pslot[l] >= 0;
+      slot0[blk + 1] = slot0[blk] + c;
     }
-    slot0[b.nb] = slot;
+    // records of 64 blocks per work item; an error ends the work and the
+    // first one flagged is reported
+    std::atomic<int> bad{0};
+    auto flag = [&](int code) {
+      int none = 0;
+      bad.compare_exchange_strong(none, code);
+    };
+    constexpr int CH = 64;
+    parallel_items((b.nb + CH - 1) / CH, [&](long ch) {
+      for (int blk = (int)ch * CH; blk < std::min(b.nb, (int)(ch + 1) * CH) && !bad.load(); blk++) {
+        const int n0 = b.node_off[blk], cbase = 4 * b.elem_off[blk];
+        int slot = slot0[blk];
+        for (int l = n0; l < b.node_off[blk + 1]; l++) {
+          const int g = b.loc_node[l], ps = b.loc_pslot[l], c0 = b.lcsr_off[l] - cbase;
+          if (c0 < 0 || c0 > LN_BEGIN) return flag(1);
+          int meta = c0 | (has_rec[g] ? LN_BC : 0);
+          if (ps >= 0) {
+            if (ps - slot0[blk] > LN_RANK || ps != slot) return flag(2);
+            meta |= LN_SHARED | ((ps - slot0[blk]) << LN_RANK_SHIFT) | (ps == first_slot[g] ? LN_FIRST : 0);
+            slot++;
+          }
+          ln[l] = make_int2(g, meta);
+        }
+        for (int e = b.elem_off[blk]; e < b.elem_off[blk + 1]; e++) {
+          unsigned short c[4];
+          for (int a = 0; a < 4; a++) {
+            const int j = b.lconn[4L * e + a], l = n0 + j, g = b.loc_node[l];
+            const int rank = (int)b.lcsr[4L * e + a] - (b.lcsr_off[l] - cbase);
+            if (j > EC_ID || rank < 0 || rank > EC_RANK) return flag(3);
+            c[a] = (unsigned short)(j | (rank << EC_RANK_SHIFT) | (has_rec[g] ? BC_BIT : 0));
+          }
+          ecr[e] = make_ushort4(c[0], c[1], c[2], c[3]);
+        }
+      }
+    });
+    if (bad.load() == 1) return fail((set_error("block contribution runs overflow 10 bits"), FEMX_ERR_ARG));
+    if (bad.load() == 2) return fail((set_error("block partial slots out of order"), FEMX_ERR_ARG));
+    if (bad.load() == 3) return fail((set_error("a node has more than 128 elements in one block"), FEMX_ERR_ARG));
     if ((r = s->ecorn.upload(ecr.data(), ecr.size(), s->st)) != FEMX_OK) return fail(r);
     if ((r = s->lnode.upload(ln.data(), ln.size(), s->st)) != FEMX_OK) return fail(r);
     if ((r = s->slot0.upload(slot0.data(), slot0.size(), s->st)) != FEMX_OK) return fail(r);
@@ -1428,20 +1445,27 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
                                                                   s->bc.node_rec.p, s->sh_im.p, s->sh_rec.p);
   }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
-  // min element length (:947-952), for the CFL print
+  // min element length (:947-952), for the CFL print; the minimum does not
+  // depend on the order, so the elements are split over host threads
   double mh_ = 1.79769313486231570815e+308;
-  for (int e = 0; e < m.E; e++) {
-    const double *x[4];
-    for (int k = 0; k < 4; k++) x[k] = &m.h_coords[3L * m.h_conn[4L * e + k]];
-    double j[3][3];
-    for (int i = 0; i < 3; i++)
-      for (int c = 0; c < 3; c++) j[i][c] = x[c + 1][i] - x[0][i];
-    const double det = j[0][0] * (j[1][1] * j[2][2] - j[2][1] * j[1][2]) +
-                       j[0][1] * (j[2][0] * j[1][2] - j[1][0] * j[2][2]) +
-                       j[0][2] * (j[1][0] * j[2][1] - j[2][0] * j[1][1]);
-    const double hh = std::pow(6.0 * (det / 6.0), 1.0 / 3.0);
-    if (hh < mh_) mh_ = hh;
-  }
+  std::mutex mh_mu;
+  parallel_ranges(m.E, [&](long e0, long e1) {
+    double mr = 1.79769313486231570815e+308;
+    for (long e = e0; e < e1; e++) {
+      const double *x[4];
+      for (int k = 0; k < 4; k++) x[k] = &m.h_coords[3L * m.h_conn[4L * e + k]];
+      double j[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) j[i][c] = x[c + 1][i] - x[0][i];
+      const double det = j[0][0] * (j[1][1] * j[2][2] - j[2][1] * j[1][2]) +
+                         j[0][1] * (j[2][0] * j[1][2] - j[1][0] * j[2][2]) +
+                         j[0][2] * (j[1][0] * j[2][1] - j[2][0] * j[1][1]);
+      const double hh = std::pow(6.0 * (det / 6.0), 1.0 / 3.0);
+      if (hh < mr) mr = hh;
+    }
+    std::lock_guard<std::mutex> lk(mh_mu);
+    if (mr < mh_) mh_ = mr;
+  });
   s->min_h = mh_;
   *out = s;
   return FEMX_OK;

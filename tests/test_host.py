@@ -63,10 +63,14 @@ def test_generators():
     assert np.all(g.coords[ch.pres_node, 0] == 1.0)
 
 
-@pytest.mark.parametrize("kind,n,B", [("tet", 6, 256), ("tet", 9, 512), ("hex", 7, 128)])
+@pytest.mark.parametrize("kind,n,B", [("tet", 6, 256), ("tet", 9, 512), ("hex", 7, 128), ("tet", 36, 256),
+                                       ("hex", 60, 128)])
 def test_block_partition_invariants(kind, n, B):
+    """The last two meshes (279,936 tets, 216,000 hexa) span several of the
+    builder's independently cut ranges of 2^17 elements."""
     m = meshgen.tet_cube(n, 0.1, 1) if kind == "tet" else meshgen.hex_cube(n, 0.1, 1)
     s = femx.block_stats(m, B)
+    assert femx.block_stats(m, B) == s  # deterministic (threads take ranges in any order)
     deg = np.bincount(m.conn.ravel(), minlength=m.n_nodes)
     assert s["elements"] == m.n_elems
     assert s["max_local"] <= 256 and s["max_elems"] <= B

@@ -25,38 +25,47 @@ femx_status hip_fail(hipError_t err, const char *what);
   } while (0)
 
 // ------------------------------------------------------ host threads
-// f(begin, end) over [0, n) in contiguous ranges on up to 8 host threads.
-// Every range's work is independent of the others, so the results do not
-// depend on the thread count.
+// Helper threads the library's parallel loops may run at once, over every
+// thread that calls them (a solve server prepares several meshes at a time):
+// a loop takes up to 7 helpers from a process-wide budget of
+// hardware_concurrency() - 2 and runs its remaining share on the calling
+// thread, alone when no helper is free.
+int take_helpers(int want);
+void give_helpers(int n);
+
+// f(begin, end) over [0, n) in contiguous ranges on the caller and its
+// helpers.  Every range's work is independent of the others, so the results
+// do not depend on the number of ranges.
 template <class F>
 void parallel_ranges(long n, const F &f) {
-  const long T = std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
-  if (n < (1L << 16) || T == 1) {
+  const int h = n < (1L << 16) ? 0 : take_helpers(7);
+  if (h == 0) {
     f(0L, n);
     return;
   }
+  const long T = h + 1;
   std::vector<std::thread> th;
-  for (long t = 0; t < T; t++) th.emplace_back([&f, n, t, T] { f(n * t / T, n * (t + 1) / T); });
+  for (long t = 1; t < T; t++) th.emplace_back([&f, n, t, T] { f(n * t / T, n * (t + 1) / T); });
+  f(0L, n / T);
   for (auto &x : th) x.join();
+  give_helpers(h);
 }
 
-// f(k) for every k in [0, n) on up to 8 host threads, items taken in order
-// from a shared counter.  Every item writes only its own outputs, so the
-// results do not depend on the thread count or the schedule.
+// f(k) for every k in [0, n), items taken in order from a shared counter by
+// the caller and its helpers (none unless big).  Every item writes only its
+// own outputs, so the results do not depend on the schedule.
 template <class F>
-void parallel_items(long n, const F &f) {
-  const long T = std::min<long>(std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8), n);
-  if (T <= 1) {
-    for (long k = 0; k < n; k++) f(k);
-    return;
-  }
+void parallel_items(long n, const F &f, bool big = true) {
+  const int h = big && n > 1 ? take_helpers((int)std::min<long>(7, n - 1)) : 0;
   std::atomic<long> next{0};
+  auto run = [&] {
+    for (long k; (k = next.fetch_add(1)) < n;) f(k);
+  };
   std::vector<std::thread> th;
-  for (long t = 0; t < T; t++)
-    th.emplace_back([&] {
-      for (long k; (k = next.fetch_add(1)) < n;) f(k);
-    });
+  for (int t = 0; t < h; t++) th.emplace_back(run);
+  run();
   for (auto &x : th) x.join();
+  if (h) give_helpers(h);
 }
 
 // ------------------------------------------------------- device memory
@@ -151,6 +160,7 @@ struct DevBuf {
 //                           has fewer; 0 when it has none)
 struct Blocks {
   int B = 0, nb = 0, max_local = 0, n_sh = 0, total_local = 0;
+  bool elem_order = false;  // blocks are runs of consecutive element ids (perm = identity)
   std::vector<int> perm, elem_off, node_off, loc_node, loc_pslot, lcsr_off, sh_node, sh_off, sh_slot;
   std::vector<int4> sh_first4;
   std::vector<uint16_t> lcsr;
@@ -179,13 +189,17 @@ int node_csr(const std::vector<int> &conn, long N, std::vector<int> &off, std::v
 // Builds m.blk on the host (needs h_coords, h_conn, the global CSR) and uploads it.
 // elem_group (optional): group id per element; blocks never mix groups and
 // each group is Morton-ordered in its own bounding box (batches of meshes).
-void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr);
-femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr);
+// order FEMX_BLOCKS_ELEMENT_ORDER (one mesh only): blocks are runs of
+// consecutive element ids instead of Morton-ordered.
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr,
+                       int order = FEMX_BLOCKS_MORTON);
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group = nullptr,
+                         int order = FEMX_BLOCKS_MORTON);
 // Fills m from host arrays (coordinates [N][3], 0-based connectivity
 // [E][npe]) and uploads it: coordinates, connectivity, node->element CSR,
 // element blocks of at most B elements (0: 256 for tet4, 128 for hexa8).
 femx_status mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
-                       const std::vector<int> *elem_group = nullptr);
+                       const std::vector<int> *elem_group = nullptr, int order = FEMX_BLOCKS_MORTON);
 
 // --------------------------------------------------- boundary conditions
 // One record per BC-touched node, merging the reference's three lists with

@@ -24,6 +24,22 @@ femx_status hip_fail(hipError_t err, const char *what) {
 
 int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
 
+// ------------------------------------------------ host helper threads
+namespace {
+std::atomic<int> g_free_helpers{std::max(0, (int)std::thread::hardware_concurrency() - 2)};
+}
+
+int take_helpers(int want) {
+  int cur = g_free_helpers.load();
+  while (cur > 0 && want > 0) {
+    const int grant = std::min(cur, want);
+    if (g_free_helpers.compare_exchange_weak(cur, cur - grant)) return grant;
+  }
+  return 0;
+}
+
+void give_helpers(int n) { g_free_helpers += n; }
+
 // ------------------------------------------------ device memory cache
 namespace {
 constexpr int CACHE_DEVICES = 64;
@@ -246,11 +262,13 @@ namespace {
 void morton_order(const std::vector<uint64_t> &code, std::vector<int> &order) {
   const long E = (long)code.size();
   constexpr int NBK = 1 << 16, SHIFT = 63 - 16;
-  const long T = E < (1L << 16) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
+  const int helpers = E < (1L << 16) ? 0 : take_helpers(7);
+  const long T = helpers + 1;
   std::vector<int> pos((size_t)T * NBK, 0);
   auto each_range = [&](const auto &fn) {
     std::vector<std::thread> th;
-    for (long t = 0; t < T; t++) th.emplace_back([&, t] { fn(t, E * t / T, E * (t + 1) / T); });
+    for (long t = 1; t < T; t++) th.emplace_back([&, t] { fn(t, E * t / T, E * (t + 1) / T); });
+    fn(0L, 0L, E / T);
     for (auto &x : th) x.join();
   };
   each_range([&](long t, long e0, long e1) {
@@ -274,26 +292,28 @@ void morton_order(const std::vector<uint64_t> &code, std::vector<int> &order) {
     int *h = pos.data() + t * NBK;
     for (long e = e0; e < e1; e++) order[h[code[e] >> SHIFT]++] = (int)e;
   });
+  if (helpers) give_helpers(helpers);
   constexpr int CHUNK = 256;
   parallel_items(NBK / CHUNK, [&](long c) {
     for (int k = (int)c * CHUNK; k < (int)(c + 1) * CHUNK; k++)
       if (bstart[k + 1] - bstart[k] > 1)
         std::sort(order.begin() + bstart[k], order.begin() + bstart[k + 1],
                   [&](int x, int y) { return code[x] != code[y] ? code[x] < code[y] : x < y; });
-  });
+  }, E >= (1L << 16));
 }
 }  // namespace
 
 int node_csr(const std::vector<int> &conn, long N, std::vector<int> &off, std::vector<int> *slot) {
   const long C = (long)conn.size();
-  const long T = C < (1L << 18) ? 1 : std::min<long>(std::max(1u, std::thread::hardware_concurrency()), 8);
-  // per thread (contiguous ranges of corners) and node: the count, then the
-  // first position of the thread's corners of the node
+  const int helpers = C < (1L << 18) ? 0 : take_helpers(7);
+  const long T = helpers + 1;
+  // per range of corners (contiguous) and node: the count, then the first
+  // position of the range's corners of the node
   std::vector<int> pos((size_t)T * N, 0);
   auto each_range = [&](const auto &fn) {
-    if (T == 1) return fn(0L, 0L, C);
     std::vector<std::thread> th;
-    for (long t = 0; t < T; t++) th.emplace_back([&, t] { fn(t, C * t / T, C * (t + 1) / T); });
+    for (long t = 1; t < T; t++) th.emplace_back([&, t] { fn(t, C * t / T, C * (t + 1) / T); });
+    fn(0L, 0L, C / T);
     for (auto &x : th) x.join();
   };
   each_range([&](long t, long i0, long i1) {
@@ -319,13 +339,20 @@ int node_csr(const std::vector<int> &conn, long N, std::vector<int> &off, std::v
       for (long i = i0; i < i1; i++) (*slot)[p[conn[i]]++] = (int)i;
     });
   }
+  if (helpers) give_helpers(helpers);
   return max_deg;
 }
 
-void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
+void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group, int order) {
   Blocks &b = m.blk;
   const long E = m.E, npe = m.npe;
   b.B = B;
+  b.elem_order = order == FEMX_BLOCKS_ELEMENT_ORDER && !elem_group;
+  if (b.elem_order) {
+    // the mesh's own element order: block k holds a run of consecutive ids
+    b.perm.resize(E);
+    std::iota(b.perm.begin(), b.perm.end(), 0);
+  } else {
   // Morton order of element centroids, inside each element group (a batch
   // of meshes: blocks never mix meshes, each mesh gets its own bounding box)
   const int ng = elem_group ? 1 + *std::max_element(elem_group->begin(), elem_group->end()) : 1;
@@ -388,6 +415,7 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
     order.swap(o2);
   }
   b.perm.swap(order);
+  }
   // greedy blocks: <= B elements, <= 256 local nodes, and at most
   // MAX_NODE_CORNERS corners of one node (so a corner's rank among its
   // node's corners in the block fits 7 bits).  The Morton order is cut into
@@ -514,7 +542,7 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
         b.loc_pslot[n0 + j] = local_deg == global_deg ? -1 : -2;  // -2: shared, slot assigned below
       }
     }
-  });
+  }, E >= (1L << 16));
   b.lcsr_off[L] = (int)(E * npe);
   // shared nodes: partial slots grouped per node, in block order
   std::vector<int> nparts(m.N, 0);
@@ -553,8 +581,8 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
   }
 }
 
-femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group) {
-  build_blocks_host(m, g_off, B, elem_group);
+femx_status build_blocks(Mesh &m, const std::vector<int> &g_off, int B, const std::vector<int> *elem_group, int order) {
+  build_blocks_host(m, g_off, B, elem_group, order);
   Blocks &b = m.blk;
   const long E = m.E;
   const long L = b.total_local;
@@ -916,7 +944,7 @@ extern "C" femx_status femx_valu_probe(int kind, int grid, int iters, int reps, 
 }
 
 femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, long N, long E, int npe, int B,
-                             const std::vector<int> *elem_group) {
+                             const std::vector<int> *elem_group, int order) {
   m.N = (int)N;
   m.E = (int)E;
   m.npe = npe;
@@ -932,25 +960,43 @@ femx_status femx::mesh_setup(Mesh &m, const double *coords, const int *conn, lon
   femx_status s;
   if ((s = m.xyz.upload(xyz.data(), N, st)) != FEMX_OK || (s = m.conn.upload(m.h_conn.data(), npe * E, st)) != FEMX_OK ||
       (s = m.g_off.upload(off.data(), N + 1, st)) != FEMX_OK || (s = m.g_slot.upload(slot.data(), npe * E, st)) != FEMX_OK ||
-      (s = build_blocks(m, off, B > 0 ? B : (npe == 4 ? 256 : 128), elem_group)) != FEMX_OK)
+      (s = build_blocks(m, off, B > 0 ? B : (npe == 4 ? 256 : 128), elem_group, order)) != FEMX_OK)
     return s;
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "mesh upload");
   return FEMX_OK;
 }
 
-extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out) {
+namespace {
+// every corner names a node of the mesh (checked on host threads)
+bool conn_in_range(const femx_mesh_desc *d) {
+  const long N = d->n_nodes, C = (long)d->npe * d->n_elems;
+  std::atomic<bool> ok{true};
+  parallel_ranges(C, [&](long i0, long i1) {
+    for (long i = i0; i < i1; i++)
+      if (d->conn[i] < 0 || d->conn[i] >= N) {
+        ok = false;
+        return;
+      }
+  });
+  return ok;
+}
+}  // namespace
+
+extern "C" femx_status femx_mesh_upload_blocks(const femx_mesh_desc *d, int block_elems, int order, femx_mesh **out) {
   if (!d || !out || !d->coords || !d->conn) return set_error("null argument"), FEMX_ERR_ARG;
   if (d->n_nodes <= 0 || d->n_elems <= 0 || (d->npe != 4 && d->npe != 8))
     return set_error("mesh sizes: need n_nodes > 0, n_elems > 0, npe in {4, 8}"), FEMX_ERR_ARG;
+  if (block_elems < 0 || block_elems > (d->npe == 4 ? 256 : 128) ||
+      (order != FEMX_BLOCKS_MORTON && order != FEMX_BLOCKS_ELEMENT_ORDER))
+    return set_error("block_elems 0 (default) .. 256 (tet4) or 128 (hexa8); order FEMX_BLOCKS_*"), FEMX_ERR_ARG;
   int ndev = femx_device_count();
   if (ndev <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
   const long N = d->n_nodes, E = d->n_elems, npe = d->npe;
   if (npe * E > 2147483647L) return set_error("mesh too large for 32-bit slots"), FEMX_ERR_ARG;
-  for (long i = 0; i < npe * E; i++)
-    if (d->conn[i] < 0 || d->conn[i] >= N) return set_error("connectivity out of range"), FEMX_ERR_ARG;
+  if (!conn_in_range(d)) return set_error("connectivity out of range"), FEMX_ERR_ARG;
   femx_mesh *mh = new femx_mesh();
-  femx_status s = mesh_setup(mh->m, d->coords, d->conn, N, E, (int)npe, 0);
+  femx_status s = mesh_setup(mh->m, d->coords, d->conn, N, E, (int)npe, block_elems, nullptr, order);
   if (s != FEMX_OK) {
     delete mh;
     return s;
@@ -959,16 +1005,21 @@ extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out
   return FEMX_OK;
 }
 
+extern "C" femx_status femx_mesh_upload(const femx_mesh_desc *d, femx_mesh **out) {
+  return femx_mesh_upload_blocks(d, 0, FEMX_BLOCKS_MORTON, out);
+}
+
 // Host-only partition statistics (no device needed): stats[0..7] =
 // blocks, elements, total local nodes, max local nodes, shared nodes,
 // partial slots, complete local nodes, max elements in a block.
-extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_elems, long long *stats) {
+extern "C" femx_status femx_mesh_block_stats_order(const femx_mesh_desc *d, int block_elems, int order,
+                                                   long long *stats) {
   if (!d || !stats || !d->coords || !d->conn || d->n_nodes <= 0 || d->n_elems <= 0 || (d->npe != 4 && d->npe != 8) ||
-      block_elems <= 0 || block_elems * d->npe > 65536)
+      block_elems <= 0 || block_elems * d->npe > 65536 ||
+      (order != FEMX_BLOCKS_MORTON && order != FEMX_BLOCKS_ELEMENT_ORDER))
     return set_error("bad argument"), FEMX_ERR_ARG;
   const long N = d->n_nodes, E = d->n_elems, npe = d->npe;
-  for (long i = 0; i < npe * E; i++)
-    if (d->conn[i] < 0 || d->conn[i] >= N) return set_error("connectivity out of range"), FEMX_ERR_ARG;
+  if (!conn_in_range(d)) return set_error("connectivity out of range"), FEMX_ERR_ARG;
   Mesh m;
   m.N = (int)N;
   m.E = (int)E;
@@ -977,7 +1028,7 @@ extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_
   m.h_conn.assign(d->conn, d->conn + npe * E);
   std::vector<int> off;
   node_csr(m.h_conn, N, off, nullptr);
-  build_blocks_host(m, off, block_elems);
+  build_blocks_host(m, off, block_elems, nullptr, order);
   const Blocks &b = m.blk;
   long long complete = 0, maxe = 0;
   for (int x : b.loc_pslot) complete += x < 0;
@@ -991,6 +1042,10 @@ extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_
   stats[6] = complete;
   stats[7] = maxe;
   return FEMX_OK;
+}
+
+extern "C" femx_status femx_mesh_block_stats(const femx_mesh_desc *d, int block_elems, long long *stats) {
+  return femx_mesh_block_stats_order(d, block_elems, FEMX_BLOCKS_MORTON, stats);
 }
 
 extern "C" femx_status femx_mesh_free(femx_mesh *mesh) {

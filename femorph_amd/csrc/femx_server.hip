@@ -343,7 +343,9 @@ void femx_server::union_loop() {
       }
       changed = true;
     }
-    if (!changed) std::this_thread::yield();
+    // nothing completed or was issued: wait a little instead of spinning,
+    // which would take a core from the preparation threads
+    if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(100));
   }
   (void)hipStreamSynchronize(ust);
   for (auto &c : inflight) {
@@ -420,7 +422,9 @@ void femx_server::gpu_loop(int t) {
       active.erase(active.begin() + i);
       changed = true;
     }
-    if (!changed) std::this_thread::yield();
+    // nothing completed or was issued: wait a little instead of spinning,
+    // which would take a core from the preparation threads
+    if (!changed) std::this_thread::sleep_for(std::chrono::microseconds(100));
   }
 }
 

@@ -1414,7 +1414,7 @@ extern "C" femx_status femx_vms_create(femx_mesh *mh, const femx_bc_desc *bc, co
           ecr[e] = make_ushort4(c[0], c[1], c[2], c[3]);
         }
       }
-    });
+    }, m.E >= (1L << 16));
     if (bad.load() == 1) return fail((set_error("block contribution runs overflow 10 bits"), FEMX_ERR_ARG));
     if (bad.load() == 2) return fail((set_error("block partial slots out of order"), FEMX_ERR_ARG));
     if (bad.load() == 3) return fail((set_error("a node has more than 128 elements in one block"), FEMX_ERR_ARG));

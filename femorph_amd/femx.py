@@ -96,6 +96,8 @@ SIGNATURES = [
     ("femx_mesh_upload", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("femx_mesh_free", C.c_int, [C.c_void_p]),
     ("femx_mesh_block_stats", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_longlong)]),
+    ("femx_mesh_block_stats_order", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
+    ("femx_mesh_upload_blocks", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("femx_vms_create", C.c_int, [C.c_void_p, C.POINTER(BCDesc), C.POINTER(VMSParams), C.POINTER(C.c_void_p)]),
     ("femx_vms_destroy", C.c_int, [C.c_void_p]),
     ("femx_vms_set_state", C.c_int, [C.c_void_p, _dp, _dp]),
@@ -212,13 +214,17 @@ def _p(a, t=_dp):
     return a.ctypes.data_as(t)
 
 
-def block_stats(mesh: Mesh, block_elems: int) -> dict:
-    """Host-only statistics of the element-block partition (no GPU needed)."""
+BLOCKS_MORTON, BLOCKS_ELEMENT_ORDER = 0, 1
+
+
+def block_stats(mesh: Mesh, block_elems: int, order: int = BLOCKS_MORTON) -> dict:
+    """Host-only statistics of the element-block partition (no GPU needed);
+    order BLOCKS_MORTON (femx_mesh_upload's) or BLOCKS_ELEMENT_ORDER."""
     coords = np.ascontiguousarray(mesh.coords, np.float64)
     conn = np.ascontiguousarray(mesh.conn, np.int32)
     d = MeshDesc(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(coords), _p(conn, _ip))
     st = (C.c_longlong * 8)()
-    _check(load().femx_mesh_block_stats(C.byref(d), block_elems, st))
+    _check(load().femx_mesh_block_stats_order(C.byref(d), block_elems, order, st))
     keys = ["blocks", "elements", "local_nodes", "max_local", "shared_nodes", "partials", "complete", "max_elems"]
     return dict(zip(keys, [int(x) for x in st]))
 
@@ -261,16 +267,18 @@ def valu_probe(kind: str, grid: int = 8192, iters: int = 4096, reps: int = 10) -
 
 
 class DeviceMesh:
-    """A mesh uploaded to the GPU (femx_mesh_upload)."""
+    """A mesh uploaded to the GPU (femx_mesh_upload_blocks): element blocks
+    of block_elems (0: the default) in Morton order or in the mesh's element
+    order (BLOCKS_ELEMENT_ORDER)."""
 
-    def __init__(self, mesh: Mesh):
+    def __init__(self, mesh: Mesh, block_elems: int = 0, order: int = BLOCKS_MORTON):
         L = load()
         self.mesh = mesh
         self._coords = np.ascontiguousarray(mesh.coords, np.float64)
         self._conn = np.ascontiguousarray(mesh.conn, np.int32)
         d = MeshDesc(mesh.n_nodes, mesh.n_elems, mesh.npe, _p(self._coords), _p(self._conn, _ip))
         h = C.c_void_p()
-        _check(L.femx_mesh_upload(C.byref(d), C.byref(h)))
+        _check(L.femx_mesh_upload_blocks(C.byref(d), int(block_elems), int(order), C.byref(h)))
         self._h = h
 
     def __del__(self):

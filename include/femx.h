@@ -115,12 +115,24 @@ size_t femx_device_cache_bytes(int device);
  * are copied; the caller keeps ownership. Replaces the per-solve pointer
  * chasing over model->nodeList / elementList (src/femModel.h:24-25). */
 femx_status femx_mesh_upload(const femx_mesh_desc *desc, femx_mesh **out);
+/* The same with a choice of element blocks: block_elems 0 (the default:
+ * 256 for tet4, 128 for hexa8) or at most that, and their order:
+ * FEMX_BLOCKS_MORTON (femx_mesh_upload's: along a Morton curve of the
+ * element centroids) or FEMX_BLOCKS_ELEMENT_ORDER (runs of consecutive
+ * element ids).  The VMS per-iteration norms follow the reference's element
+ * order (src/femVMSExplicitFluidSolver.cpp:1065-1090), and with blocks in that
+ * order femx_vms_set_diagnostics computes them in streaming passes; the
+ * drop-in VMS solver uploads its mesh so.  Blocks in element order touch as
+ * few nodes as the mesh's numbering allows. */
+enum { FEMX_BLOCKS_MORTON = 0, FEMX_BLOCKS_ELEMENT_ORDER = 1 };
+femx_status femx_mesh_upload_blocks(const femx_mesh_desc *desc, int block_elems, int order, femx_mesh **out);
 femx_status femx_mesh_free(femx_mesh *mesh);
 /* Host-only statistics of the element-block partition femx_mesh_upload
  * builds (no device needed): stats[0..7] = blocks, elements, block-local
  * nodes, max local nodes per block, nodes shared between blocks, partial
  * sum slots, complete local nodes, max elements per block. */
 femx_status femx_mesh_block_stats(const femx_mesh_desc *desc, int block_elems, long long *stats);
+femx_status femx_mesh_block_stats_order(const femx_mesh_desc *desc, int block_elems, int order, long long *stats);
 
 /* ------------------------------------------------------------------ VMS */
 /* vmsProps[0..4] = rho, mu, invEps, c1, c2 (src/femModel.h:32, read at

@@ -63,14 +63,15 @@ def test_generators():
     assert np.all(g.coords[ch.pres_node, 0] == 1.0)
 
 
+@pytest.mark.parametrize("order", [femx.BLOCKS_MORTON, femx.BLOCKS_ELEMENT_ORDER])
 @pytest.mark.parametrize("kind,n,B", [("tet", 6, 256), ("tet", 9, 512), ("hex", 7, 128), ("tet", 36, 256),
                                        ("hex", 60, 128)])
-def test_block_partition_invariants(kind, n, B):
-    """The last two meshes (279,936 tets, 216,000 hexa) span several of the
+def test_block_partition_invariants(kind, n, B, order):
+    """The larger meshes (279,936 tets, 216,000 hexa) span several of the
     builder's independently cut ranges of 2^17 elements."""
     m = meshgen.tet_cube(n, 0.1, 1) if kind == "tet" else meshgen.hex_cube(n, 0.1, 1)
-    s = femx.block_stats(m, B)
-    assert femx.block_stats(m, B) == s  # deterministic (threads take ranges in any order)
+    s = femx.block_stats(m, B, order)
+    assert femx.block_stats(m, B, order) == s  # deterministic (threads take ranges in any order)
     deg = np.bincount(m.conn.ravel(), minlength=m.n_nodes)
     assert s["elements"] == m.n_elems
     assert s["max_local"] <= 256 and s["max_elems"] <= B
@@ -95,3 +96,16 @@ def test_block_cap_on_corners_of_one_node():
     s = femx.block_stats(m, 256)
     assert s["elements"] == 512 and s["max_elems"] <= 128 and s["blocks"] >= 4
     assert s["complete"] + s["partials"] == s["local_nodes"]
+
+
+def test_element_order_blocks_follow_numbering():
+    """Blocks in element order keep runs of consecutive ids, so a randomly
+    renumbered mesh touches many more nodes per block than in Morton order,
+    while the Morton partition does not depend on the numbering."""
+    m = meshgen.tet_cube(10, 0.1, 1)
+    perm = np.random.default_rng(0).permutation(m.n_elems)
+    r = Mesh(m.coords, m.conn[perm])
+    a, b = femx.block_stats(r, 256, femx.BLOCKS_MORTON), femx.block_stats(m, 256, femx.BLOCKS_MORTON)
+    assert a == b
+    e = femx.block_stats(r, 256, femx.BLOCKS_ELEMENT_ORDER)
+    assert e["local_nodes"] > 3 * a["local_nodes"] and e["max_local"] <= 256

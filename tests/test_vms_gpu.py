@@ -29,13 +29,13 @@ def _case(kind, n, jitter=0.0, seed=0):
     return m, bc, params, ic
 
 
-def _pair(femx, m, bc, params, ic, precision=0, tot_dual_steps=0, sdus=None):
+def _pair(femx, m, bc, params, ic, precision=0, tot_dual_steps=0, sdus=None, block_elems=0, order=0):
     o = VMSOracle(m, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"], params["dt"],
                   tot_dual_steps=tot_dual_steps)
     o.set_ic(ic)
     if sdus is not None:
         o.set_sdus(sdus)
-    dm = femx.DeviceMesh(m)
+    dm = femx.DeviceMesh(m, block_elems, order)
     g = femx.VMSSolver(dm, bc, params["rho"], params["mu"], params["inv_eps"], params["c1"], params["c2"],
                        params["dt"], tot_dual_steps=tot_dual_steps, precision=precision)
     g.set_state(ic, sdus)
@@ -49,6 +49,27 @@ def test_mass_and_initial_state(gpu):
     sol, sdus, t = g.get_state()
     assert np.array_equal(sol, o.get("sol_n"))
     assert t == 0.0 and not sdus.any()
+
+
+@pytest.mark.parametrize("block_elems,order", [(0, 1), (64, 0), (100, 1)])
+def test_block_choices_keep_parity(gpu, block_elems, order):
+    """Blocks in the mesh's element order, and smaller blocks: per-step FP64
+    parity with the oracle and mixed precision within 1e-5."""
+    m, bc, params, ic = _case("channel", 6, 0.15, seed=4)
+    st = gpu.block_stats(m, block_elems or 256, order)
+    assert st["max_elems"] <= (block_elems or 256) and st["blocks"] >= 5
+    o, g, _ = _pair(gpu, m, bc, params, ic, block_elems=block_elems, order=order)
+    for step in range(6):
+        o.step()
+        g.step()
+        sol, sdus, _ = g.get_state()
+        assert rel_l2(sol, o.get("sol_n")) <= TOL64, (step, rel_l2(sol, o.get("sol_n")))
+        assert rel_l2(sdus, o.get("sdus")) <= 1e-11
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=1, block_elems=block_elems, order=order)
+    for step in range(6):
+        o.step()
+        g.step()
+    assert rel_l2(g.get_state()[0], o.get("sol_n")) <= 1e-5
 
 
 @pytest.mark.parametrize("kind,n,jitter", [("cavity", 6, 0.0), ("channel", 6, 0.15), ("cavity", 5, 0.2)])

@@ -138,7 +138,8 @@ struct DevBuf {
 // Element blocks (the unit of one workgroup of the element kernels):
 // elements are renumbered along a Morton curve of their centroids and cut
 // greedily into blocks of at most B elements touching at most 256 nodes (the
-// curve in ranges of 2^17 elements, each range cut on its own).
+// curve in ranges of 2^17 elements, each range cut on its own); inside a
+// block the elements are in ascending original id.
 //   perm      int[E]        block order -> original element id
 //   elem_off  int[nb+1]     element range of each block (block order)
 //   node_off  int[nb+1]     range of each block's local nodes in loc_node

@@ -457,6 +457,9 @@ void build_blocks_host(Mesh &m, const std::vector<int> &g_off, int B, const std:
     };
     long e_begin = i0;
     auto close_block = [&](long e_end) {
+      // the block's elements in ascending id: each node's contributions in a
+      // block then follow the reference's element order
+      if (!b.elem_order) std::sort(b.perm.begin() + e_begin, b.perm.begin() + e_end);
       std::sort(cur.begin(), cur.end());
       for (size_t j = 0; j < cur.size(); j++) hval[find(cur[j])] = (int)j;
       for (long i = e_begin; i < e_end; i++) {

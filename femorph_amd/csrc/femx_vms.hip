@@ -374,6 +374,16 @@ template <typename T>
 __device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
 template <typename T>
 __device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
+// four-component records as native vectors (the builtins take no structs)
+__device__ __forceinline__ void st_stream(float4 *p, float4 v) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  __builtin_nontemporal_store(v4f{v.x, v.y, v.z, v.w}, reinterpret_cast<v4f *>(p));
+}
+__device__ __forceinline__ void st_stream(double4 *p, double4 v) {
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(v2d{v.x, v.y}, reinterpret_cast<v2d *>(p));
+  __builtin_nontemporal_store(v2d{v.z, v.w}, reinterpret_cast<v2d *>(p) + 1);
+}
 __device__ __forceinline__ ushort4 ld_stream(const ushort4 *p) {
   const unsigned long long w = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
   return make_ushort4((unsigned short)w, (unsigned short)(w >> 16), (unsigned short)(w >> 32),
@@ -415,7 +425,9 @@ struct BlockArgs {
   double *bmax;  // [nblk] per-block max of the pre-BC |u|^2
   // element diagnostics (eval_centroid_*, :699-724), NULL when off:
   const int *perm;   // block order -> original element id
-  double *diag_res;  // [E][4][4] element residual contributions, original order
+  // [4E] T4: per block, each node's running sum of the block's contributions
+  // in ascending element id, at 4 e0 + the contribution's node-sorted position
+  void *diag_img;
   double *diag_sdu;  // [E][3] mean over the Gauss points of the subgrid velocity the residual used (NULL: norms only)
   // batches of meshes (NULL for one mesh): per block its mesh, per mesh the
   // constants (incl. its time) and its first node (the quirk's nodes 0..3)
@@ -727,21 +739,14 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
     }
     if (RES) {
       vms_residual<T, ACC>(DN, V, ev, p, dd, sd, res, k);
-      if (a.diag_res) {
+      if (a.diag_sdu) {
         const size_t eo = (size_t)a.perm[e];
-        double *d = a.diag_res + eo * 16;
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int i = 0; i < 3; i++) {
+          double sm = 0.0;
 #pragma unroll
-          for (int i = 0; i < 4; i++) d[b * 4 + i] = (double)res[b][i];
-        if (a.diag_sdu) {
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            double sm = 0.0;
-#pragma unroll
-            for (int g = 0; g < 4; g++) sm += (double)sd(g, i) * 0.25;
-            a.diag_sdu[eo * 3 + i] = sm;
-          }
+          for (int g = 0; g < 4; g++) sm += (double)sd(g, i) * 0.25;
+          a.diag_sdu[eo * 3 + i] = sm;
         }
       }
     }
@@ -785,6 +790,8 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
   }
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
+  // (with diagnostics, each contribution is replaced by the running sum)
+  const bool dg = a.diag_img != nullptr;
   double m2 = 0.0;
   if (has_n) {
     const int g = g_own;
@@ -793,11 +800,18 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
     const int s1 = sbeg[tid + 1];
 #pragma unroll 4
     for (int s = sbeg[tid]; s < s1; s++) {
-      const T4 v = sres[s];
+      T4 v = sres[s];
       Rt[0] += (S)v.x;
       Rt[1] += (S)v.y;
       Rt[2] += (S)v.z;
       Rt[3] += (S)v.w;
+      if (dg) {
+        v.x = (T)Rt[0];
+        v.y = (T)Rt[1];
+        v.z = (T)Rt[2];
+        v.w = (T)Rt[3];
+        sres[s] = v;
+      }
     }
     double R[4] = {(double)Rt[0], (double)Rt[1], (double)Rt[2], (double)Rt[3]};
     if (complete) {
@@ -821,6 +835,12 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
       }
       static_cast<typename Vec4<S>::type *>(a.partials)[slot_base + ((meta_own >> LN_RANK_SHIFT) & LN_RANK)] = v;
     }
+  }
+  // diagnostics: the block's running sums per node, in node-sorted order
+  if (dg) {
+    __syncthreads();
+    T4 *img = static_cast<T4 *>(a.diag_img) + 4L * e0;
+    for (int q = tid; q < 4 * ne; q += VB) st_stream(img + q, sres[q]);
   }
   block_max_store<VB>(m2, a.bmax, row);
 }
@@ -1000,71 +1020,126 @@ __global__ void vms_bc_only(int nrec, const int *rec_node, const BCRecord *rec, 
 // Element diagnostics.  The reference scatters each element's residual into
 // the global rhs and reads the element's centroid residual right after its
 // own scatter (:1065-1090, :699-711), so node n holds, at that moment, the
-// sum of the contributions of the elements <= e that touch it.  Per node, walk
-// its elements in ascending original order and replace every contribution by
-// that running sum (slot e*4+a belongs to exactly one node, so in place).
-__global__ __launch_bounds__(256) void vms_diag_prefix(int N, const int *g_off, const int *g_slot, double4 *contrib) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-  for (int s = g_off[n]; s < g_off[n + 1]; s++) {
-    const int slot = g_slot[s];
-    const double4 v = contrib[slot];
-    r0 += v.x;
-    r1 += v.y;
-    r2 += v.z;
-    r3 += v.w;
-    contrib[slot] = make_double4(r0, r1, r2, r3);
+// sum of the contributions of the elements <= e that touch it.
+//
+// A block's elements are in ascending id, so a node's contributions inside
+// one block form a run in the reference's order, and the block kernel leaves
+// the run's running sums in the image.  The running sum at contribution x of
+// node n is then its own run's value at x plus, for every other block that
+// holds n, that block's run value at its last element below x's (positions
+// fixed by the mesh: xoff / xpos, built once).  One workgroup per block forms
+// them for all its contributions in LDS, then each element averages its four
+// corners (eval_centroid_residuals :699-711) and the block keeps the maxima of
+// |res_u| and |res_p| (eval_residual_norms :774-793).  el_u / el_p NULL: the
+// maxima only.  (An element lists each node once: a corner's running sum
+// then includes exactly the elements up to its own.)
+template <typename T4>
+__global__ __launch_bounds__(256) void vms_diag_centroid(const int *elem_off, const int *node_off,
+                                                         const ushort4 *ecorn, const int2 *lnode, const int *perm,
+                                                         const T4 *img, const int *xoff, const int *xpos,
+                                                         double *el_u, double *el_p, double *nmax) {
+  // the running sums of the block's contributions, in the element math's
+  // precision (FP32 with FEMX_MIXED: half the LDS, twice the waves)
+  __shared__ T4 sp[4 * VB];
+  const int blk = blockIdx.x, tid = threadIdx.x;
+  const int e0 = elem_off[blk], ne = elem_off[blk + 1] - e0, n0 = node_off[blk];
+  // a thread's 4 contributions and up to XN cross terms each: every load of
+  // one round is issued before the next round needs it
+  constexpr int QN = 4, XN = 3;
+  T4 v[QN];
+  int xa[QN], xb[QN], xp[QN][XN];
+#pragma unroll
+  for (int u = 0; u < QN; u++) {
+    const int q = tid + u * VB;
+    const long p = 4L * e0 + q;
+    xa[u] = xb[u] = 0;
+    if (q < 4 * ne) {
+      v[u] = img[p];
+      xa[u] = xoff[p];
+      xb[u] = xoff[p + 1];
+    }
   }
-}
-
-// Centroid residual of every element (eval_centroid_residuals :699-711) and
-// per-block maxima of |res_u| and |res_p| (eval_residual_norms :774-793);
-// el_u / el_p NULL: the maxima only.
-__global__ __launch_bounds__(256) void vms_diag_centroid(int E, const double4 *prefix, double *el_u, double *el_p,
-                                                         double *nmax) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < QN; u++)
+#pragma unroll
+    for (int c = 0; c < XN; c++) xp[u][c] = xa[u] + c < xb[u] ? xpos[xa[u] + c] : -1;
+#pragma unroll
+  for (int u = 0; u < QN; u++) {
+    const int q = tid + u * VB;
+    if (q >= 4 * ne) continue;
+    double r0 = (double)v[u].x, r1 = (double)v[u].y, r2 = (double)v[u].z, r3 = (double)v[u].w;
+#pragma unroll
+    for (int c = 0; c < XN; c++)
+      if (xp[u][c] >= 0) {
+        const T4 w = img[xp[u][c]];
+        r0 += (double)w.x;
+        r1 += (double)w.y;
+        r2 += (double)w.z;
+        r3 += (double)w.w;
+      }
+    for (int c = xa[u] + XN; c < xb[u]; c++) {
+      const T4 w = img[xpos[c]];
+      r0 += (double)w.x;
+      r1 += (double)w.y;
+      r2 += (double)w.z;
+      r3 += (double)w.w;
+    }
+    T4 o;
+    o.x = r0;
+    o.y = r1;
+    o.z = r2;
+    o.w = r3;
+    sp[q] = o;
+  }
+  __syncthreads();
   double un = 0.0, pn = 0.0;
-  if (e < E) {
+  if (tid < ne) {
+    const int i = e0 + tid;
+    const ushort4 ec = ecorn[i];
+    const unsigned short c[4] = {ec.x, ec.y, ec.z, ec.w};
     double u0 = 0.0, u1 = 0.0, u2 = 0.0, p = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-      const double4 v = prefix[4L * e + a];
-      u0 += v.x * 0.25;
-      u1 += v.y * 0.25;
-      u2 += v.z * 0.25;
-      p += v.w * 0.25;
+      const int j = c[a] & EC_ID, rank = (c[a] >> EC_RANK_SHIFT) & EC_RANK;
+      const T4 w = sp[(lnode[n0 + j].y & LN_BEGIN) + rank];
+      u0 += (double)w.x * 0.25;
+      u1 += (double)w.y * 0.25;
+      u2 += (double)w.z * 0.25;
+      p += (double)w.w * 0.25;
     }
     if (el_u) {
-      el_u[3L * e] = u0;
-      el_u[3L * e + 1] = u1;
-      el_u[3L * e + 2] = u2;
-      el_p[e] = p;
+      const long eo = perm[i];
+      el_u[3 * eo] = u0;
+      el_u[3 * eo + 1] = u1;
+      el_u[3 * eo + 2] = u2;
+      el_p[eo] = p;
     }
     un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
     pn = fabs(p);
   }
-  __shared__ double wm[2][4];
+  __shared__ double wm[2][VB / 64];
   for (int off = 32; off > 0; off >>= 1) {
-    const double a = __shfl_xor(un, off), b = __shfl_xor(pn, off);
-    un = a > un ? a : un;
-    pn = b > pn ? b : pn;
+    const double x = __shfl_xor(un, off), y = __shfl_xor(pn, off);
+    un = x > un ? x : un;
+    pn = y > pn ? y : pn;
   }
-  if ((threadIdx.x & 63) == 0) {
-    wm[0][threadIdx.x >> 6] = un;
-    wm[1][threadIdx.x >> 6] = pn;
+  if ((tid & 63) == 0) {
+    wm[0][tid >> 6] = un;
+    wm[1][tid >> 6] = pn;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
-    double m = wm[threadIdx.x][0];
-    for (int i = 1; i < 4; i++) m = wm[threadIdx.x][i] > m ? wm[threadIdx.x][i] : m;
-    nmax[2L * blockIdx.x + threadIdx.x] = m;
+  if (tid < 2) {
+    double m = wm[tid][0];
+    for (int w = 1; w < VB / 64; w++) m = wm[tid][w] > m ? wm[tid][w] : m;
+    nmax[2L * blk + tid] = m;
   }
 }
 
-// One workgroup: out[0..2] = max |res_u|, max |res_p|, max pre-BC |u|^2.
-__global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nmax, int n_bmax, const double *bmax,
-                                                       double *out) {
+// One workgroup of RED_T threads: out[0..2] = max |res_u|, max |res_p|, max
+// pre-BC |u|^2.
+constexpr int RED_T = 1024;
+__global__ __launch_bounds__(RED_T) void vms_diag_reduce(int nb, const double *nmax, int n_bmax, const double *bmax,
+                                                         double *out) {
   double v[3] = {0.0, 0.0, 0.0};
   // unrolled so that many independent loads are in flight per thread
 #pragma unroll 16
@@ -1074,7 +1149,7 @@ __global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nma
   }
 #pragma unroll 16
   for (int i = threadIdx.x; i < n_bmax; i += blockDim.x) v[2] = bmax[i] > v[2] ? bmax[i] : v[2];
-  __shared__ double wm[3][4];
+  __shared__ double wm[3][RED_T / 64];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -1086,7 +1161,7 @@ __global__ __launch_bounds__(256) void vms_diag_reduce(int nb, const double *nma
   __syncthreads();
   if (threadIdx.x < 3) {
     double m = wm[threadIdx.x][0];
-    for (int i = 1; i < 4; i++) m = wm[threadIdx.x][i] > m ? wm[threadIdx.x][i] : m;
+    for (int i = 1; i < RED_T / 64; i++) m = wm[threadIdx.x][i] > m ? wm[threadIdx.x][i] : m;
     out[threadIdx.x] = m;
   }
 }
@@ -1142,7 +1217,11 @@ struct femx_vms {
   // element diagnostics of every dual iteration (femx_vms_set_diagnostics):
   // 0 off, 1 norms and element fields, 2 norms only
   int diag = 0;
-  DevBuf<double> diag_res;   // [E][4][4] contributions, then their running sums
+  DevBuf<unsigned char> diag_img;  // [4E] T4: per block, each node's running sums in its run
+  // per image entry x: the entries of the node's runs in other blocks that
+  // sum its elements below x's (xpos[xoff[x] .. xoff[x+1]))
+  DevBuf<int> diag_xoff, diag_xpos;
+  long diag_xn = 0;
   DevBuf<double> diag_sdu;   // [E][3]
   DevBuf<double> diag_el;    // [E][3] centroid residual u v w, then [E] p
   DevBuf<double> diag_nmax;  // [blocks][2]
@@ -1228,7 +1307,7 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   a.err = s->bc.err.p;
   a.bmax = s->bmax.p;
   a.perm = b.d_perm.p;
-  a.diag_res = s->diag ? s->diag_res.p : nullptr;
+  a.diag_img = s->diag ? s->diag_img.p : nullptr;
   a.diag_sdu = s->diag == 1 ? s->diag_sdu.p : nullptr;
   const bool batch = s->batch;
   a.blk_mesh = batch ? s->blk_mesh.p : nullptr;
@@ -1306,18 +1385,30 @@ femx_status launch_fixup(femx_vms *s, bool first_dual, double t) {
   return FEMX_OK;
 }
 
-// Diagnostics of dual iteration d from the contributions the block kernel
-// just wrote: running sums per node, centroid residuals, maxima.
+// Diagnostics of dual iteration d from the running sums the block kernel
+// just left in the image: centroid residuals per block, then the maxima.
+template <typename T4>
+void launch_diag_t(femx_vms *s, int d) {
+  Mesh &m = *s->mesh;
+  const Blocks &b = m.blk;
+  double *el = s->diag == 1 ? s->diag_el.p : nullptr;
+  vms_diag_centroid<T4><<<b.nb, VB, 0, s->st>>>(b.d_elem_off.p, b.d_node_off.p, s->ecorn.p, s->lnode.p, b.d_perm.p,
+                                                reinterpret_cast<const T4 *>(s->diag_img.p), s->diag_xoff.p,
+                                                s->diag_xpos.p, el, el ? el + 3L * m.E : nullptr, s->diag_nmax.p);
+  const long row = (s->diag_steps % DIAG_RING) * (s->p.tot_dual_steps + 1) + d;
+  vms_diag_reduce<<<1, RED_T, 0, s->st>>>(b.nb, s->diag_nmax.p, s->n_bmax, s->bmax.p, s->diag_ring.p + 3L * row);
+}
+
 femx_status launch_diag(femx_vms *s, int d) {
   Mesh &m = *s->mesh;
-  TimedLaunch tl(s->timer, s->timer.kind("vms_diag", m.E * (128.0 * 3 + 32.0 + 8.0)), s->st);
-  double4 *c = reinterpret_cast<double4 *>(s->diag_res.p);
-  vms_diag_prefix<<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, c);
-  const int nb = grid_for(m.E, 256);
-  double *el = s->diag == 1 ? s->diag_el.p : nullptr;
-  vms_diag_centroid<<<nb, 256, 0, s->st>>>(m.E, c, el, el ? el + 3L * m.E : nullptr, s->diag_nmax.p);
-  const long row = (s->diag_steps % DIAG_RING) * (s->p.tot_dual_steps + 1) + d;
-  vms_diag_reduce<<<1, 256, 0, s->st>>>(nb, s->diag_nmax.p, s->n_bmax, s->bmax.p, s->diag_ring.p + 3L * row);
+  const bool f32 = f32_elem(s->p.precision);
+  // per corner its image entry and offset, per cross term a position and an
+  // entry (other blocks' runs), per element its corner record
+  const double rec = f32 ? 16.0 : 32.0;
+  const double xn = s->diag_xn;
+  TimedLaunch tl(s->timer, s->timer.kind("vms_diag", m.E * 4.0 * (rec + 4.0) + xn * (rec + 4.0) + m.E * 8.0), s->st);
+  if (f32) launch_diag_t<float4>(s, d);
+  else launch_diag_t<double4>(s, d);
   FEMX_HIP(hipGetLastError());
   return FEMX_OK;
 }
@@ -1594,10 +1685,61 @@ extern "C" femx_status femx_vms_set_diagnostics(femx_vms *s, int enable) {
   if (s->batch && enable) return set_error("diagnostics are not offered on batches"), FEMX_ERR_UNSUPPORTED;
   const long E = s->mesh->E;
   femx_status r;
-  if (enable && !s->diag_res.p) {
+  if (enable && !s->diag_img.p) {
+    const Mesh &m = *s->mesh;
+    const Blocks &b = m.blk;
     const long nrow = DIAG_RING * (s->p.tot_dual_steps + 1);
-    if ((r = s->diag_res.alloc(16 * E)) != FEMX_OK || (r = s->diag_nmax.alloc(2L * grid_for(E, 256))) != FEMX_OK ||
-        (r = s->diag_ring.alloc(3L * nrow)) != FEMX_OK)
+    const size_t rec = f32_elem(s->p.precision) ? sizeof(float4) : sizeof(double4);
+    // per image entry its element; per node its runs (one per block that
+    // holds it, each in ascending element id)
+    std::vector<int> eid(4 * E), roff(m.N + 1, 0);
+    std::vector<int2> runs(std::max(b.total_local, 1), make_int2(0, 0));
+    parallel_items(b.nb, [&](long k) {
+      const int e0 = b.elem_off[k];
+      for (int i = e0; i < b.elem_off[k + 1]; i++)
+        for (int c = 0; c < 4; c++) eid[4L * e0 + b.lcsr[4L * i + c]] = b.perm[i];
+    }, E >= (1L << 16));
+    for (int l = 0; l < b.total_local; l++) roff[b.loc_node[l] + 1]++;
+    for (int n = 0; n < m.N; n++) roff[n + 1] += roff[n];
+    {
+      std::vector<int> fill(roff.begin(), roff.end() - 1);
+      for (int l = 0; l < b.total_local; l++)
+        runs[fill[b.loc_node[l]]++] = make_int2(b.lcsr_off[l], b.lcsr_off[l + 1]);
+    }
+    // cross terms of entry x in run i: for every other run j of the node,
+    // the last entry of j whose element is below x's (none if there is no
+    // such entry); counted, then written, per node (its entries are its own)
+    auto cross = [&](int n, const auto &emit) {
+      for (int i = roff[n]; i < roff[n + 1]; i++)
+        for (int j = roff[n]; j < roff[n + 1]; j++) {
+          if (j == i) continue;
+          int t = runs[j].x;
+          for (int x = runs[i].x; x < runs[i].y; x++) {
+            while (t < runs[j].y && eid[t] < eid[x]) t++;
+            if (t > runs[j].x) emit(x, t - 1);
+          }
+        }
+    };
+    constexpr int NCH = 4096;  // nodes per work item
+    const long nitems = (m.N + NCH - 1) / NCH;
+    std::vector<int> xoff(4 * E + 1, 0);
+    parallel_items(nitems, [&](long c) {
+      for (int n = (int)(c * NCH); n < std::min<long>(m.N, (c + 1) * NCH); n++) cross(n, [&](int x, int) { xoff[x + 1]++; });
+    }, E >= (1L << 16));
+    for (long x = 0; x < 4 * E; x++) xoff[x + 1] += xoff[x];
+    std::vector<int> xpos(std::max(xoff[4 * E], 1), 0);
+    {
+      std::vector<int> fill(xoff.begin(), xoff.end() - 1);
+      parallel_items(nitems, [&](long c) {
+        for (int n = (int)(c * NCH); n < std::min<long>(m.N, (c + 1) * NCH); n++)
+          cross(n, [&](int x, int y) { xpos[fill[x]++] = y; });
+      }, E >= (1L << 16));
+    }
+    s->diag_xn = xoff[4 * E];
+    if ((r = s->diag_img.alloc(4 * E * rec)) != FEMX_OK ||
+        (r = s->diag_xoff.upload(xoff.data(), xoff.size(), s->st)) != FEMX_OK ||
+        (r = s->diag_xpos.upload(xpos.data(), xpos.size(), s->st)) != FEMX_OK ||
+        (r = s->diag_nmax.alloc(2L * std::max(b.nb, 1))) != FEMX_OK || (r = s->diag_ring.alloc(3L * nrow)) != FEMX_OK)
       return r;
     FEMX_HIP(hipMemsetAsync(s->diag_ring.p, 0, 3L * nrow * sizeof(double), s->st));
   }
@@ -2116,7 +2258,7 @@ femx_status union_upload(femx_vms_union *u) {
     femx_vms *h = u->mem[i];
     VmsMember &e = t[i];
     e.b = block_args<float>(h, true, 0.0, h->sdus32.p);
-    e.b.diag_res = nullptr;
+    e.b.diag_img = nullptr;
     e.b.diag_sdu = nullptr;
     e.f = fix_args(h, true, 0.0);
     for (int k = 0; k < 2; k++) {

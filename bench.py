@@ -169,14 +169,15 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+CEILING_GRIDS = (4096, 16384, 65536, 262144)
 CEILING_PROBE = ("best of femx_hbm_probe mix_7r5w and mix_7r5w_nt (7 read / 5 write float4 streams, 1 GiB) "
-                 "over 1024/4096/16384 workgroups")
+                 "over 4096/16384/65536/262144 workgroups")
 
 
 def stream_ceiling_gbs(reps: int = 10) -> float:
     """Practical HBM ceiling for a read/write mix like the VMS block kernel's."""
     return max(femx.hbm_probe(k, 1 << 30, g, reps) for k in ("mix_7r5w", "mix_7r5w_nt")
-               for g in (1024, 4096, 16384)) * 1e3
+               for g in CEILING_GRIDS) * 1e3
 
 
 def workload_hbm(args):
@@ -185,7 +186,7 @@ def workload_hbm(args):
     res = {}
     for kind in ("copy", "read", "write", "mix_7r5w", "copy_nt", "read_nt", "write_nt", "mix_7r5w_nt"):
         res[kind] = {str(g): round(femx.hbm_probe(kind, 1 << 30, g, 20) * 1e3, 1)
-                     for g in (1024, 2048, 4096, 8192, 16384)}
+                     for g in (1024, 4096, 16384, 65536, 262144)}
         print(f"hbm {kind}: {res[kind]}", file=sys.stderr, flush=True)
     print(json.dumps({"metric": "HBM streaming bandwidth (float4 grid-stride kernels, 1 GiB per buffer)",
                       "unit": "GB/s", "peak": HBM_PEAK_GBS, "by_kind_and_grid": res,

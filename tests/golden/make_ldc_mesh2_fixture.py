@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/ldc_mesh2_re100.npz from the reference's example
+"""Writes tests/golden/ldc_mesh2.npz from the reference's example
 08_2D_hexa_ldc_mesh2 (data files only):
 
 - fluid.vtu: the 29 x 29 x 1 hexa8 lid-driven cavity mesh (1,800 nodes,
@@ -8,10 +8,10 @@
   GlobalNodeID (1-based) of the lid and wall nodes, which its model writer
   (create_exVMS_simul.py) turns into NODEVEL (lid u = 1) and NODEDIRBC
   (walls, zero) lines;
-- 07_comparison/hor05_100.csv and vert05_100.csv: velocity sampled by the
-  reference's own Re = 100 run along the vertical (x = 0.5) and horizontal
-  (y = 0.5) centrelines, columns DXYZ:0..2, velocity:0..2, dirBC, neuBC,
-  Points:0..2;
+- 07_comparison/{hor05,vert05}_{100,1K,10K}.csv: velocity sampled by the
+  reference's own Re = 100, 1,000 and 10,000 runs along the vertical
+  (x = 0.5) and horizontal (y = 0.5) centrelines, columns DXYZ:0..2,
+  velocity:0..2, dirBC, neuBC, Points:0..2 (the fields of its VTK writer);
 - 07_comparison/re_100_horizontal.txt and re_100_vertical.txt: the
   published Re = 100 centreline benchmark values the reference plots its run
   against.
@@ -34,7 +34,7 @@ import numpy as np
 
 REF = os.environ.get("FEMX_REFERENCE", "/root/reference")
 EX = os.path.join(REF, "examples/02_explicit_fluid_VMS/08_2D_hexa_ldc_mesh2")
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ldc_mesh2_re100.npz")
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ldc_mesh2.npz")
 DTYPES = {"Int64": "<i8", "Int32": "<i4", "UInt8": "u1", "Float64": "<f8", "Float32": "<f4"}
 
 
@@ -87,9 +87,13 @@ def main():
     ver = np.loadtxt(os.path.join(cmp, "vert05_100.csv"), skiprows=1, delimiter=",")
     bench_h = np.loadtxt(os.path.join(cmp, "re_100_horizontal.txt"), skiprows=1)
     bench_v = np.loadtxt(os.path.join(cmp, "re_100_vertical.txt"), skiprows=1)
+    more = {}
+    for re in ("1K", "10K"):
+        for line in ("hor05", "vert05"):
+            more[f"{line}_{re}"] = np.loadtxt(os.path.join(cmp, f"{line}_{re}.csv"), skiprows=1, delimiter=",")
     np.savez_compressed(OUT, coords=coords, conn=conn, top=walls["top"], left=walls["left"],
                         right=walls["right"], bottom=walls["bottom"], hor05_100=hor, vert05_100=ver,
-                        bench_horizontal=bench_h, bench_vertical=bench_v)
+                        bench_horizontal=bench_h, bench_vertical=bench_v, **more)
     print(f"wrote {OUT}: {len(coords)} nodes, {len(conn)} hexa8, lid {len(walls['top'])} nodes, "
           f"probes {len(hor)} + {len(ver)}")
 

@@ -634,29 +634,37 @@ constexpr int DIAG_ONE_PASS = 2048;  // up to this many rows, one workgroup and 
 // sets l_CFL_max once before its time loop, :996, and resets only the v- and
 // a-CFL maxima per step, :1078-1079): the kernel that writes the step's row
 // folds in the previous step's row (prev, null on the first step).
-__global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int ne_blk, const double *nblk,
-                                                         int nn_blk, int with_elem, double *part, const int *halt,
-                                                         const double *prev) {
-  if (halted(halt)) return;
+// One workgroup of 256 threads folds the rows of workgroups part, part +
+// nparts, ... into out[0 .. D_COLS).  AT: the rows were written by other
+// workgroups of the running kernel, so they are read with device-scope
+// atomic loads (coherent across the XCDs' L2 caches).
+template <bool AT>
+__device__ __forceinline__ double ld_row(const double *p) {
+  if constexpr (AT) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool AT>
+__device__ __forceinline__ void diag_fold(const double *eblk, int ne_blk, const double *nblk, int nn_blk,
+                                          int with_elem, double *out, const double *prev, int part, int nparts) {
   double m[D_COLS];
 #pragma unroll
   for (int c = 0; c < D_COLS; c++) m[c] = d_identity(c);
-  const int stride = gridDim.x * blockDim.x;
+  const int stride = nparts * 256;
   // node columns: the fix-up's rows and the element blocks' rows; element
   // columns: the element blocks' rows when the element diagnostics are on
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nn_blk; b += stride) {
+  for (int b = part * 256 + threadIdx.x; b < nn_blk; b += stride) {
 #pragma unroll
     for (int c = 0; c < D_COLS; c++)
       if (c < 6 || c == D_UMOD) {
-        const double x = nblk[(long)b * D_COLS + c];
+        const double x = ld_row<AT>(nblk + (long)b * D_COLS + c);
         m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
       }
   }
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ne_blk; b += stride) {
+  for (int b = part * 256 + threadIdx.x; b < ne_blk; b += stride) {
 #pragma unroll
     for (int c = 0; c < D_COLS; c++)
       if (with_elem || c < 6 || c == D_UMOD) {
-        const double x = eblk[(long)b * D_COLS + c];
+        const double x = ld_row<AT>(eblk + (long)b * D_COLS + c);
         m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
       }
   }
@@ -675,8 +683,16 @@ __global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int
     double v = red[0][c];
     for (int w = 1; w < 4; w++) v = d_is_min(c) ? fmin(v, red[w][c]) : fmax(v, red[w][c]);
     if (c == D_LCFL && prev) v = fmax(v, prev[D_LCFL]);
-    part[(long)blockIdx.x * D_COLS + c] = v;
+    out[c] = v;
   }
+}
+
+__global__ __launch_bounds__(256) void supg_diag_partial(const double *eblk, int ne_blk, const double *nblk,
+                                                         int nn_blk, int with_elem, double *part, const int *halt,
+                                                         const double *prev) {
+  if (halted(halt)) return;
+  diag_fold<false>(eblk, ne_blk, nblk, nn_blk, with_elem, part + (long)blockIdx.x * D_COLS, prev, blockIdx.x,
+                   gridDim.x);
 }
 
 __global__ __launch_bounds__(64) void supg_diag_final(const double *part, int np, double *row, const int *halt,
@@ -709,6 +725,12 @@ struct SupgFix {
   int *err;
   const int *halt;
   double *nblk;  // [workgroups][D_COLS]: node columns of each workgroup's row
+  // few rows: the fix-up's last workgroup folds this step's rows into the
+  // log row (NULL: a separate reduction does)
+  double *fold_row;
+  const double *fold_prev, *eblk;
+  int ne_blk, with_elem;
+  unsigned *fold_count;  // workgroups done, reset by the last one
   SupgConst k;
 };
 
@@ -726,6 +748,21 @@ __global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
   if (s < a.n_sh) fixup_node<T>(a, s, nv, nm2[0]);
   // this workgroup's row of velocity extrema and max |u|^2 (:604-655)
   block_node_store<256>(nv, nm2[0], a.nblk);
+  if (a.fold_row) {
+    // the last workgroup to finish folds every row (its own and the
+    // others', made visible by their fences before they counted)
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      last = atomicAdd(a.fold_count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      __threadfence();
+      diag_fold<true>(a.eblk, a.ne_blk, a.nblk, gridDim.x, a.with_elem, a.fold_row, a.fold_prev, 0, 1);
+      if (threadIdx.x == 0) *a.fold_count = 0u;
+    }
+  }
 }
 
 template <typename T>
@@ -840,6 +877,7 @@ struct femx_supg {
   DevBuf<double> ring, eblk, nblk;  // [RING][D_COLS] reduced rows; per-block rows
   DevBuf<double> dpart;             // [DIAG_PARTS][D_COLS] partial rows of the reduction
   DevBuf<int> halt;
+  DevBuf<unsigned> fold_count;      // the fix-up's workgroups done (its fold of the log row)
   int cur = 0;
   long loop_time = 0;
   double time = 0.0;
@@ -972,7 +1010,8 @@ femx_status launch_supg(femx_supg *s, bool dyn, bool press, bool euler_step, dou
   return FEMX_OK;
 }
 
-femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
+femx_status launch_fix(femx_supg *s, bool euler_step, double t, double *fold_row = nullptr,
+                       const double *fold_prev = nullptr) {
   Blocks &b = s->mesh->blk;
   if (b.n_sh == 0) return FEMX_OK;
   const bool f32p = f32_sum(s->p.precision);  // float4 partial records
@@ -997,6 +1036,12 @@ femx_status launch_fix(femx_supg *s, bool euler_step, double t) {
   a.err = s->bc.err.p;
   a.halt = s->halt.p;
   a.nblk = s->nblk.p;
+  a.fold_row = fold_row;
+  a.fold_prev = fold_prev;
+  a.eblk = s->eblk.p;
+  a.ne_blk = b.nb;
+  a.with_elem = s->diag_enabled ? 1 : 0;
+  a.fold_count = s->fold_count.p;
   a.k = supg_const(s, t, euler_step);
   if (f32p) supg_fixup<float><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
   else supg_fixup<double><<<grid_for(b.n_sh, 256), 256, 0, s->st>>>(a);
@@ -1055,7 +1100,10 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
   if ((r = s->partials.alloc(std::max(nparts, 1))) != FEMX_OK) return fail(r);
   if ((r = s->lam.alloc(m.E)) != FEMX_OK || (r = s->regime.alloc(m.E)) != FEMX_OK || (r = s->pres.alloc(m.E)) != FEMX_OK)
     return fail(r);
-  if ((r = s->ring.alloc((size_t)RING * D_COLS)) != FEMX_OK || (r = s->halt.alloc(1)) != FEMX_OK) return fail(r);
+  if ((r = s->ring.alloc((size_t)RING * D_COLS)) != FEMX_OK || (r = s->halt.alloc(1)) != FEMX_OK ||
+      (r = s->fold_count.alloc(1)) != FEMX_OK)
+    return fail(r);
+  if (hipMemsetAsync(s->fold_count.p, 0, sizeof(unsigned), s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   if ((r = s->eblk.alloc((size_t)m.blk.nb * D_COLS)) != FEMX_OK) return fail(r);
   if ((r = s->nblk.alloc((size_t)grid_for(m.N, 256) * D_COLS)) != FEMX_OK) return fail(r);
   if ((r = s->dpart.alloc((size_t)DIAG_PARTS * D_COLS)) != FEMX_OK) return fail(r);
@@ -1128,14 +1176,18 @@ femx_status supg_enqueue(femx_supg *s, int nsteps) {
     const double *prev = s->loop_time > 1 ? s->ring.p + ((s->loop_time - 1) % RING) * D_COLS : nullptr;
     femx_status r = launch_supg(s, true, s->press_pending, euler_step, s->time, s->loop_time, prev);
     if (r != FEMX_OK) return r;
-    if ((r = launch_fix(s, euler_step, s->time)) != FEMX_OK) return r;
+    // node extrema rows: one per fix-up workgroup (nblk) and one per element
+    // block (node columns of eblk), written where the new states are formed;
+    // with few rows, the fix-up's last workgroup folds them into the log row
+    const int nnb = s->mesh->blk.n_sh ? grid_for(s->mesh->blk.n_sh, 256) : 0;
+    const bool fold = nnb > 0 && nnb + s->mesh->blk.nb <= DIAG_ONE_PASS;
+    if ((r = launch_fix(s, euler_step, s->time, fold ? row : nullptr, fold ? prev : nullptr)) != FEMX_OK) return r;
     s->cur ^= 1;
     s->press_pending = true;
-    // node extrema rows: one per fix-up workgroup (nblk) and one per element
-    // block (node columns of eblk), written where the new states are formed
-    const int nnb = s->mesh->blk.n_sh ? grid_for(s->mesh->blk.n_sh, 256) : 0;
-    if (nnb + s->mesh->blk.nb <= DIAG_ONE_PASS) {
-      // few rows: one workgroup folds them straight into the log row
+    if (fold) {
+      // folded by the fix-up
+    } else if (nnb + s->mesh->blk.nb <= DIAG_ONE_PASS) {
+      // few rows and no fix-up: one workgroup folds them into the log row
       supg_diag_partial<<<1, 256, 0, s->st>>>(s->eblk.p, s->mesh->blk.nb, s->nblk.p, nnb, s->diag_enabled ? 1 : 0,
                                               row, s->halt.p, prev);
     } else {

@@ -18,13 +18,17 @@
 //                 request, reads finished states back and frees their
 //                 handles and budget.
 //
-//   union thread  continuous batching of the VMS requests in the mixed
-//                 precision without dual iterations: after its first step on
-//                 its own stream a request joins a union of handles
-//                 (femx_vms_union_*), which advances all of its members by
-//                 one launch of each phase per step in chunks of `chunk`
-//                 steps, two chunks in flight; members join between chunks
-//                 and leave when their own step counts are done.
+//   union threads continuous batching, one thread per solver kind: after
+//                 its first step on its own stream a request that a union
+//                 accepts (VMS: mixed precision without dual iterations;
+//                 SUPG: mixed hexa8 with B-hat, no fixed DOFs, at most 2048
+//                 rows of extrema) joins a union of handles
+//                 (femx_vms_union_*, femx_supg_union_*), which advances all
+//                 of its members by one launch of each phase per step in
+//                 chunks of `chunk` steps, two chunks in flight; members join
+//                 between chunks and leave when their own step counts are
+//                 done (a SUPG member's exit condition is then checked as on
+//                 its own stream).
 //
 // Each request's arithmetic is that of its own handle, so a result does not
 // depend on what else is being served (bitwise).
@@ -80,7 +84,7 @@ struct Request {
   int done = 0, issued = 0;
   femx_request_stats stats{};
   std::vector<double> result;
-  bool joins_union = false;  // runs in the union of the VMS handles
+  bool joins_union = false;  // runs in the union of its kind's handles
 
   femx_bc_desc bc() const {
     femx_bc_desc b;
@@ -134,7 +138,7 @@ struct femx_server {
   int device = 0;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<std::shared_ptr<Request>> pending, ready, union_ready;
+  std::deque<std::shared_ptr<Request>> pending, ready, union_ready[2];
   std::vector<std::vector<std::shared_ptr<Request>>> active;  // per GPU thread
   std::map<long long, std::shared_ptr<Request>> finished;
   std::vector<femx_request_stats> unpolled;
@@ -144,11 +148,11 @@ struct femx_server {
   bool stop = false;
   std::string first_error;
   std::vector<std::thread> preps, gpus;
-  std::thread uni;
+  std::thread uni[2];
 
   void prep_loop();
   void gpu_loop(int t);
-  void union_loop();
+  void union_loop(int kind);
   void fail(const std::shared_ptr<Request> &r, femx_status st);
   void finish(const std::shared_ptr<Request> &r, femx_status st);
 };
@@ -238,10 +242,22 @@ void femx_server::prep_loop() {
           st = femx_supg_set_state(r->sh, zero.data());
           r->st = (hipStream_t)femx_supg_stream(r->sh);
         }
+        // a union member runs its first step on its own stream, checked for
+        // the exit condition there (the union's steps follow it)
+        r->joins_union = st == FEMX_OK && r->steps >= 2 && femx_supg_union_accepts(r->sh);
+        if (r->joins_union) {
+          int d = 0;
+          st = femx_supg_step(r->sh, 1, &d);
+          r->issued = r->done = d;
+        }
       }
     }
     for (auto &e : r->ev)
       if (st == FEMX_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) st = FEMX_ERR_HIP;
+    if (st == FEMX_EXIT_CONDITION) {
+      finish(r, st);  // the first step already exceeded EXITCOND
+      continue;
+    }
     if (st != FEMX_OK) {
       fail(r, st);
       continue;
@@ -252,21 +268,60 @@ void femx_server::prep_loop() {
     {
       std::lock_guard<std::mutex> lk(mu);
       r->stats.t_start = now_s();
-      (r->joins_union ? union_ready : ready).push_back(r);
+      if (r->joins_union) union_ready[r->kind == FEMX_REQ_VMS ? 0 : 1].push_back(r);
+      else ready.push_back(r);
     }
     cv.notify_all();
   }
 }
 
-void femx_server::union_loop() {
+namespace {
+// The union calls of one solver kind, so one loop serves both.
+struct UnionOps {
+  void *u = nullptr;
+  bool vms = true;
+  femx_status create() {
+    return vms ? femx_vms_union_create(reinterpret_cast<femx_vms_union **>(&u))
+               : femx_supg_union_create(reinterpret_cast<femx_supg_union **>(&u));
+  }
+  void destroy() {
+    if (vms) femx_vms_union_destroy(static_cast<femx_vms_union *>(u));
+    else femx_supg_union_destroy(static_cast<femx_supg_union *>(u));
+  }
+  hipStream_t stream() {
+    return (hipStream_t)(vms ? femx_vms_union_stream(static_cast<femx_vms_union *>(u))
+                             : femx_supg_union_stream(static_cast<femx_supg_union *>(u)));
+  }
+  femx_status add(Request &r, int steps) {
+    return vms ? femx_vms_union_add(static_cast<femx_vms_union *>(u), r.vh, steps)
+               : femx_supg_union_add(static_cast<femx_supg_union *>(u), r.sh, steps);
+  }
+  femx_status step(int k) {
+    return vms ? femx_vms_union_step(static_cast<femx_vms_union *>(u), k)
+               : femx_supg_union_step(static_cast<femx_supg_union *>(u), k);
+  }
+  int left(Request &r) {
+    return vms ? femx_vms_union_left(static_cast<femx_vms_union *>(u), r.vh)
+               : femx_supg_union_left(static_cast<femx_supg_union *>(u), r.sh);
+  }
+  femx_status remove(Request &r) {
+    return vms ? femx_vms_union_remove(static_cast<femx_vms_union *>(u), r.vh)
+               : femx_supg_union_remove(static_cast<femx_supg_union *>(u), r.sh);
+  }
+};
+}  // namespace
+
+void femx_server::union_loop(int kind) {
   (void)hipSetDevice(device);
-  femx_vms_union *u = nullptr;
-  if (femx_vms_union_create(&u) != FEMX_OK) {
+  UnionOps ops;
+  ops.vms = kind == 0;
+  if (ops.create() != FEMX_OK) {
     std::lock_guard<std::mutex> lk(mu);
     if (first_error.empty()) first_error = femx_last_error();
     return;
   }
-  hipStream_t ust = (hipStream_t)femx_vms_union_stream(u);
+  hipStream_t ust = ops.stream();
+  std::deque<std::shared_ptr<Request>> &queue = union_ready[kind];
   std::vector<std::shared_ptr<Request>> members;
   // enqueued chunks, oldest first: the event after the chunk and the members
   // whose steps end in it (removed from the union right after enqueueing it)
@@ -276,21 +331,35 @@ void femx_server::union_loop() {
   };
   std::deque<Chunk> inflight;
   std::vector<hipEvent_t> spare;
+  // a member whose steps are all enqueued has left the union; once they have
+  // run, its state is read back (a SUPG member's steps are first checked for
+  // the exit condition, as on its own stream)
+  auto retire = [&](const std::shared_ptr<Request> &r) {
+    femx_status st = FEMX_OK;
+    if (r->kind == FEMX_REQ_SUPG) {
+      int d = 0;
+      st = femx_supg_finish(r->sh, &d);
+      r->done += d;
+    } else {
+      r->done = r->steps;
+    }
+    finish(r, st);
+  };
   for (;;) {
     std::vector<std::shared_ptr<Request>> fresh;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv.wait_for(lk, std::chrono::microseconds(members.empty() && inflight.empty() ? 2000 : 50),
-                  [&] { return stop || !union_ready.empty() || !members.empty() || !inflight.empty(); });
+                  [&] { return stop || !queue.empty() || !members.empty() || !inflight.empty(); });
       if (stop) break;
-      while (!union_ready.empty()) {
-        fresh.push_back(union_ready.front());
-        union_ready.pop_front();
+      while (!queue.empty()) {
+        fresh.push_back(queue.front());
+        queue.pop_front();
       }
     }
     bool changed = false;
     for (auto &r : fresh) {
-      const femx_status st = femx_vms_union_add(u, r->vh, r->steps - 1);
+      const femx_status st = ops.add(*r, r->steps - r->issued);
       if (st != FEMX_OK) {
         fail(r, st);
         continue;
@@ -300,17 +369,14 @@ void femx_server::union_loop() {
     }
     // retire the members of completed chunks, oldest first
     while (!inflight.empty() && hipEventQuery(inflight.front().ev) != hipErrorNotReady) {
-      for (auto &r : inflight.front().done) {
-        r->done = r->steps;
-        finish(r, FEMX_OK);
-      }
+      for (auto &r : inflight.front().done) retire(r);
       spare.push_back(inflight.front().ev);
       inflight.pop_front();
       changed = true;
     }
     // keep two chunks in flight while members have steps left
     if (!members.empty() && inflight.size() < 2) {
-      femx_status st = femx_vms_union_step(u, chunk);
+      femx_status st = ops.step(chunk);
       Chunk c{nullptr, {}};
       if (st == FEMX_OK) {
         if (!spare.empty()) {
@@ -323,7 +389,7 @@ void femx_server::union_loop() {
       if (st == FEMX_OK && hipEventRecord(c.ev, ust) != hipSuccess) st = FEMX_ERR_HIP;
       if (st != FEMX_OK) {
         for (auto &r : members) {
-          (void)femx_vms_union_remove(u, r->vh);
+          (void)ops.remove(*r);
           fail(r, st);
         }
         members.clear();
@@ -331,8 +397,8 @@ void femx_server::union_loop() {
         // members whose steps end in this chunk leave the union now; their
         // streams wait for it, and they are read back once it has completed
         for (size_t i = 0; i < members.size();) {
-          if (femx_vms_union_left(u, members[i]->vh) == 0) {
-            (void)femx_vms_union_remove(u, members[i]->vh);
+          if (ops.left(*members[i]) == 0) {
+            (void)ops.remove(*members[i]);
             c.done.push_back(members[i]);
             members.erase(members.begin() + i);
           } else {
@@ -353,11 +419,11 @@ void femx_server::union_loop() {
     (void)hipEventDestroy(c.ev);
   }
   for (auto &r : members) {
-    (void)femx_vms_union_remove(u, r->vh);
+    (void)ops.remove(*r);
     r->release();
   }
   for (auto e : spare) (void)hipEventDestroy(e);
-  femx_vms_union_destroy(u);
+  ops.destroy();
 }
 
 void femx_server::gpu_loop(int t) {
@@ -442,7 +508,7 @@ extern "C" femx_status femx_server_create(size_t hbm_budget_bytes, int max_activ
   s->active.resize(gpu_threads);
   for (int i = 0; i < prep_threads; i++) s->preps.emplace_back([s] { s->prep_loop(); });
   for (int i = 0; i < gpu_threads; i++) s->gpus.emplace_back([s, i] { s->gpu_loop(i); });
-  s->uni = std::thread([s] { s->union_loop(); });
+  for (int k = 0; k < 2; k++) s->uni[k] = std::thread([s, k] { s->union_loop(k); });
   *out = s;
   return FEMX_OK;
 }
@@ -476,6 +542,7 @@ extern "C" femx_status femx_server_submit(femx_server *s, const femx_request *q,
   r->pres_val.assign(b.pres_val, b.pres_val + b.n_pres);
   r->vms = q->vms;
   r->supg = q->supg;
+  // VMS: known from the request; SUPG: decided once its handle exists
   r->joins_union = q->kind == FEMX_REQ_VMS && q->vms.precision == FEMX_MIXED && q->vms.tot_dual_steps == 0 &&
                    q->steps >= 2;
   const bool vms = q->kind == FEMX_REQ_VMS;
@@ -523,7 +590,8 @@ extern "C" femx_status femx_server_drain(femx_server *s, double timeout_s) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
   const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
-    return s->pending.empty() && s->ready.empty() && s->union_ready.empty() && s->n_resident == 0;
+    return s->pending.empty() && s->ready.empty() && s->union_ready[0].empty() && s->union_ready[1].empty() &&
+           s->n_resident == 0;
   });
   if (!ok) return set_error("drain timed out"), FEMX_ERR_ARG;
   if (!s->first_error.empty()) return set_error(s->first_error), FEMX_ERR_HIP;
@@ -552,14 +620,16 @@ extern "C" femx_status femx_server_destroy(femx_server *s) {
   s->cv.notify_all();
   for (auto &t : s->preps) t.join();
   for (auto &t : s->gpus) t.join();
-  if (s->uni.joinable()) s->uni.join();
+  for (auto &t : s->uni)
+    if (t.joinable()) t.join();
   for (auto &list : s->active)
     for (auto &r : list) {
       if (r->st) (void)hipStreamSynchronize(r->st);
       r->release();
     }
   for (auto &r : s->ready) r->release();
-  for (auto &r : s->union_ready) r->release();
+  for (auto &q : s->union_ready)
+    for (auto &r : q) r->release();
   delete s;
   return FEMX_OK;
 }

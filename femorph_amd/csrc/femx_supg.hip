@@ -65,7 +65,7 @@ __device__ __forceinline__ void node_extrema(const double4 u, double v[6], doubl
 }
 
 // Block-wide velocity extrema and max |u|^2 (columns D_UMIN .. D_UMIN + 5
-// and D_UMOD) in one pass; thread 0 stores them to out[blockIdx.x * D_COLS].
+// and D_UMOD) in one pass; thread 0 stores them to the block's row out.
 template <int NT>
 __device__ __forceinline__ void block_node_store(double v[6], double m2, double *out) {
   __shared__ double wv[NT / 64][7];
@@ -88,12 +88,12 @@ __device__ __forceinline__ void block_node_store(double v[6], double m2, double 
       const bool mn = i < 6 && d_is_min(D_UMIN + i);
       double r = wv[0][i];
       for (int w = 1; w < NT / 64; w++) r = mn ? fmin(r, wv[w][i]) : fmax(r, wv[w][i]);
-      out[(long)blockIdx.x * D_COLS + (i < 6 ? D_UMIN + i : D_UMOD)] = r;
+      out[i < 6 ? D_UMIN + i : D_UMOD] = r;
     }
 }
 
 // Block-wide min/max of the columns [c0, c0+K) of a diagnostic row; thread 0
-// stores the block's values to out[blockIdx.x * D_COLS + c]. NaN never wins
+// stores the block's values to its row out[c]. NaN never wins
 // (like the reference's `if (x > max)` tests).
 template <int NT, int K>
 __device__ __forceinline__ void block_diag_store(double v[K], int c0, double *out) {
@@ -115,7 +115,7 @@ __device__ __forceinline__ void block_diag_store(double v[K], int c0, double *ou
     for (int i = 0; i < K; i++) {
       double m = wv[0][i];
       for (int w = 1; w < NT / 64; w++) m = d_is_min(c0 + i) ? fmin(m, wv[w][i]) : fmax(m, wv[w][i]);
-      out[(long)blockIdx.x * D_COLS + c0 + i] = m;
+      out[c0 + i] = m;
     }
 }
 
@@ -206,8 +206,9 @@ __device__ __forceinline__ void supg_node(const double F[3], double4 fprev, doub
 // Two waves per SIMD: the hexa8 variants then spill ~140-240 B per lane to
 // scratch, which costs less than running one wave per SIMD (0.58 vs 0.77 ms
 // per launch on the 2.1M-hexa cavity).
+// Element block blk of one mesh; its diagnostic row is eblk[row].
 template <typename T, int NPE, bool BHAT, bool DYN, bool PRESS, bool PSEP, bool DIAG>
-__global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> a) {
+__device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int blk, const int row) {
   constexpr int NGP = Shape<NPE>::NGP;
   constexpr int SB = threads_for<NPE>();
   __shared__ double sx[256], sy[256], sz[256], sm[256];
@@ -224,7 +225,6 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
       return;
     }
   }
-  const int blk = xcd_block(blockIdx.x, a.nblk);
   const int n0 = a.node_off[blk], nn = a.node_off[blk + 1] - n0;
   const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
   const bool has_e = tid < ne;
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
     }
   }
   if (!DYN) return;
-  if (DIAG) block_diag_store<SB, 11>(dg, D_TAU, a.eblk);
+  if (DIAG) block_diag_store<SB, 11>(dg, D_TAU, a.eblk + (long)row * D_COLS);
   __syncthreads();
   // velocity extrema and max |u|^2 of the new states this block completes
   // (get_solution_extrema :604-655), into the node columns of its row
@@ -619,7 +619,12 @@ __global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> 
         static_cast<double4 *>(a.partials)[psl] = make_double4(F[0], F[1], F[2], 0.0);
     }
   }
-  block_node_store<SB>(nv, nm2[0], a.eblk);
+  block_node_store<SB>(nv, nm2[0], a.eblk + (long)row * D_COLS);
+}
+
+template <typename T, int NPE, bool BHAT, bool DYN, bool PRESS, bool PSEP, bool DIAG>
+__global__ __launch_bounds__(threads_for<NPE>(), 2) void supg_block(SupgArgs<T> a) {
+  supg_block_run<T, NPE, BHAT, DYN, PRESS, PSEP, DIAG>(a, xcd_block(blockIdx.x, a.nblk), blockIdx.x);
 }
 
 
@@ -635,9 +640,10 @@ constexpr int DIAG_ONE_PASS = 2048;  // up to this many rows, one workgroup and 
 // a-CFL maxima per step, :1078-1079): the kernel that writes the step's row
 // folds in the previous step's row (prev, null on the first step).
 // One workgroup of 256 threads folds the rows of workgroups part, part +
-// nparts, ... into out[0 .. D_COLS).  AT: the rows were written by other
-// workgroups of the running kernel, so they are read with device-scope
-// atomic loads (coherent across the XCDs' L2 caches).
+// nparts, ... into out[0 .. D_COLS).  AT: the fix-up's rows (nblk) were
+// written by other workgroups of the running kernel, so they are read with
+// device-scope atomic loads (coherent across the XCDs' L2 caches); the
+// element blocks' rows come from the previous kernel.
 template <bool AT>
 __device__ __forceinline__ double ld_row(const double *p) {
   if constexpr (AT) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -664,7 +670,7 @@ __device__ __forceinline__ void diag_fold(const double *eblk, int ne_blk, const 
 #pragma unroll
     for (int c = 0; c < D_COLS; c++)
       if (with_elem || c < 6 || c == D_UMOD) {
-        const double x = ld_row<AT>(eblk + (long)b * D_COLS + c);
+        const double x = eblk[(long)b * D_COLS + c];
         m[c] = d_is_min(c) ? fmin(m[c], x) : fmax(m[c], x);
       }
   }
@@ -737,32 +743,37 @@ struct SupgFix {
 template <typename T>
 __device__ __forceinline__ void fixup_node(const SupgFix &a, int s, double nv[6], double &nm2);
 
+// Fix-up workgroup wg of nwg of one mesh.
 template <typename T>
-__global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
-  using T4 = typename Vec4<T>::type;
+__device__ __forceinline__ void supg_fixup_run(const SupgFix &a, const int wg, const int nwg) {
   if (halted(a.halt)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = wg * 256 + (int)threadIdx.x;
   double nv[6], nm2[1] = {0.0};
 #pragma unroll
   for (int i = 0; i < 6; i++) nv[i] = d_identity(D_UMIN + i);
   if (s < a.n_sh) fixup_node<T>(a, s, nv, nm2[0]);
   // this workgroup's row of velocity extrema and max |u|^2 (:604-655)
-  block_node_store<256>(nv, nm2[0], a.nblk);
+  block_node_store<256>(nv, nm2[0], a.nblk + (long)wg * D_COLS);
   if (a.fold_row) {
     // the last workgroup to finish folds every row (its own and the
     // others', made visible by their fences before they counted)
     __shared__ bool last;
     if (threadIdx.x == 0) {
       __threadfence();
-      last = atomicAdd(a.fold_count, 1u) == gridDim.x - 1;
+      last = atomicAdd(a.fold_count, 1u) == (unsigned)nwg - 1u;
     }
     __syncthreads();
     if (last) {
       __threadfence();
-      diag_fold<true>(a.eblk, a.ne_blk, a.nblk, gridDim.x, a.with_elem, a.fold_row, a.fold_prev, 0, 1);
+      diag_fold<true>(a.eblk, a.ne_blk, a.nblk, nwg, a.with_elem, a.fold_row, a.fold_prev, 0, 1);
       if (threadIdx.x == 0) *a.fold_count = 0u;
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void supg_fixup(SupgFix a) {
+  supg_fixup_run<T>(a, blockIdx.x, gridDim.x);
 }
 
 template <typename T>
@@ -889,6 +900,11 @@ struct femx_supg {
   double pend_t0 = 0.0;
   int pend_cur0 = 0;
   KernelTimer timer;
+  // member of a union of handles (femx_supg_union_*): its time, parity,
+  // step number and steps left live on the device while it is a member
+  femx_supg_union *in_union = nullptr;
+  int u_left = 0;
+  DevBuf<unsigned char> dyn;  // one SupgDyn
 };
 
 namespace {
@@ -1119,6 +1135,7 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
 
 extern "C" femx_status femx_supg_destroy(femx_supg *s) {
   if (!s) return FEMX_OK;
+  if (s->in_union) (void)femx_supg_union_remove(s->in_union, s);
   hipStream_t st = s->st;
   if (st) (void)hipStreamSynchronize(st);
   {
@@ -1131,6 +1148,7 @@ extern "C" femx_status femx_supg_destroy(femx_supg *s) {
 
 extern "C" femx_status femx_supg_set_state(femx_supg *s, const double *sol) {
   if (!s || !sol) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   Mesh &m = *s->mesh;
   std::vector<double4> h(m.N);
   for (int n = 0; n < m.N; n++) h[n] = make_double4(sol[3L * n], sol[3L * n + 1], sol[3L * n + 2], 0.0);
@@ -1244,6 +1262,7 @@ femx_status supg_finish(femx_supg *s, int *steps_done) {
 
 extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done) {
   if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   if (steps_done) *steps_done = 0;
   if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
   femx_status r = supg_enqueue(s, nsteps);
@@ -1253,12 +1272,14 @@ extern "C" femx_status femx_supg_step(femx_supg *s, int nsteps, int *steps_done)
 
 extern "C" femx_status femx_supg_step_async(femx_supg *s, int nsteps) {
   if (!s || nsteps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   if (s->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
   return supg_enqueue(s, nsteps);
 }
 
 extern "C" femx_status femx_supg_finish(femx_supg *s, int *steps_done) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   if (s->halted && s->pend_n == 0) {
     if (steps_done) *steps_done = 0;
     return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
@@ -1269,6 +1290,7 @@ extern "C" femx_status femx_supg_finish(femx_supg *s, int *steps_done) {
 extern "C" femx_status femx_supg_get_state(femx_supg *s, double *sol, double *pres, double *lam, double *regime,
                                            double *time) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   Mesh &m = *s->mesh;
   if (s->pend_n > 0) {
     const femx_status f = supg_finish(s, nullptr);
@@ -1327,6 +1349,7 @@ extern "C" femx_status femx_supg_get_log(femx_supg *s, double *cols19) {
 
 extern "C" femx_status femx_supg_set_diagnostics(femx_supg *s, int enable) {
   if (!s) return set_error("null argument"), FEMX_ERR_ARG;
+  if (s->in_union) return set_error("the handle is in a union"), FEMX_ERR_ARG;
   s->diag_enabled = enable != 0;
   return FEMX_OK;
 }
@@ -1351,3 +1374,330 @@ extern "C" int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int m
   if (!s) return -1;
   return s->timer.export_to(out, max_entries);
 }
+
+// ------------------------------------------------------- unions of handles
+// Continuous batching of resident SUPG solves: FEMX_MIXED hexa8 handles with
+// B-hat, no fixed DOFs and no element diagnostics, whose log rows the
+// fix-up's last workgroup folds (at most DIAG_ONE_PASS rows), are advanced
+// together, one launch of the element kernel, one of the fix-up and one of
+// the per-member fold of the log rows and tick, per step for all members,
+// each by its own number of steps.  A member's
+// step is its handle's own (the same blocks, sums and order): its time,
+// state parity and step number come from a device record the tick advances
+// with the same arithmetic as the host's loop, and its exit condition is
+// checked as on its own stream (femx_supg_finish after it leaves).
+namespace {
+
+struct SupgDyn {
+  double t;  // time of the last step run
+  int par;   // current state buffer
+  int lt;    // loop_time of the last step run
+  int left;  // steps left
+};
+
+struct SupgMember {
+  SupgArgs<float> a;  // per step: U, Upress, Un, k.t, step_id, prev_row
+  SupgFix f;          // per step: U, Un, k.t, fold_row, fold_prev
+  double4 *Ub[2];
+  double *ring;
+  SupgDyn *dyn;
+  double dt;
+  int nwg;  // fix-up workgroups
+};
+
+__device__ __forceinline__ int supg_member_of(const int *off, int nm, int g) {
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uni(off[mid]) <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(threads_for<8>(), 2) void supg_block_union(const SupgMember *mem, const int *blk_off,
+                                                                        int nm, int ntot) {
+  const int g = xcd_block(blockIdx.x, ntot);
+  const int h = supg_member_of(blk_off, nm, g);
+  const SupgMember &me = mem[h];
+  if (uni(me.dyn->left) <= 0) return;
+  const int par = uni(me.dyn->par), lt = uni(me.dyn->lt) + 1;
+  SupgArgs<float> a = me.a;
+  a.U = me.Ub[par];
+  a.Upress = a.U;
+  a.Un = me.Ub[par ^ 1];
+  a.k.t = uni(me.dyn->t) + me.dt;  // currentTime += timeStep (:1041)
+  a.step_id = lt;
+  a.prev_row = lt > 1 ? me.ring + (long)((lt - 1) % RING) * D_COLS : nullptr;
+  const int blk = g - uni(blk_off[h]);
+  supg_block_run<float, 8, true, true, true, false, false>(a, blk, blk);
+}
+
+__global__ __launch_bounds__(256) void supg_fixup_union(const SupgMember *mem, const int *fx_off, int nm) {
+  const int g = blockIdx.x;
+  const int h = supg_member_of(fx_off, nm, g);
+  const SupgMember &me = mem[h];
+  if (uni(me.dyn->left) <= 0) return;
+  const int par = uni(me.dyn->par), lt = uni(me.dyn->lt) + 1;
+  SupgFix f = me.f;
+  f.U = me.Ub[par];
+  f.Un = me.Ub[par ^ 1];
+  f.k.t = uni(me.dyn->t) + me.dt;
+  (void)lt;
+  supg_fixup_run<double>(f, g - uni(fx_off[h]), me.nwg);
+}
+
+// End of a union step, one workgroup per member with steps left: the step's
+// rows (element blocks and fix-up, both written by the previous kernels)
+// folded into its log row, then the member moves on by one step.
+__global__ __launch_bounds__(256) void supg_union_tick(const SupgMember *mem) {
+  const SupgMember &me = mem[blockIdx.x];
+  SupgDyn *d = me.dyn;
+  if (uni(d->left) <= 0) return;
+  const int lt = uni(d->lt) + 1;
+  if (!halted(me.a.halt))
+    diag_fold<false>(me.f.eblk, me.f.ne_blk, me.f.nblk, me.nwg, 0, me.ring + (long)(lt % RING) * D_COLS,
+                     lt > 1 ? me.ring + (long)((lt - 1) % RING) * D_COLS : nullptr, 0, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d->t = d->t + me.dt;
+    d->par ^= 1;
+    d->lt = lt;
+    d->left -= 1;
+  }
+}
+
+__global__ void supg_union_join(SupgDyn *d, double t, int par, int lt, int left) {
+  d->t = t;
+  d->par = par;
+  d->lt = lt;
+  d->left = left;
+}
+
+bool union_member_ok(const femx_supg *s) {
+  const Blocks &b = s->mesh->blk;
+  const int nnb = b.n_sh ? grid_for(b.n_sh, 256) : 0;
+  return s->p.precision == FEMX_MIXED && s->mesh->npe == 8 && s->p.use_bhat && s->p.fixed_dofs == 0 &&
+         !s->diag_enabled && nnb > 0 && nnb + b.nb <= DIAG_ONE_PASS;
+}
+
+}  // namespace
+
+struct femx_supg_union {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<femx_supg *> mem;
+  bool dirty = true;
+  int nblk = 0, nfx = 0;
+  DevBuf<SupgMember> tab;
+  DevBuf<int> off;  // blk_off [nm + 1] then fx_off [nm + 1]
+  size_t cap = 0;
+  // pinned staging of the table uploads, a ring of slots, each reused once
+  // the copy that read it has run
+  static constexpr int NSLOT = 3;
+  unsigned char *pin[NSLOT] = {nullptr, nullptr, nullptr};
+  size_t pin_cap[NSLOT] = {0, 0, 0};
+  hipEvent_t pin_ev[NSLOT] = {nullptr, nullptr, nullptr};
+  int pin_next = 0;
+};
+
+namespace {
+
+femx_status supg_union_upload(femx_supg_union *u) {
+  const int nm = (int)u->mem.size();
+  std::vector<SupgMember> t(nm);
+  std::vector<int> off(2L * (nm + 1), 0);
+  for (int i = 0; i < nm; i++) {
+    femx_supg *h = u->mem[i];
+    SupgMember &e = t[i];
+    const bool euler = h->p.use_euler != 0;  // the first step ran on the handle's own stream
+    e.a = supg_args<float>(h, euler, 0.0);
+    e.a.Unp = nullptr;
+    // the fix-up's arguments, as launch_fix builds them
+    const Blocks &b = h->mesh->blk;
+    SupgFix &f = e.f;
+    f.n_sh = b.n_sh;
+    f.sh_node = b.d_sh_node.p;
+    f.sh_off = b.d_sh_off.p;
+    f.sh_slot = b.d_sh_slot.p;
+    f.sh_first4 = b.d_sh_first4.p;
+    f.partials = h->partials.p;
+    f.xyzm = h->xyzm.p;
+    f.U = nullptr;
+    f.Un = nullptr;
+    f.Unp = nullptr;
+    f.Fprev = h->Fprev.p;
+    f.node_rec = h->bc.node_rec.p;
+    f.rec = h->bc.rec.p;
+    f.row_off = h->bc.row_off.p;
+    f.rows = h->bc.rows.p;
+    f.err = h->bc.err.p;
+    f.halt = h->halt.p;
+    f.nblk = h->nblk.p;
+    f.fold_row = nullptr;
+    f.fold_prev = nullptr;
+    f.eblk = h->eblk.p;
+    f.ne_blk = b.nb;
+    f.with_elem = 0;
+    f.fold_count = h->fold_count.p;
+    f.k = supg_const(h, 0.0, euler);
+    for (int k = 0; k < 2; k++) e.Ub[k] = h->U[k].p;
+    e.ring = h->ring.p;
+    e.dyn = reinterpret_cast<SupgDyn *>(h->dyn.p);
+    e.dt = h->p.dt;
+    e.nwg = grid_for(b.n_sh, 256);
+    off[i + 1] = off[i] + b.nb;
+    off[nm + 1 + i + 1] = off[nm + 1 + i] + e.nwg;
+  }
+  u->nblk = off[nm];
+  u->nfx = off[2 * nm + 1];
+  femx_status r;
+  if ((size_t)nm > u->cap) {
+    // grow (the old table may still be read by enqueued steps: wait for them)
+    FEMX_HIP(hipStreamSynchronize(u->st));
+    const size_t cap = std::max<size_t>(2 * (size_t)nm, 16);
+    if ((r = u->tab.alloc(cap)) != FEMX_OK || (r = u->off.alloc(2 * (cap + 1))) != FEMX_OK) return r;
+    u->cap = cap;
+  }
+  const size_t bytes_t = nm * sizeof(SupgMember), bytes_o = off.size() * sizeof(int);
+  const int k = u->pin_next;
+  u->pin_next = (k + 1) % femx_supg_union::NSLOT;
+  if (u->pin_ev[k]) FEMX_HIP(hipEventSynchronize(u->pin_ev[k]));
+  if (u->pin_cap[k] < bytes_t + bytes_o) {
+    if (u->pin[k]) FEMX_HIP(hipHostFree(u->pin[k]));
+    u->pin[k] = nullptr;
+    const size_t want = 2 * (bytes_t + bytes_o) + 4096;
+    FEMX_HIP(hipHostMalloc((void **)&u->pin[k], want, hipHostMallocDefault));
+    u->pin_cap[k] = want;
+  }
+  if (nm > 0) {
+    std::memcpy(u->pin[k], t.data(), bytes_t);
+    FEMX_HIP(hipMemcpyAsync(u->tab.p, u->pin[k], bytes_t, hipMemcpyHostToDevice, u->st));
+  }
+  std::memcpy(u->pin[k] + bytes_t, off.data(), bytes_o);
+  FEMX_HIP(hipMemcpyAsync(u->off.p, u->pin[k] + bytes_t, bytes_o, hipMemcpyHostToDevice, u->st));
+  if (!u->pin_ev[k]) FEMX_HIP(hipEventCreateWithFlags(&u->pin_ev[k], hipEventDisableTiming));
+  FEMX_HIP(hipEventRecord(u->pin_ev[k], u->st));
+  u->dirty = false;
+  return FEMX_OK;
+}
+
+}  // namespace
+
+extern "C" int femx_supg_union_accepts(femx_supg *h) {
+  return h && !h->in_union && !h->halted && union_member_ok(h) ? 1 : 0;
+}
+
+extern "C" femx_status femx_supg_union_create(femx_supg_union **out) {
+  if (!out) return set_error("null argument"), FEMX_ERR_ARG;
+  if (femx_device_count() <= 0) return set_error("no HIP device visible"), FEMX_ERR_NO_DEVICE;
+  femx_supg_union *u = new femx_supg_union();
+  (void)hipGetDevice(&u->device);
+  if (hipStreamCreateWithFlags(&u->st, hipStreamNonBlocking) != hipSuccess) {
+    delete u;
+    return set_error("hipStreamCreate failed"), FEMX_ERR_HIP;
+  }
+  *out = u;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_union_destroy(femx_supg_union *u) {
+  if (!u) return FEMX_OK;
+  while (!u->mem.empty()) (void)femx_supg_union_remove(u, u->mem.back());
+  (void)hipStreamSynchronize(u->st);
+  for (int k = 0; k < femx_supg_union::NSLOT; k++) {
+    if (u->pin[k]) (void)hipHostFree(u->pin[k]);
+    if (u->pin_ev[k]) (void)hipEventDestroy(u->pin_ev[k]);
+  }
+  hipStream_t st = u->st;
+  delete u;
+  (void)hipStreamDestroy(st);
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_union_add(femx_supg_union *u, femx_supg *h, int steps) {
+  if (!u || !h || steps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  if (h->in_union) return set_error("handle already in a union"), FEMX_ERR_ARG;
+  if (!union_member_ok(h))
+    return set_error("union members: FEMX_MIXED hexa8 with B-hat, no fixed DOFs, no diagnostics, at most 2048 "
+                     "element blocks and fix-up workgroups together, some shared nodes"),
+           FEMX_ERR_UNSUPPORTED;
+  if (h->loop_time < 1) return set_error("union members run their first step on their own"), FEMX_ERR_ARG;
+  if (h->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;
+  femx_status r;
+  if (!h->dyn.p && (r = h->dyn.alloc(sizeof(SupgDyn))) != FEMX_OK) return r;
+  // the union's stream waits for the work already queued on the handle's
+  hipEvent_t ev = nullptr;
+  FEMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const bool ok = hipEventRecord(ev, h->st) == hipSuccess && hipStreamWaitEvent(u->st, ev, 0) == hipSuccess;
+  (void)hipEventDestroy(ev);
+  if (!ok) return set_error("hipStreamWaitEvent failed"), FEMX_ERR_HIP;
+  supg_union_join<<<1, 1, 0, u->st>>>(reinterpret_cast<SupgDyn *>(h->dyn.p), h->time, h->cur, (int)h->loop_time,
+                                      steps);
+  FEMX_HIP(hipGetLastError());
+  h->in_union = u;
+  h->u_left = steps;
+  u->mem.push_back(h);
+  u->dirty = true;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_union_step(femx_supg_union *u, int k) {
+  if (!u || k < 0) return set_error("bad argument"), FEMX_ERR_ARG;
+  femx_status r;
+  if (u->dirty && (r = supg_union_upload(u)) != FEMX_OK) return r;
+  const int nm = (int)u->mem.size();
+  if (nm == 0 || k == 0) return FEMX_OK;
+  const int *blk_off = u->off.p, *fx_off = u->off.p + nm + 1;
+  for (int i = 0; i < k; i++) {
+    supg_block_union<<<u->nblk, threads_for<8>(), 0, u->st>>>(u->tab.p, blk_off, nm, u->nblk);
+    supg_fixup_union<<<u->nfx, 256, 0, u->st>>>(u->tab.p, fx_off, nm);
+    supg_union_tick<<<nm, 256, 0, u->st>>>(u->tab.p);
+  }
+  FEMX_HIP(hipGetLastError());
+  // the host's view of every member: the steps enqueued as on its own stream
+  // (femx_supg_finish checks them for the exit condition after it leaves)
+  for (femx_supg *h : u->mem) {
+    const int n = std::min(k, h->u_left);
+    if (n > 0 && h->pend_n == 0) {
+      h->pend_first = h->loop_time;
+      h->pend_t0 = h->time;
+      h->pend_cur0 = h->cur;
+    }
+    for (int i = 0; i < n; i++) h->time += h->p.dt;  // currentTime += timeStep (:1041)
+    h->loop_time += n;
+    h->cur ^= n & 1;
+    h->pend_n += n;
+    h->u_left -= n;
+  }
+  return FEMX_OK;
+}
+
+extern "C" int femx_supg_union_left(femx_supg_union *u, femx_supg *h) {
+  if (!u || !h || h->in_union != u) return -1;
+  return h->u_left;
+}
+
+extern "C" femx_status femx_supg_union_remove(femx_supg_union *u, femx_supg *h) {
+  if (!u || !h || h->in_union != u) return set_error("handle not in this union"), FEMX_ERR_ARG;
+  // the handle's stream waits for the union's work on it
+  hipEvent_t ev = nullptr;
+  FEMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const bool ok = hipEventRecord(ev, u->st) == hipSuccess && hipStreamWaitEvent(h->st, ev, 0) == hipSuccess;
+  (void)hipEventDestroy(ev);
+  if (!ok) return set_error("hipStreamWaitEvent failed"), FEMX_ERR_HIP;
+  u->mem.erase(std::find(u->mem.begin(), u->mem.end(), h));
+  h->in_union = nullptr;
+  h->u_left = 0;
+  u->dirty = true;
+  return FEMX_OK;
+}
+
+extern "C" femx_status femx_supg_union_sync(femx_supg_union *u) {
+  if (!u) return set_error("null argument"), FEMX_ERR_ARG;
+  FEMX_HIP(hipStreamSynchronize(u->st));
+  return FEMX_OK;
+}
+
+extern "C" void *femx_supg_union_stream(femx_supg_union *u) { return u ? (void *)u->st : nullptr; }
+

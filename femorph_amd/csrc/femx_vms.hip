@@ -483,19 +483,6 @@ __device__ __forceinline__ double node_update(const double4 base, double im, con
   return m2;
 }
 
-// A value every lane of the wave holds, moved to SGPRs bit for bit
-// (readfirstlane moves 32-bit integers: wider or floating values go through
-// their bit patterns, never a numeric conversion).  The kernel also writes
-// through pointers of the types it reads, so without this the compiler keeps
-// per-block values in VGPRs.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uni(double v) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)uni((int)(unsigned)(b & 0xffffffffLL));
-  const unsigned hi = (unsigned)uni((int)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ double4 uni(double4 v) { return make_double4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
 
 // One element block `blk` of one mesh; its per-block maximum goes to
 // a.bmax[row].

@@ -134,6 +134,15 @@ SIGNATURES = [
     ("femx_ffd_morph_noalloc", C.c_int, [C.POINTER(FFDBox), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("femx_supg_step_async", C.c_int, [C.c_void_p, C.c_int]),
     ("femx_supg_finish", C.c_int, [C.c_void_p, _ip]),
+    ("femx_supg_union_accepts", C.c_int, [C.c_void_p]),
+    ("femx_supg_union_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("femx_supg_union_destroy", C.c_int, [C.c_void_p]),
+    ("femx_supg_union_add", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("femx_supg_union_step", C.c_int, [C.c_void_p, C.c_int]),
+    ("femx_supg_union_left", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("femx_supg_union_remove", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("femx_supg_union_sync", C.c_int, [C.c_void_p]),
+    ("femx_supg_union_stream", C.c_void_p, [C.c_void_p]),
     ("femx_server_create", C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("femx_server_submit", C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)]),
     ("femx_server_poll", C.c_int, [C.c_void_p, C.POINTER(RequestStats), C.c_int]),
@@ -371,6 +380,42 @@ class VMSUnion:
         _check(load().femx_vms_union_sync(self._h))
 
 
+class SUPGUnion:
+    """Continuous batching of SUPG handles (femx_supg_union_*): FEMX_MIXED
+    hexa8 members advanced together, one launch per phase per step, each by
+    its own number of steps; bit for bit the handles stepped on their own.
+    After remove(), the member's finish() checks its steps for the exit
+    condition."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(load().femx_supg_union_create(C.byref(h)))
+        self._h = h
+        self._members = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.femx_supg_union_destroy(self._h)
+            self._h = None
+
+    def add(self, solver, steps):
+        _check(load().femx_supg_union_add(self._h, solver._h, int(steps)))
+        self._members[id(solver)] = solver
+
+    def step(self, k):
+        _check(load().femx_supg_union_step(self._h, int(k)))
+
+    def left(self, solver):
+        return load().femx_supg_union_left(self._h, solver._h)
+
+    def remove(self, solver):
+        _check(load().femx_supg_union_remove(self._h, solver._h))
+        self._members.pop(id(solver), None)
+
+    def sync(self):
+        _check(load().femx_supg_union_sync(self._h))
+
+
 class VMSBatch:
     """Many VMS tet4 solves advanced together (femx_vms_batch_*).
 
@@ -535,6 +580,20 @@ class SUPGSolver:
         """Runs n steps; returns the number done (fewer if the exit condition fired)."""
         done = C.c_int(0)
         st = load().femx_supg_step(self._h, n, C.byref(done))
+        if st == EXIT_CONDITION:
+            return done.value
+        _check(st)
+        return done.value
+
+    def step_async(self, n=1):
+        """Enqueues n steps without waiting (femx_supg_step_async)."""
+        _check(load().femx_supg_step_async(self._h, n))
+
+    def finish(self):
+        """Waits for the enqueued steps (also a union member's, after it
+        left) and returns the number done (femx_supg_finish)."""
+        done = C.c_int(0)
+        st = load().femx_supg_finish(self._h, C.byref(done))
         if st == EXIT_CONDITION:
             return done.value
         _check(st)

@@ -76,6 +76,7 @@ typedef struct femx_mesh femx_mesh;
 typedef struct femx_vms femx_vms;
 typedef struct femx_vms_union femx_vms_union;
 typedef struct femx_supg femx_supg;
+typedef struct femx_supg_union femx_supg_union;
 
 /* Text of the last error on this thread ("" if none). */
 const char *femx_last_error(void);
@@ -301,6 +302,30 @@ void *femx_supg_stream(femx_supg *s);
  * exactly as femx_supg_step would have for the same steps. */
 femx_status femx_supg_step_async(femx_supg *s, int nsteps);
 femx_status femx_supg_finish(femx_supg *s, int *steps_done);
+
+/* ---------------------------------------------- unions of SUPG handles */
+/* Continuous batching of resident SUPG solves (no counterpart in the
+ * reference, which runs one solve per process), as femx_vms_union_*: one
+ * launch of the element kernel, one of the fix-up and one tick per step for
+ * all members, each by its own number of steps; bit for bit each handle
+ * stepped on its own.  Members: FEMX_MIXED hexa8 with B-hat, no fixed DOFs,
+ * no diagnostics, at most 2048 element blocks and fix-up workgroups
+ * together, and their first step run on their own.  A member's union steps
+ * count as enqueued steps of its own: after femx_supg_union_remove,
+ * femx_supg_finish checks them for the exit condition (a step past one
+ * whose max |u| exceeded EXITCOND does not run) and femx_supg_get_state
+ * reads the state.  While in a union a handle is not stepped, set, read or
+ * switched to diagnostics on its own. */
+/* 1 if h can join a union (its options and sizes allow it), else 0. */
+int femx_supg_union_accepts(femx_supg *h);
+femx_status femx_supg_union_create(femx_supg_union **out);
+femx_status femx_supg_union_destroy(femx_supg_union *u);
+femx_status femx_supg_union_add(femx_supg_union *u, femx_supg *h, int steps);
+femx_status femx_supg_union_step(femx_supg_union *u, int k);
+int femx_supg_union_left(femx_supg_union *u, femx_supg *h);
+femx_status femx_supg_union_remove(femx_supg_union *u, femx_supg *h);
+femx_status femx_supg_union_sync(femx_supg_union *u);
+void *femx_supg_union_stream(femx_supg_union *u);
 
 /* -------------------------------------------------------------- server */
 /* Keeps many solves resident on one GPU and admits new ones while others

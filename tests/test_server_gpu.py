@@ -1,6 +1,7 @@
 """The solve server (femx_server_*): requests admitted by device footprint
-while others run; mixed-precision VMS requests advanced together in a union
-of resident handles (continuous batching), the others each on its own
+while others run; mixed-precision VMS requests and mixed hexa8 SUPG requests
+of more than one element block advanced together in a union of resident
+handles of their kind (continuous batching), the others each on its own
 stream.
 
 Tolerance: bitwise.  A served request runs the arithmetic of its own
@@ -55,9 +56,10 @@ def _alone(femx, req):
 
 def test_results_alone_equal_in_the_mix(gpu):
     probe_v = _vms_req(6, 1, 37)
-    probe_s = _supg_req(5, 2, 23)
+    probe_s = _supg_req(7, 2, 23)
     mix = [_vms_req(4, 3, 40), _supg_req(4, 4, 31), _vms_req(8, 5, 25), probe_v, _supg_req(6, 6, 19),
-           _vms_req(5, 7, 52), probe_s, _vms_req(3, 8, 60), _supg_req(3, 9, 44)]
+           _vms_req(5, 7, 52), probe_s, _vms_req(3, 8, 60), _supg_req(3, 9, 44), _supg_req(9, 10, 27),
+           _supg_req(5, 11, 16)]
     print("server mix of", len(mix), "requests", flush=True)
     srv = gpu.Server(budget_bytes=4 << 20, max_active=4, prep_threads=2, chunk_steps=7)
     ids = [_submit(srv, r) for r in mix]
@@ -76,7 +78,8 @@ def test_results_alone_equal_in_the_mix(gpu):
     assert peak <= 4
     results = {i: srv.result(i) for i in ids}
     srv.close()
-    # every request, VMS ones served through the union of resident handles
+    # every request, the VMS ones and the SUPG ones of more than one block
+    # served through the unions of resident handles
     for req, i in zip(mix, ids):
         assert stats[i]["steps_done"] == req[3]
         assert np.array_equal(results[i], _alone(gpu, req)), req[0]

@@ -91,3 +91,69 @@ def test_union_rejects_non_members(gpu):
     g64.step(1)
     with pytest.raises(gpu.FemxError, match="FEMX_MIXED"):
         u.add(g64, 2)
+
+
+def _supg_member(gpu, n, seed, re, exit_condition=1.0e5, ramp=False):
+    m = meshgen.hex_cube(n, 0.2, seed)
+    bc = meshgen.lid_cavity_bc(m)
+    if ramp:
+        # the lid speeds up linearly, 3 t, so max |u| passes 1 after t = 1/3
+        rows = bc.vel_rows.reshape(-1, 2, 4).copy()
+        rows[:, 0] = [0.0, 0.0, 0.0, 0.0]
+        rows[:, 1] = [300.0, 0.0, 0.0, 100.0]
+        bc.vel_rows = rows.reshape(-1, 4)
+    return m, bc, dict(rho=1.0, mu=1.0 / re, alpha=1.0, cI=4.0, dt=0.1 / n, exit_condition=exit_condition)
+
+
+def _supg_solver(gpu, case):
+    m, bc, p = case
+    dm = gpu.DeviceMesh(m)
+    g = gpu.SUPGSolver(dm, bc, precision=gpu.FEMX_MIXED, **p)
+    g.set_state(np.zeros((m.n_nodes, 3)))
+    return dm, g
+
+
+def test_supg_union_members_equal_their_own_runs(gpu):
+    """SUPG unions: hexa8 members of different sizes and Reynolds numbers
+    join at different times; every member's state, pressure, lambda and log
+    row equal, bit for bit, those of the handle stepped on its own.  One
+    member's exit condition fires inside the union: it stops at the same
+    step as on its own."""
+    cases = [_supg_member(gpu, 7, 1, 100.0), _supg_member(gpu, 8, 2, 300.0), _supg_member(gpu, 10, 3, 50.0),
+             _supg_member(gpu, 6, 4, 200.0, exit_condition=1.0, ramp=True), _supg_member(gpu, 9, 5, 400.0)]
+    steps = [9, 14, 5, 40, 11]
+    refs = []
+    for case, n in zip(cases, steps):
+        dm, g = _supg_solver(gpu, case)
+        done = g.step(n)
+        st = g.get_state()
+        refs.append((done, st, g.get_log()))
+        del g, dm
+    assert refs[3][0] < steps[3], "the low exit condition must fire inside the run"
+    u = gpu.SUPGUnion()
+    live, joined, rounds = {}, 0, 0
+    while joined < len(cases) or live:
+        if joined < len(cases):
+            i = joined
+            dm, g = _supg_solver(gpu, cases[i])
+            g.step_async(1)  # the first step on its own stream
+            u.add(g, steps[i] - 1)
+            live[i] = (dm, g)
+            joined += 1
+        print(f"supg round {rounds}: members {sorted(live)}", flush=True)
+        u.step(3)
+        rounds += 1
+        for i in list(live):
+            dm, g = live[i]
+            if u.left(g) == 0:
+                u.remove(g)
+                done = g.finish()
+                st = g.get_state()
+                ref_done, ref, ref_log = refs[i]
+                assert done == ref_done, (i, done, ref_done)
+                for key in ("sol", "pres", "lam", "regime"):
+                    np.testing.assert_array_equal(st[key], ref[key], err_msg=f"member {i} {key}")
+                assert st["time"] == ref["time"]
+                np.testing.assert_array_equal(g.get_log(), ref_log)
+                del live[i]
+    assert rounds > 3

@@ -412,6 +412,88 @@ def workload_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def workload_union(args):
+    """Unions of resident solves: 10 and 40 cavities of the serving size mix
+    (n log-uniform in [8, 64], seeded), VMS tet4 and SUPG hexa8 in mixed
+    precision, stepped together by femx_{vms,supg}_union_step against the
+    same handles each stepped on its own stream."""
+    rng = np.random.default_rng(args.seed + 6)
+    L = femx.load()
+    rows = []
+    for kind in ("vms", "supg"):
+        cache = {}
+        for count in (10, 40):
+            ns = np.rint(np.exp(rng.uniform(np.log(8), np.log(64), count))).astype(int)
+            hs, E, skipped = [], 0, []
+            for n in ns:
+                n = int(n)
+                if n not in cache:
+                    if kind == "vms":
+                        m = meshgen.tet_cube(n, 0.15, n)
+                        cache[n] = (m, meshgen.lid_cavity_bc(m), 0.1 * meshgen.min_tet_h(m))
+                    else:
+                        m = meshgen.hex_cube(n, 0.2, n)
+                        cache[n] = (m, meshgen.lid_cavity_bc(m), 0.1 / n)
+                m, bc, dt = cache[n]
+                dm = femx.DeviceMesh(m)
+                if kind == "vms":
+                    g = femx.VMSSolver(dm, bc, 1.0, 1.0 / 200, 1.0, 4.0, 2.0, dt, precision=femx.FEMX_MIXED)
+                    g.set_state(np.zeros((m.n_nodes, 4)))
+                    g.step(1)
+                else:
+                    g = femx.SUPGSolver(dm, bc, 1.0, 1.0 / 200, 1.0, 4.0, dt, precision=femx.FEMX_MIXED)
+                    g.set_state(np.zeros((m.n_nodes, 3)))
+                    g.step(1)
+                    if not L.femx_supg_union_accepts(g._h):
+                        skipped.append(n)
+                        continue
+                hs.append((dm, g))
+                E += m.n_elems
+            u = femx.VMSUnion() if kind == "vms" else femx.SUPGUnion()
+            for dm, g in hs:
+                u.add(g, 1 << 30)
+            k = args.steps
+            u.step(max(k // 10, 2))
+            u.sync()
+            t0 = time.perf_counter()
+            u.step(k)
+            u.sync()
+            t1 = time.perf_counter()
+            for dm, g in hs:
+                u.remove(g)
+            if kind == "supg":
+                for dm, g in hs:
+                    g.finish()
+            # the same members, each on its own stream
+            ks = max(k // 4, 2)
+            for rep_ in range(2):
+                ta = time.perf_counter()
+                for dm, g in hs:
+                    if kind == "vms":
+                        g.step(ks if rep_ else 2)
+                    else:
+                        g.step_async(ks if rep_ else 2)
+                for dm, g in hs:
+                    g.sync() if kind == "vms" else g.finish()
+                tb = time.perf_counter()
+            row = {"kind": kind, "members": len(hs), "elements": int(E), "skipped_sizes": skipped,
+                   "union_ms_per_step": round((t1 - t0) / k * 1e3, 4),
+                   "union_element_steps_per_s": round(E * k / (t1 - t0), 1),
+                   "own_streams_element_steps_per_s": round(E * ks / (tb - ta), 1)}
+            print(json.dumps(row), file=sys.stderr, flush=True)
+            rows.append(row)
+            del u, hs
+    top = rows[1]
+    out = {"metric": "element-steps/s of a union of resident solves", "value": top["union_element_steps_per_s"],
+           "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": max(args.steps // 10, 2),
+           "ms_per_step": top["union_ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32-element/f64-state", "data": "synthetic (seeded jittered cavities)",
+           "config": {"workload": "unions of 10 and 40 resident lid cavities, n log-uniform [8,64]: VMS tet4 "
+                                  "and SUPG hexa8 (value: 40 VMS members)"},
+           "unions": rows}
+    print(json.dumps(out), flush=True)
+
+
 def workload_c2(args):
     """SUPG hexa8 on a jittered 128^3 cube (2,097,152 hexa, 2,146,689 nodes),
     Re 400 lid cavity, AB2 + B-hat + K1..K4, mixed and FP64."""
@@ -752,7 +834,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (profiler counter calibration)")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "accuracy", "ins", "hbm", "dropin"],
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "serve", "union", "accuracy", "ins", "hbm", "dropin"],
                     default="c3",
                     help="c3: one 5.31M-tet VMS mesh (default); c4: batch of 256 cavities; serve: request "
                          "stream; hbm: streaming-bandwidth probe")
@@ -777,7 +859,7 @@ def main():
     global SETTLE_MAX_S
     SETTLE_MAX_S = args.settle_max
     if args.steps is None:
-        args.steps = {"c3": 500, "c2": 300, "c4": 200, "accuracy": 1000}.get(args.workload, 50)
+        args.steps = {"c3": 500, "c2": 300, "c4": 200, "union": 200, "accuracy": 1000}.get(args.workload, 50)
     if args.workload == "serve" and args.hw_queues:
         # before the first HIP call: resident requests each run on their own stream
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
@@ -796,6 +878,9 @@ def main():
         return
     if args.workload == "serve":
         workload_serve(args)
+        return
+    if args.workload == "union":
+        workload_union(args)
         return
     if args.workload == "ins":
         workload_ins(args)

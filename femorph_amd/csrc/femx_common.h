@@ -305,6 +305,22 @@ __device__ __forceinline__ double uni(double v) {
 }
 __device__ __forceinline__ double4 uni(double4 v) { return make_double4(uni(v.x), uni(v.y), uni(v.z), uni(v.w)); }
 
+// A pointer a kernel loads from memory (a table of several meshes' arguments)
+// is generic to the compiler, unlike a pointer kernel argument: its loads and
+// stores become flat instructions, which also count on the LDS wait counter,
+// so every LDS wait then waits for the HBM loads in flight as well.  A
+// pointer cast to the global address space and back is generic again once
+// the compiler folds the two casts; the empty asm statement (no instruction)
+// keeps the global one, so the address-space inference sees where it comes
+// from and emits global loads and stores.  For a workgroup-uniform pointer
+// (it lives in SGPRs) to global memory.
+template <typename T>
+__device__ __forceinline__ void to_global(T *&p) {
+  auto g = (__attribute__((address_space(1))) T *)p;
+  __asm__("" : "+s"(g));
+  p = (T *)g;
+}
+
 // gfx950 deals workgroups round-robin over its 8 XCDs (each with its own L2);
 // give every XCD a contiguous range of the Morton-ordered element blocks so
 // neighbouring blocks share node data in one L2.  Bijective on [0, nblk).

@@ -1405,6 +1405,23 @@ struct SupgMember {
   int nwg;  // fix-up workgroups
 };
 
+// every pointer of a member's arguments as a global one (to_global)
+__device__ __forceinline__ void args_to_global(SupgArgs<float> &a) {
+  to_global(a.elem_off), to_global(a.node_off), to_global(a.loc_node), to_global(a.loc_pslot);
+  to_global(a.lcsr_off), to_global(a.lpos), to_global(a.lconn), to_global(a.xyzm), to_global(a.U);
+  to_global(a.Upress), to_global(a.Un), to_global(a.Unp), to_global(a.Fprev), to_global(a.partials);
+  to_global(a.lam), to_global(a.regime), to_global(a.pres), to_global(a.node_rec), to_global(a.rec);
+  to_global(a.row_off), to_global(a.rows), to_global(a.err), to_global(a.eblk), to_global(a.halt);
+  to_global(a.prev_row);
+}
+__device__ __forceinline__ void args_to_global(SupgFix &a) {
+  to_global(a.sh_node), to_global(a.sh_off), to_global(a.sh_slot), to_global(a.sh_first4);
+  to_global(a.partials), to_global(a.xyzm), to_global(a.U), to_global(a.Un), to_global(a.Unp);
+  to_global(a.Fprev), to_global(a.node_rec), to_global(a.rec), to_global(a.row_off), to_global(a.rows);
+  to_global(a.err), to_global(a.halt), to_global(a.nblk), to_global(a.fold_row), to_global(a.fold_prev);
+  to_global(a.eblk), to_global(a.fold_count);
+}
+
 __device__ __forceinline__ int supg_member_of(const int *off, int nm, int g) {
   int lo = 0, hi = nm;
   while (hi - lo > 1) {
@@ -1420,15 +1437,18 @@ __global__ __launch_bounds__(threads_for<8>(), 2) void supg_block_union(const Su
   const int g = xcd_block(blockIdx.x, ntot);
   const int h = supg_member_of(blk_off, nm, g);
   const SupgMember &me = mem[h];
-  if (uni(me.dyn->left) <= 0) return;
-  const int par = uni(me.dyn->par), lt = uni(me.dyn->lt) + 1;
+  SupgDyn *dyn = me.dyn;
+  to_global(dyn);
+  if (uni(dyn->left) <= 0) return;
+  const int par = uni(dyn->par), lt = uni(dyn->lt) + 1;
   SupgArgs<float> a = me.a;
   a.U = me.Ub[par];
   a.Upress = a.U;
   a.Un = me.Ub[par ^ 1];
-  a.k.t = uni(me.dyn->t) + me.dt;  // currentTime += timeStep (:1041)
+  a.k.t = uni(dyn->t) + me.dt;  // currentTime += timeStep (:1041)
   a.step_id = lt;
   a.prev_row = lt > 1 ? me.ring + (long)((lt - 1) % RING) * D_COLS : nullptr;
+  args_to_global(a);
   const int blk = g - uni(blk_off[h]);
   supg_block_run<float, 8, true, true, true, false, false>(a, blk, blk);
 }
@@ -1437,13 +1457,15 @@ __global__ __launch_bounds__(256) void supg_fixup_union(const SupgMember *mem, c
   const int g = blockIdx.x;
   const int h = supg_member_of(fx_off, nm, g);
   const SupgMember &me = mem[h];
-  if (uni(me.dyn->left) <= 0) return;
-  const int par = uni(me.dyn->par), lt = uni(me.dyn->lt) + 1;
+  SupgDyn *dyn = me.dyn;
+  to_global(dyn);
+  if (uni(dyn->left) <= 0) return;
+  const int par = uni(dyn->par);
   SupgFix f = me.f;
   f.U = me.Ub[par];
   f.Un = me.Ub[par ^ 1];
-  f.k.t = uni(me.dyn->t) + me.dt;
-  (void)lt;
+  f.k.t = uni(dyn->t) + me.dt;
+  args_to_global(f);
   supg_fixup_run<double>(f, g - uni(fx_off[h]), me.nwg);
 }
 
@@ -1453,11 +1475,16 @@ __global__ __launch_bounds__(256) void supg_fixup_union(const SupgMember *mem, c
 __global__ __launch_bounds__(256) void supg_union_tick(const SupgMember *mem) {
   const SupgMember &me = mem[blockIdx.x];
   SupgDyn *d = me.dyn;
+  to_global(d);
   if (uni(d->left) <= 0) return;
   const int lt = uni(d->lt) + 1;
-  if (!halted(me.a.halt))
-    diag_fold<false>(me.f.eblk, me.f.ne_blk, me.f.nblk, me.nwg, 0, me.ring + (long)(lt % RING) * D_COLS,
-                     lt > 1 ? me.ring + (long)((lt - 1) % RING) * D_COLS : nullptr, 0, 1);
+  const int *halt = me.a.halt;
+  const double *eblk = me.f.eblk, *nblk = me.f.nblk;
+  double *ring = me.ring;
+  to_global(halt), to_global(eblk), to_global(nblk), to_global(ring);
+  if (!halted(halt))
+    diag_fold<false>(eblk, me.f.ne_blk, nblk, me.nwg, 0, ring + (long)(lt % RING) * D_COLS,
+                     lt > 1 ? ring + (long)((lt - 1) % RING) * D_COLS : nullptr, 0, 1);
   __syncthreads();
   if (threadIdx.x == 0) {
     d->t = d->t + me.dt;

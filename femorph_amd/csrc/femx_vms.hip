@@ -2159,6 +2159,21 @@ struct VmsMember {
   double dt;
 };
 
+// every pointer of a member's arguments as a global one (as_global)
+__device__ __forceinline__ void args_to_global(BlockArgs<float> &a) {
+  to_global(a.elem_off), to_global(a.node_off), to_global(a.slot0), to_global(a.lnode), to_global(a.ecorn);
+  to_global(a.node_rec), to_global(a.xyzm), to_global(a.Q), to_global(a.Pr), to_global(a.Qn), to_global(a.Prn);
+  to_global(a.partials), to_global(a.sdus), to_global(a.rec), to_global(a.row_off), to_global(a.rows);
+  to_global(a.err), to_global(a.bmax), to_global(a.perm), to_global(a.diag_img), to_global(a.diag_sdu);
+  to_global(a.blk_mesh), to_global(a.kt), to_global(a.mesh_node0);
+}
+__device__ __forceinline__ void args_to_global(FixArgs &a) {
+  to_global(a.sh_node), to_global(a.sh_off), to_global(a.sh_slot), to_global(a.sh_first4), to_global(a.sh_im);
+  to_global(a.sh_rec), to_global(a.partials), to_global(a.Q), to_global(a.Pr), to_global(a.Qn), to_global(a.Prn);
+  to_global(a.rec), to_global(a.row_off), to_global(a.rows), to_global(a.err), to_global(a.bmax);
+  to_global(a.node_mesh), to_global(a.kt);
+}
+
 // member h with off[h] <= g < off[h + 1] (off ascending, nm + 1 entries)
 __device__ __forceinline__ int member_of(const int *off, int nm, int g) {
   int lo = 0, hi = nm;
@@ -2176,15 +2191,18 @@ __global__ __launch_bounds__(VB, VMS_BLOCK_WAVES(float, false)) void vms_block_u
   const int g = xcd_block(blockIdx.x, ntot);
   const int h = member_of(blk_off, nm, g);
   const VmsMember &me = mem[h];
-  const int left = uni(me.dyn->left);
+  VmsDyn *dyn = me.dyn;
+  to_global(dyn);
+  const int left = uni(dyn->left);
   if (left <= 0) return;
-  const int par = uni(me.dyn->par);
+  const int par = uni(dyn->par);
   BlockArgs<float> a = me.b;
   a.Q = me.Qb[par];
   a.Qn = me.Qb[par ^ 1];
   a.Pr = me.Prb[par];
   a.Prn = me.Prb[par ^ 1];
-  a.k.t = uni(me.dyn->t) + me.dt;  // currentTime += timeStep (:1028)
+  a.k.t = uni(dyn->t) + me.dt;  // currentTime += timeStep (:1028)
+  args_to_global(a);
   const int blk = g - uni(blk_off[h]);
   vms_block_run<float, double, true, true, false>(a, blk, blk);
 }
@@ -2193,15 +2211,18 @@ __global__ __launch_bounds__(256) void vms_fixup_union(const VmsMember *mem, con
   const int g = blockIdx.x;
   const int h = member_of(fx_off, nm, g);
   const VmsMember &me = mem[h];
-  const int left = uni(me.dyn->left);
+  VmsDyn *dyn = me.dyn;
+  to_global(dyn);
+  const int left = uni(dyn->left);
   if (left <= 0) return;
-  const int par = uni(me.dyn->par);
+  const int par = uni(dyn->par);
   FixArgs a = me.f;
   a.Q = me.Qb[par];
   a.Qn = me.Qb[par ^ 1];
   a.Pr = me.Prb[par];
   a.Prn = me.Prb[par ^ 1];
-  a.k.t = uni(me.dyn->t) + me.dt;
+  a.k.t = uni(dyn->t) + me.dt;
+  args_to_global(a);
   const int wg = g - uni(fx_off[h]);
   vms_fixup_run<double>(a, wg * 256 + (int)threadIdx.x, wg);
 }

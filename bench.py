@@ -34,8 +34,8 @@ F32SUM_NOTE = ("opt-in FEMX_MIXED_F32SUM: FP32 element math with FP32 in-block n
                "partials (sum over blocks, state and update FP64); not the headline, which keeps the residual "
                "accumulation in FP64")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vms_block_c3_traffic.json")
-PROFILED_KERNEL = {"mixed": "vms_block<float, double, true, true, false>",
-                   "fp64": "vms_block<double, double, true, true, false>"}
+PROFILED_KERNEL = {"mixed": "vms_block<float, double, true, true, false, false>",
+                   "fp64": "vms_block<double, double, true, true, false, false>"}
 
 
 def profiled_traffic(precision: str, n: int):

@@ -485,8 +485,9 @@ __device__ __forceinline__ double node_update(const double4 base, double im, con
 
 
 // One element block `blk` of one mesh; its per-block maximum goes to
-// a.bmax[row].
-template <typename T, typename S, bool PROJ, bool RES, bool ACC>
+// a.bmax[row].  DG: the element diagnostics' outputs (a.diag_img,
+// a.diag_sdu) may be set; without it their code is compiled out.
+template <typename T, typename S, bool PROJ, bool RES, bool ACC, bool DG>
 __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int blk, const int row) {
   using T4 = typename Vec4<T>::type;
   // FP64: staged node records as 16-byte pairs {x,y} {z,pu} {pv,pw} {qu,qv} {qw,qp}.
@@ -726,7 +727,7 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
     }
     if (RES) {
       vms_residual<T, ACC>(DN, V, ev, p, dd, sd, res, k);
-      if (a.diag_sdu) {
+      if (DG && a.diag_sdu) {
         const size_t eo = (size_t)a.perm[e];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -778,7 +779,7 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
   __syncthreads();
   // 3. per local node: sum its contiguous contributions in a fixed order
   // (with diagnostics, each contribution is replaced by the running sum)
-  const bool dg = a.diag_img != nullptr;
+  const bool dg = DG && a.diag_img != nullptr;
   double m2 = 0.0;
   if (has_n) {
     const int g = g_own;
@@ -844,10 +845,10 @@ __device__ __forceinline__ void vms_block_run(const BlockArgs<T> &a, const int b
 
 // occupancy (waves per SIMD): FP32 element math 5 (4 with dual iterations), FP64 2
 #define VMS_BLOCK_WAVES(T, ACC) (sizeof(T) == 4 && !(ACC) ? 5 : (sizeof(T) == 4 ? 4 : 2))
-template <typename T, typename S, bool PROJ, bool RES, bool ACC>
+template <typename T, typename S, bool PROJ, bool RES, bool ACC, bool DG>
 __global__ __launch_bounds__(VB, VMS_BLOCK_WAVES(T, ACC)) void vms_block(BlockArgs<T> a) {
   const int blk = a.xcd ? xcd_block(blockIdx.x, a.nblk) : (int)blockIdx.x;
-  vms_block_run<T, S, PROJ, RES, ACC>(a, blk, blockIdx.x);
+  vms_block_run<T, S, PROJ, RES, ACC, DG>(a, blk, blockIdx.x);
 }
 
 __device__ __forceinline__ VmsConst ld_const(const VmsConst *p) {
@@ -930,7 +931,12 @@ __device__ __forceinline__ void vms_fixup_run(const FixArgs &a, const int s, con
       R[2] += (double)q.z;
       R[3] += (double)q.w;
     }
-    VmsConst k = a.k;
+    // the single mesh's constants as values (not loads), so the compiler does
+    // not turn the two arms into one load through a pointer that may point to
+    // a private copy (a scratch store and a flat load per thread)
+    VmsConst k;
+    k.nu = uni(a.k.nu), k.inv_eps = uni(a.k.inv_eps), k.c1 = uni(a.k.c1), k.c2 = uni(a.k.c2);
+    k.inv_dt = uni(a.k.inv_dt), k.dual_step = uni(a.k.dual_step), k.t = uni(a.k.t);
     if (a.kt) {
       const VmsConst *q = a.kt + a.node_mesh[g];
       k.nu = q->nu;
@@ -1315,15 +1321,21 @@ BlockArgs<T> block_args(femx_vms *s, bool first_dual, double t, T *sdus) {
   return a;
 }
 
+template <typename T, typename S, bool DG>
+void launch_block_tsd(const BlockArgs<T> &a, int grd, hipStream_t st, bool proj, bool res, bool acc) {
+  if (proj && res && acc) vms_block<T, S, true, true, true, DG><<<grd, VB, 0, st>>>(a);
+  else if (proj && res) vms_block<T, S, true, true, false, DG><<<grd, VB, 0, st>>>(a);
+  else if (res && acc) vms_block<T, S, false, true, true, DG><<<grd, VB, 0, st>>>(a);
+  else if (res) vms_block<T, S, false, true, false, DG><<<grd, VB, 0, st>>>(a);
+  else vms_block<T, S, true, false, false, false><<<grd, VB, 0, st>>>(a);
+}
+
 template <typename T, typename S>
 void launch_block_ts(femx_vms *s, bool proj, bool res, bool acc, bool first_dual, double t, T *sdus) {
   BlockArgs<T> a = block_args<T>(s, first_dual, t, sdus);
   const int grd = s->mesh->blk.nb;
-  if (proj && res && acc) vms_block<T, S, true, true, true><<<grd, VB, 0, s->st>>>(a);
-  else if (proj && res) vms_block<T, S, true, true, false><<<grd, VB, 0, s->st>>>(a);
-  else if (res && acc) vms_block<T, S, false, true, true><<<grd, VB, 0, s->st>>>(a);
-  else if (res) vms_block<T, S, false, true, false><<<grd, VB, 0, s->st>>>(a);
-  else vms_block<T, S, true, false, false><<<grd, VB, 0, s->st>>>(a);
+  if (res && (a.diag_img || a.diag_sdu)) launch_block_tsd<T, S, true>(a, grd, s->st, proj, res, acc);
+  else launch_block_tsd<T, S, false>(a, grd, s->st, proj, res, acc);
 }
 
 femx_status launch_block(femx_vms *s, bool proj, bool res, bool acc, bool first_dual, double t) {
@@ -2204,7 +2216,7 @@ __global__ __launch_bounds__(VB, VMS_BLOCK_WAVES(float, false)) void vms_block_u
   a.k.t = uni(dyn->t) + me.dt;  // currentTime += timeStep (:1028)
   args_to_global(a);
   const int blk = g - uni(blk_off[h]);
-  vms_block_run<float, double, true, true, false>(a, blk, blk);
+  vms_block_run<float, double, true, true, false, false>(a, blk, blk);
 }
 
 __global__ __launch_bounds__(256) void vms_fixup_union(const VmsMember *mem, const int *fx_off, int nm) {

@@ -1377,11 +1377,12 @@ extern "C" int femx_supg_kernel_times(femx_supg *s, femx_kernel_time *out, int m
 
 // ------------------------------------------------------- unions of handles
 // Continuous batching of resident SUPG solves: FEMX_MIXED hexa8 handles with
-// B-hat, no fixed DOFs and no element diagnostics, whose log rows the
-// fix-up's last workgroup folds (at most DIAG_ONE_PASS rows), are advanced
-// together, one launch of the element kernel, one of the fix-up and one of
-// the per-member fold of the log rows and tick, per step for all members,
-// each by its own number of steps.  A member's
+// B-hat, no fixed DOFs and no element diagnostics are advanced together, one
+// launch of the element kernel, one of the fix-up and one of the per-member
+// fold of the log rows and tick, per step for all members, each by its own
+// number of steps.  Every log column is a minimum or a maximum, so one
+// workgroup's fold of a member's rows gives its own launches' value at any
+// row count.  A member's
 // step is its handle's own (the same blocks, sums and order): its time,
 // state parity and step number come from a device record the tick advances
 // with the same arithmetic as the host's loop, and its exit condition is
@@ -1505,7 +1506,7 @@ bool union_member_ok(const femx_supg *s) {
   const Blocks &b = s->mesh->blk;
   const int nnb = b.n_sh ? grid_for(b.n_sh, 256) : 0;
   return s->p.precision == FEMX_MIXED && s->mesh->npe == 8 && s->p.use_bhat && s->p.fixed_dofs == 0 &&
-         !s->diag_enabled && nnb > 0 && nnb + b.nb <= DIAG_ONE_PASS;
+         !s->diag_enabled && nnb > 0;
 }
 
 }  // namespace

@@ -309,8 +309,8 @@ femx_status femx_supg_finish(femx_supg *s, int *steps_done);
  * launch of the element kernel, one of the fix-up and one tick per step for
  * all members, each by its own number of steps; bit for bit each handle
  * stepped on its own.  Members: FEMX_MIXED hexa8 with B-hat, no fixed DOFs,
- * no diagnostics, at most 2048 element blocks and fix-up workgroups
- * together, and their first step run on their own.  A member's union steps
+ * no diagnostics, more than one element block, and their first step run on
+ * their own.  A member's union steps
  * count as enqueued steps of its own: after femx_supg_union_remove,
  * femx_supg_finish checks them for the exit condition (a step past one
  * whose max |u| exceeded EXITCOND does not run) and femx_supg_get_state

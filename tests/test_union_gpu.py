@@ -119,9 +119,13 @@ def test_supg_union_members_equal_their_own_runs(gpu):
     row equal, bit for bit, those of the handle stepped on its own.  One
     member's exit condition fires inside the union: it stops at the same
     step as on its own."""
+    # the 60^3 member has more log rows (element blocks and fix-up
+    # workgroups) than one workgroup reduces on its own stream, where a
+    # two-stage reduction makes its log row
     cases = [_supg_member(gpu, 7, 1, 100.0), _supg_member(gpu, 8, 2, 300.0), _supg_member(gpu, 10, 3, 50.0),
-             _supg_member(gpu, 6, 4, 200.0, exit_condition=1.0, ramp=True), _supg_member(gpu, 9, 5, 400.0)]
-    steps = [9, 14, 5, 40, 11]
+             _supg_member(gpu, 6, 4, 200.0, exit_condition=1.0, ramp=True), _supg_member(gpu, 9, 5, 400.0),
+             _supg_member(gpu, 60, 6, 150.0)]
+    steps = [9, 14, 5, 40, 11, 7]
     refs = []
     for case, n in zip(cases, steps):
         dm, g = _supg_solver(gpu, case)

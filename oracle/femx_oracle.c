@@ -41,6 +41,7 @@
 #include "femx_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -643,6 +644,139 @@ int ox_vms_step(oxVMS *s) {
   memcpy(s->sol_prev, s->sol_n, sizeof(double) * 4 * (size_t)N);
   memcpy(s->sol_n, s->sol_tau, sizeof(double) * 4 * (size_t)N);
   s->steps++;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* All-cores timing variant of ox_vms_step (the CPU baseline's second figure).
+ * The per-element work of each sweep (residual, projection, subscale update)
+ * and the node-wise dual update run on nthreads threads over element (node)
+ * ranges; the element-order scatter into the global rhs and the centroid
+ * diagnostics that read the partially assembled rhs (:394-398, :699-724)
+ * stay on the calling thread in ascending element order, from the buffered
+ * element results.  So the results are bitwise those of ox_vms_step. */
+typedef struct {
+  oxVMS *s;
+  int phase;       /* 0 residual into buf, 1 projection, 2 subscales, 3 dual update */
+  long base;       /* first element of the buffered chunk */
+  double *buf;     /* per element: l[4][4], sdu[4][3], sgp[4] */
+  long b, e;
+} oxVmsJob;
+
+#define OX_VMS_BUF 32
+#define OX_VMS_CHUNK (1L << 17)
+
+static void *ox_vms_job(void *p) {
+  oxVmsJob *j = (oxVmsJob *)p;
+  oxVMS *s = j->s;
+  for (long e = j->b; e < j->e; e++) {
+    if (j->phase == 0) {
+      double *o = j->buf + (e - j->base) * OX_VMS_BUF;
+      vms_residual(s, (int)e, s->sol_tau, s->sol_n, (double(*)[4])o, (double(*)[3])(o + 16), o + 28);
+    } else if (j->phase == 1) {
+      vms_projection(s, (int)e, s->sol_tau);
+    } else if (j->phase == 2) {
+      vms_update_subscales(s, (int)e);
+    } else {
+      for (int k = 0; k < 4; k++) {
+        s->sol_tau[e * 4 + k] = s->sol_n[e * 4 + k] - s->p.dual_step * s->rhs[e * 4 + k] / s->nd_mass[e];
+        s->sol_n[e * 4 + k] = s->sol_tau[e * 4 + k];
+      }
+    }
+  }
+  return NULL;
+}
+
+static int ox_vms_parallel(oxVMS *s, int nt, int phase, long b, long e, double *buf) {
+  pthread_t th[64];
+  oxVmsJob job[64];
+  if (nt > 64) nt = 64;
+  if (nt < 1) nt = 1;
+  const long n = e - b;
+  int started = 0;
+  for (int t = 0; t < nt; t++) {
+    job[t].s = s;
+    job[t].phase = phase;
+    job[t].base = b;
+    job[t].buf = buf;
+    job[t].b = b + n * t / nt;
+    job[t].e = b + n * (t + 1) / nt;
+  }
+  for (int t = 1; t < nt; t++) {
+    if (pthread_create(&th[t], NULL, ox_vms_job, &job[t]) != 0) break;
+    started = t;
+  }
+  /* a thread that could not start runs its range here */
+  for (int t = started + 1; t < nt; t++) ox_vms_job(&job[t]);
+  ox_vms_job(&job[0]);
+  for (int t = 1; t <= started; t++) pthread_join(th[t], NULL);
+  return 0;
+}
+
+int ox_vms_step_threads(oxVMS *s, int nthreads) {
+  const long N = s->N, E = s->E;
+  double *buf = (double *)malloc(sizeof(double) * OX_VMS_BUF * (size_t)(E < OX_VMS_CHUNK ? (E > 0 ? E : 1) : OX_VMS_CHUNK));
+  if (!buf) return -2;
+  s->time += s->p.dt;
+  int loopTau = 0, converged = 0;
+  while (!converged) {
+    loopTau++;
+    memset(s->rhs, 0, sizeof(double) * 4 * (size_t)N);
+    memset(s->projRHS, 0, sizeof(double) * 16 * (size_t)E);
+    memset(s->el_p_res, 0, sizeof(double) * (size_t)E);
+    memset(s->el_p_sdus, 0, sizeof(double) * (size_t)E);
+    memset(s->el_u_res, 0, sizeof(double) * 3 * (size_t)E);
+    memset(s->el_u_sdus, 0, sizeof(double) * 3 * (size_t)E);
+    for (long c0 = 0; c0 < E; c0 += OX_VMS_CHUNK) {
+      const long c1 = c0 + OX_VMS_CHUNK < E ? c0 + OX_VMS_CHUNK : E;
+      ox_vms_parallel(s, nthreads, 0, c0, c1, buf);
+      for (long e = c0; e < c1; e++) {
+        const double *o = buf + (e - c0) * OX_VMS_BUF;
+        for (int a = 0; a < 4; a++) {
+          long n = s->mesh.conn[e * 4 + a];
+          for (int i = 0; i < 4; i++) s->rhs[n * 4 + i] += o[a * 4 + i];
+        }
+        for (int a = 0; a < 4; a++) {
+          long n = s->mesh.conn[e * 4 + a];
+          for (int k = 0; k < 3; k++) s->el_u_res[e * 3 + k] += s->rhs[n * 4 + k] * 0.25;
+          s->el_p_res[e] += s->rhs[n * 4 + 3] * 0.25;
+        }
+        for (int a = 0; a < 4; a++) {
+          for (int k = 0; k < 3; k++) s->el_u_sdus[e * 3 + k] += o[16 + a * 3 + k] * 0.25;
+          s->el_p_sdus[e] += o[28 + a] * 0.25;
+        }
+      }
+    }
+    ox_vms_parallel(s, nthreads, 3, 0, N, NULL);
+    ox_vms_parallel(s, nthreads, 1, 0, E, NULL);
+    ox_vms_parallel(s, nthreads, 2, 0, E, NULL);
+    s->u_res_norm = 0.0;
+    s->p_res_norm = 0.0;
+    for (long e = 0; e < E; e++) {
+      double cu = sqrt(s->el_u_res[e * 3] * s->el_u_res[e * 3] + s->el_u_res[e * 3 + 1] * s->el_u_res[e * 3 + 1] +
+                       s->el_u_res[e * 3 + 2] * s->el_u_res[e * 3 + 2]);
+      if (cu > s->u_res_norm) s->u_res_norm = cu;
+      if (fabs(s->el_p_res[e]) > s->p_res_norm) s->p_res_norm = fabs(s->el_p_res[e]);
+    }
+    if (set_node_velocity(&s->bc, s->time, s->sol_tau, 4) != 0) {
+      free(buf);
+      return -1;
+    }
+    zero_traction(&s->bc, s->sol_tau, 4);
+    dirichlet(&s->bc, s->sol_tau, 4);
+    double mx = 0.0;
+    for (long i = 0; i < N; i++) {
+      double m = sqrt(s->sol_n[i * 4] * s->sol_n[i * 4] + s->sol_n[i * 4 + 1] * s->sol_n[i * 4 + 1] +
+                      s->sol_n[i * 4 + 2] * s->sol_n[i * 4 + 2]);
+      if (m > mx) mx = m;
+    }
+    s->cfl = mx * s->p.dt / s->min_el_h;
+    converged = (loopTau > s->p.tot_dual_steps);
+  }
+  memcpy(s->sol_prev, s->sol_n, sizeof(double) * 4 * (size_t)N);
+  memcpy(s->sol_n, s->sol_tau, sizeof(double) * 4 * (size_t)N);
+  s->steps++;
+  free(buf);
   return 0;
 }
 

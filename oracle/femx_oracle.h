@@ -83,6 +83,9 @@ void ox_vms_destroy(oxVMS *s);
 int ox_vms_set_ic(oxVMS *s, const double *sol);
 /* One time step (all dual iterations). Returns 0 / -1 (BC table error). */
 int ox_vms_step(oxVMS *s);
+/* ox_vms_step with the per-element sweeps on nthreads threads (bitwise the
+ * same results; the element-order scatter stays on one thread). */
+int ox_vms_step_threads(oxVMS *s, int nthreads);
 /* Copies out: which = 0 sol_n [N][4], 1 sol_tau, 2 sdus [12][E] (reference
  * layout (g*3+i)*E+e), 3 sgps [4][E], 4 nd_mass [N], 5 volumes [E],
  * 6 char_length [E], 7 DNs [12][E], 8 elmTaus [2][E], 9 el_u_res [E][3],

@@ -60,6 +60,7 @@ def lib():
         L.ox_vms_set_ic.argtypes = [C.c_void_p, _dp]
         L.ox_vms_set_sdus.argtypes = [C.c_void_p, _dp]
         L.ox_vms_step.argtypes = [C.c_void_p]
+        L.ox_vms_step_threads.argtypes = [C.c_void_p, C.c_int]
         L.ox_vms_get.argtypes = [C.c_void_p, C.c_int, _dp]
         L.ox_vms_element_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.ox_supg_create.restype = C.c_void_p
@@ -135,10 +136,13 @@ class VMSOracle:
         sdus = np.ascontiguousarray(sdus, np.float64).reshape(12 * self.E)
         lib().ox_vms_set_sdus(self._h, _p(sdus))
 
-    def step(self, n=1):
+    def step(self, n=1, threads=1):
+        """n steps; threads > 1 runs the element sweeps on that many threads
+        (ox_vms_step_threads: bitwise the same results)."""
         for _ in range(n):
-            if lib().ox_vms_step(self._h) != 0:
-                raise ValueError("BC table error")
+            r = lib().ox_vms_step(self._h) if threads <= 1 else lib().ox_vms_step_threads(self._h, int(threads))
+            if r != 0:
+                raise ValueError("BC table error" if r == -1 else "out of host memory")
 
     def get(self, name):
         which, shape = self.GET[name]

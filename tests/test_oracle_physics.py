@@ -160,3 +160,24 @@ def test_supg_pressure_sign_and_scale():
     o.step()  # no K terms: the state is unchanged, lambda from the Gauss-point velocity
     lam = o.get("el_lam")
     np.testing.assert_allclose(o.get("pres_n"), -3.0 * lam, rtol=1e-12)
+
+
+@pytest.mark.parametrize("n,dual", [(5, 2), (28, 0)])
+def test_vms_threaded_step_is_the_serial_step(n, dual):
+    """ox_vms_step_threads (the all-cores CPU baseline) runs the element
+    sweeps on several threads and the element-order scatter on one: every
+    output is bitwise that of ox_vms_step.  28^3 x 6 tets span two element
+    chunks of the threaded residual sweep."""
+    m = meshgen.tet_cube(n, 0.15, 3)
+    bc = meshgen.channel_bc(m, u_max=1.0)
+    dt = 0.1 * meshgen.min_tet_h(m)
+    ic = meshgen.random_velocity(m, 1e-2, 4)
+    outs = []
+    for threads in (1, 3):
+        o = _vms(m, bc, mu=1.0 / 200, dt=dt, tot_dual_steps=dual)
+        o.set_ic(ic)
+        o.step(2, threads=threads)
+        outs.append({k: o.get(k) for k in ("sol_n", "sol_tau", "sdus", "elmTaus", "el_u_res", "el_p_res",
+                                           "el_u_sdus", "scalars")})
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[1][k], outs[0][k], err_msg=k)

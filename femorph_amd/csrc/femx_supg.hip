@@ -229,6 +229,8 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
   const int e0 = a.elem_off[blk], ne = a.elem_off[blk + 1] - e0;
   const bool has_e = tid < ne;
   const int e = e0 + tid;
+  constexpr bool LOWREG = sizeof(T) == 8 && NPE == 8;
+  T out[NPE][3];
   int id[NPE], ps[NPE];
   double dg[11];
 #pragma unroll
@@ -262,16 +264,34 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
     // from DN (velocity gradient, divergence, B-hat strain, the K2/K4 node
     // forces, u.grad N) is the same arithmetic regrouped through the local
     // derivatives of the table and 3x3 products per Gauss point.
-    T U0[3], dU[NPE][3];  // node velocities relative to node 0 (dU[0] = 0)
+    // FP64 hexa8 (LOWREG): the node velocities relative to node 0 are read
+    // from LDS where they are used, and each thread keeps its B-hat averages
+    // in its own slots of the contribution buffer sres, unused until after
+    // the Gauss-point loop, instead of in registers through it: with both
+    // live the kernel spilled 236 B per lane at two waves per SIMD.
+    T U0[3], dU[LOWREG ? 1 : NPE][3];  // node velocities relative to node 0 (dU[0] = 0)
     U0[0] = (T)su[id[0]];
     U0[1] = (T)sv[id[0]];
     U0[2] = (T)sw[id[0]];
+    if constexpr (!LOWREG) {
 #pragma unroll
-    for (int n = 0; n < NPE; n++) {
-      dU[n][0] = (T)(su[id[n]] - su[id[0]]);
-      dU[n][1] = (T)(sv[id[n]] - sv[id[0]]);
-      dU[n][2] = (T)(sw[id[n]] - sw[id[0]]);
+      for (int n = 0; n < NPE; n++) {
+        dU[n][0] = (T)(su[id[n]] - su[id[0]]);
+        dU[n][1] = (T)(sv[id[n]] - sv[id[0]]);
+        dU[n][2] = (T)(sw[id[n]] - sw[id[0]]);
+      }
     }
+    auto du = [&](int n, T d[3]) {
+      if constexpr (LOWREG) {
+        d[0] = (T)(su[id[n]] - su[id[0]]);
+        d[1] = (T)(sv[id[n]] - sv[id[0]]);
+        d[2] = (T)(sw[id[n]] - sw[id[0]]);
+      } else {
+        d[0] = dU[n][0];
+        d[1] = dU[n][1];
+        d[2] = dU[n][2];
+      }
+    };
     // hexa8: the trilinear map's Jacobian rows are J_a = E_a + (xi terms),
     // from 7 coefficient vectors (femHexa8 node signs, src/femHexa8.cpp:15-17);
     // tet4 (femTetra4, N3 = 1 - sum xi): constant, J_a = x_a - x_3.
@@ -332,9 +352,8 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
       }
     };
     // eval_avg_shapeDeriv (:660-707): volume average of DN (hexa only);
-    // with it the B-hat strain needs P_k = sum_n avg_nk u_nk per element
-    T avg[NPE][3], P[3] = {(T)0, (T)0, (T)0};
-    if (BHAT) {
+    // with it the B-hat strain needs P_k = sum_n avg_nk u_nk per element.
+    auto avg_pass = [&](T avg[NPE][3]) {
 #pragma unroll
       for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = (T)0;
       T den = (T)0;
@@ -361,12 +380,56 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
         avg[n][0] *= iden;
         avg[n][1] *= iden;
         avg[n][2] *= iden;
-        P[0] += avg[n][0] * dU[n][0];  // sum_n avg_n = 0: relative velocities suffice
+      }
+    };
+    T avg[LOWREG ? 1 : NPE][3], P[3] = {(T)0, (T)0, (T)0};
+    if (BHAT && LOWREG) {
+      T av[NPE][3];
+      avg_pass(av);
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        T d[3];
+        du(n, d);
+        P[0] += av[n][0] * d[0];  // sum_n avg_n = 0: relative velocities suffice
+        P[1] += av[n][1] * d[1];
+        P[2] += av[n][2] * d[2];
+        if constexpr (DYN)
+#pragma unroll
+          for (int b = 0; b < 3; b++) sres[(n * 3 + b) * SB + tid] = av[n][b];
+      }
+    } else if (BHAT) {
+      // the same as avg_pass, written out (FP32 keeps its unrolled form)
+#pragma unroll
+      for (int n = 0; n < NPE; n++) avg[n][0] = avg[n][1] = avg[n][2] = (T)0;
+      T den = (T)0;
+      for (int g = 0; g < NGP; g++) {
+        T J[3][3], iJ[3][3], detJ;
+        jacobian(g, J);
+        inv3(J, iJ, detJ);
+        const T dv = detJ * S::W();
+        T Q[3][3];
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) Q[b][c] = iJ[b][c] * dv;
+#pragma unroll
+        for (int n = 0; n < NPE; n++)
+#pragma unroll
+          for (int b = 0; b < 3; b++)
+            avg[n][b] += Q[b][0] * S::dN(g, n, 0) + Q[b][1] * S::dN(g, n, 1) + Q[b][2] * S::dN(g, n, 2);
+        den += dv;
+      }
+      const T iden = recip(den);
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        avg[n][0] *= iden;
+        avg[n][1] *= iden;
+        avg[n][2] *= iden;
+        P[0] += avg[n][0] * dU[n][0];
         P[1] += avg[n][1] * dU[n][1];
         P[2] += avg[n][2] * dU[n][2];
       }
     }
-    T out[NPE][3];
 #pragma unroll
     for (int n = 0; n < NPE; n++) out[n][0] = out[n][1] = out[n][2] = (T)0;
     // per-thread diagnostic accumulators, columns D_TAU .. D_NU+1
@@ -398,14 +461,16 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
 #pragma unroll
       for (int n = 1; n < NPE; n++) {
         const T Nn = S::N(g, n);
+        T dn[3];
+        du(n, dn);
 #pragma unroll
-        for (int i = 0; i < 3; i++) ug[i] += Nn * dU[n][i];
+        for (int i = 0; i < 3; i++) ug[i] += Nn * dn[i];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const T d = S::dN(g, n, c);
-          L[c][0] += d * dU[n][0];
-          L[c][1] += d * dU[n][1];
-          L[c][2] += d * dU[n][2];
+          L[c][0] += d * dn[0];
+          L[c][1] += d * dn[1];
+          L[c][2] += d * dn[2];
         }
       }
       T gU[3][3];
@@ -542,7 +607,14 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
         out[n][2] += cv * c[2] + (Mg[2][0] * d0 + Mg[2][1] * d1 + Mg[2][2] * d2);
       }
     }
-    if (BHAT && DYN) {
+    if (BHAT && DYN && LOWREG) {
+#pragma unroll
+      for (int n = 0; n < NPE; n++) {
+        out[n][0] += sres[(n * 3 + 0) * SB + tid] * Aacc[0];
+        out[n][1] += sres[(n * 3 + 1) * SB + tid] * Aacc[1];
+        out[n][2] += sres[(n * 3 + 2) * SB + tid] * Aacc[2];
+      }
+    } else if (BHAT && DYN) {
 #pragma unroll
       for (int n = 0; n < NPE; n++) {
         out[n][0] += avg[n][0] * Aacc[0];
@@ -562,14 +634,27 @@ __device__ __forceinline__ void supg_block_run(const SupgArgs<T> &a, const int b
         dg[D_NU - D_TAU] = fmin(dg[D_NU - D_TAU], cnu);
         dg[D_NU + 1 - D_TAU] = fmax(dg[D_NU + 1 - D_TAU], cnu);
       }
-      const T mdt = (T)(-k.dt);
+      if constexpr (!(BHAT && LOWREG)) {
+        const T mdt = (T)(-k.dt);
+#pragma unroll
+        for (int n = 0; n < NPE; n++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) sres[i * (NPE * SB) + ps[n]] = mdt * out[n][i];
+      }
+    }
+  }
+  if (!DYN) return;
+  if constexpr (BHAT && LOWREG) {
+    // every thread has read its averages back from sres
+    __syncthreads();
+    if (has_e) {
+      const T mdt = (T)(-a.k.dt);
 #pragma unroll
       for (int n = 0; n < NPE; n++)
 #pragma unroll
         for (int i = 0; i < 3; i++) sres[i * (NPE * SB) + ps[n]] = mdt * out[n][i];
     }
   }
-  if (!DYN) return;
   if (DIAG) block_diag_store<SB, 11>(dg, D_TAU, a.eblk + (long)row * D_COLS);
   __syncthreads();
   // velocity extrema and max |u|^2 of the new states this block completes

@@ -807,9 +807,12 @@ struct SupgFix {
   const int *sh_node, *sh_off, *sh_slot;
   const int4 *sh_first4;  // each shared node's first 4 slots
   const void *partials;  // T4 records of the block kernel
-  const double4 *xyzm, *U;
+  // per shared node, contiguous: its lumped mass and its BC record or -1
+  // (the fix-up then gathers no node-indexed metadata)
+  const double *sh_m;
+  const int *sh_rec;
+  const double4 *U;
   double4 *Un, *Unp, *Fprev;
-  const int *node_rec;
   const BCRecord *rec;
   const int *row_off;
   const double4 *rows;
@@ -874,9 +877,9 @@ __device__ __forceinline__ void fixup_node(const SupgFix &a, int s, double nv[6]
   const int4 f4 = a.sh_first4[s];
   const int fs[FIX_BATCH] = {f4.x, f4.y, f4.z, f4.w};
   const double4 fp = a.k.euler_step ? make_double4(0, 0, 0, 0) : a.Fprev[g];
-  const double im = a.xyzm[g].w;
+  const double im = a.sh_m[s];
   const double4 u = a.U[g];
-  const int rec = a.node_rec[g];
+  const int rec = a.sh_rec[s];
   T4 p[FIX_BATCH];
 #pragma unroll
   for (int j = 0; j < FIX_BATCH; j++) p[j] = partials[fs[j]];
@@ -940,6 +943,16 @@ __global__ __launch_bounds__(256) void supg_mass(int N, const int *g_off, const 
   xyzm[n] = make_double4(x.x, x.y, x.z, acc);
 }
 
+// Per shared node, its lumped mass and BC record in one contiguous array each.
+__global__ __launch_bounds__(256) void supg_shared_meta(int n_sh, const int *sh_node, const double4 *xyzm,
+                                                        const int *node_rec, double *sh_m, int *sh_rec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sh) return;
+  const int g = sh_node[s];
+  sh_m[s] = xyzm[g].w;
+  sh_rec[s] = node_rec[g];
+}
+
 __global__ void supg_bc_only(int nrec, const int *rec_node, const BCRecord *rec, const int *row_off, const double4 *rows,
                              double t, double4 *U, int *err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -969,6 +982,8 @@ struct femx_supg {
   BCDev bc;
   DevBuf<int> rec_node;
   DevBuf<double4> xyzm, U[2], Up[2], Fprev, partials;
+  DevBuf<double> sh_m;  // per shared node: lumped mass
+  DevBuf<int> sh_rec;   // per shared node: BC record or -1
   DevBuf<double> lam, regime, pres, scratch;
   DevBuf<double> ring, eblk, nblk;  // [RING][D_COLS] reduced rows; per-block rows
   DevBuf<double> dpart;             // [DIAG_PARTS][D_COLS] partial rows of the reduction
@@ -1125,12 +1140,12 @@ femx_status launch_fix(femx_supg *s, bool euler_step, double t, double *fold_row
   a.sh_slot = b.d_sh_slot.p;
   a.sh_first4 = b.d_sh_first4.p;
   a.partials = s->partials.p;
-  a.xyzm = s->xyzm.p;
+  a.sh_m = s->sh_m.p;
+  a.sh_rec = s->sh_rec.p;
   a.U = s->U[s->cur].p;
   a.Un = s->U[s->cur ^ 1].p;
   a.Unp = s->p.fixed_dofs ? s->Up[s->cur ^ 1].p : nullptr;
   a.Fprev = s->Fprev.p;
-  a.node_rec = s->bc.node_rec.p;
   a.rec = s->bc.rec.p;
   a.row_off = s->bc.row_off.p;
   a.rows = s->bc.rows.p;
@@ -1213,6 +1228,11 @@ extern "C" femx_status femx_supg_create(femx_mesh *mh, const femx_bc_desc *bc, c
     supg_mass<8><<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, m.conn.p, m.xyz.p, p->rho, s->xyzm.p);
   else
     supg_mass<4><<<grid_for(m.N, 256), 256, 0, s->st>>>(m.N, m.g_off.p, m.g_slot.p, m.conn.p, m.xyz.p, p->rho, s->xyzm.p);
+  if (m.blk.n_sh > 0) {
+    if ((r = s->sh_m.alloc(m.blk.n_sh)) != FEMX_OK || (r = s->sh_rec.alloc(m.blk.n_sh)) != FEMX_OK) return fail(r);
+    supg_shared_meta<<<grid_for(m.blk.n_sh, 256), 256, 0, s->st>>>(m.blk.n_sh, m.blk.d_sh_node.p, s->xyzm.p,
+                                                                     s->bc.node_rec.p, s->sh_m.p, s->sh_rec.p);
+  }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return fail(FEMX_ERR_HIP);
   *out = s;
   return FEMX_OK;
@@ -1502,8 +1522,8 @@ __device__ __forceinline__ void args_to_global(SupgArgs<float> &a) {
 }
 __device__ __forceinline__ void args_to_global(SupgFix &a) {
   to_global(a.sh_node), to_global(a.sh_off), to_global(a.sh_slot), to_global(a.sh_first4);
-  to_global(a.partials), to_global(a.xyzm), to_global(a.U), to_global(a.Un), to_global(a.Unp);
-  to_global(a.Fprev), to_global(a.node_rec), to_global(a.rec), to_global(a.row_off), to_global(a.rows);
+  to_global(a.partials), to_global(a.sh_m), to_global(a.sh_rec), to_global(a.U), to_global(a.Un);
+  to_global(a.Unp), to_global(a.Fprev), to_global(a.rec), to_global(a.row_off), to_global(a.rows);
   to_global(a.err), to_global(a.halt), to_global(a.nblk), to_global(a.fold_row), to_global(a.fold_prev);
   to_global(a.eblk), to_global(a.fold_count);
 }
@@ -1635,12 +1655,12 @@ femx_status supg_union_upload(femx_supg_union *u) {
     f.sh_slot = b.d_sh_slot.p;
     f.sh_first4 = b.d_sh_first4.p;
     f.partials = h->partials.p;
-    f.xyzm = h->xyzm.p;
+    f.sh_m = h->sh_m.p;
+    f.sh_rec = h->sh_rec.p;
     f.U = nullptr;
     f.Un = nullptr;
     f.Unp = nullptr;
     f.Fprev = h->Fprev.p;
-    f.node_rec = h->bc.node_rec.p;
     f.rec = h->bc.rec.p;
     f.row_off = h->bc.row_off.p;
     f.rows = h->bc.rows.p;

@@ -1503,7 +1503,7 @@ struct SupgDyn {
 
 struct SupgMember {
   SupgArgs<float> a;  // per step: U, Upress, Un, k.t, step_id, prev_row
-  SupgFix f;          // per step: U, Un, k.t, fold_row, fold_prev
+  SupgFix f;          // per step: U, Un, k.t (the tick folds the log row)
   double4 *Ub[2];
   double *ring;
   SupgDyn *dyn;

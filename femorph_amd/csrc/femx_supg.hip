@@ -1752,8 +1752,8 @@ extern "C" femx_status femx_supg_union_add(femx_supg_union *u, femx_supg *h, int
   if (!u || !h || steps < 0) return set_error("bad argument"), FEMX_ERR_ARG;
   if (h->in_union) return set_error("handle already in a union"), FEMX_ERR_ARG;
   if (!union_member_ok(h))
-    return set_error("union members: FEMX_MIXED hexa8 with B-hat, no fixed DOFs, no diagnostics, at most 2048 "
-                     "element blocks and fix-up workgroups together, some shared nodes"),
+    return set_error("union members: FEMX_MIXED hexa8 with B-hat, no fixed DOFs, no diagnostics, more than one "
+                     "element block"),
            FEMX_ERR_UNSUPPORTED;
   if (h->loop_time < 1) return set_error("union members run their first step on their own"), FEMX_ERR_ARG;
   if (h->halted) return set_error("the exit condition already stopped this solve"), FEMX_EXIT_CONDITION;

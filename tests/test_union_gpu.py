@@ -161,3 +161,41 @@ def test_supg_union_members_equal_their_own_runs(gpu):
                 np.testing.assert_array_equal(g.get_log(), ref_log)
                 del live[i]
     assert rounds > 3
+
+
+def test_supg_union_rejects_non_members(gpu):
+    """A SUPG union takes FEMX_MIXED hexa8 handles with B-hat, no fixed DOFs
+    and no diagnostics, after their first step; a member is not stepped or
+    read on its own until it leaves."""
+    case = _supg_member(gpu, 6, 1, 100.0)
+    dm, g = _supg_solver(gpu, case)
+    u = gpu.SUPGUnion()
+    with pytest.raises(gpu.FemxError, match="first step"):
+        u.add(g, 3)
+    g.step(1)
+    u.add(g, 3)
+    with pytest.raises(gpu.FemxError, match="in a union"):
+        g.step(1)
+    with pytest.raises(gpu.FemxError, match="in a union"):
+        g.get_state()
+    u.step(3)
+    u.remove(g)
+    assert g.finish() == 3  # the union steps count as its own enqueued steps
+    assert np.isfinite(g.get_state()["sol"]).all()
+    m, bc, p = case
+    others = [gpu.SUPGSolver(dm, bc, precision=gpu.FEMX_FP64, **p),
+              gpu.SUPGSolver(dm, bc, precision=gpu.FEMX_MIXED, fixed_dofs="Z", **p),
+              gpu.SUPGSolver(dm, bc, precision=gpu.FEMX_MIXED, exopts=(0, 0, 1, 1, 1, 1), **p)]
+    mt = meshgen.tet_cube(4, 0.1, 2)
+    dmt = gpu.DeviceMesh(mt)
+    others.append(gpu.SUPGSolver(dmt, meshgen.lid_cavity_bc(mt), precision=gpu.FEMX_MIXED,
+                                 exopts=(0, 0, 1, 1, 1, 1), **p))
+    gd = gpu.SUPGSolver(dm, bc, precision=gpu.FEMX_MIXED, **p)
+    gd.set_diagnostics(True)
+    others.append(gd)
+    for h in others:
+        h.set_state(np.zeros((h.N, 3)))
+        h.step(1)
+        assert not gpu.load().femx_supg_union_accepts(h._h)
+        with pytest.raises(gpu.FemxError, match="union members"):
+            u.add(h, 2)

@@ -46,9 +46,12 @@ def test_initial_state(gpu):
     assert not st["pres"].any() and not st["lam"].any()
 
 
+@pytest.mark.parametrize("n", [5, 7])
 @pytest.mark.parametrize("exopts", [(0, 1, 1, 1, 1, 1), (1, 0, 1, 1, 0, 0), (0, 0, 1, 1, 1, 1), (1, 1, 0, 1, 0, 1)])
-def test_hexa_per_step_parity(gpu, exopts):
-    m, bc, params, ic = _case("hexa", 5, 0.2, seed=2)
+def test_hexa_per_step_parity(gpu, exopts, n):
+    """n 5: one element block; n 7: three blocks, so block-boundary nodes go
+    through the partial sums and the fix-up."""
+    m, bc, params, ic = _case("hexa", n, 0.2, seed=2)
     o, g, _ = _pair(gpu, m, bc, params, ic, exopts)
     for step in range(12):
         assert o.step() == 0

@@ -113,8 +113,9 @@ def test_fixed_dofs(gpu, fixed):
         assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
 
 
-def test_log_columns(gpu):
-    m, bc, params, ic = _case("hexa", 5, 0.2, seed=7)
+@pytest.mark.parametrize("n", [5, 7])
+def test_log_columns(gpu, n):
+    m, bc, params, ic = _case("hexa", n, 0.2, seed=7)
     o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1))
     g.set_diagnostics(True)
     o.step(3)

@@ -109,3 +109,21 @@ def test_element_order_blocks_follow_numbering():
     assert a == b
     e = femx.block_stats(r, 256, femx.BLOCKS_ELEMENT_ORDER)
     assert e["local_nodes"] > 3 * a["local_nodes"] and e["max_local"] <= 256
+
+
+def test_bad_meshes_rejected():
+    """An empty mesh and connectivity outside the nodes are refused by the
+    host checks, before anything is allocated (no GPU needed)."""
+    from femorph_amd.mesh import Mesh
+    m = meshgen.tet_cube(2)
+    bad = Mesh(m.coords, m.conn.copy())
+    bad.conn[3, 2] = m.n_nodes
+    with pytest.raises(femx.FemxError, match="out of range"):
+        femx.block_stats(bad, 256)
+    neg = Mesh(m.coords, m.conn.copy())
+    neg.conn[0, 0] = -1
+    with pytest.raises(femx.FemxError, match="out of range"):
+        femx.block_stats(neg, 256)
+    empty = Mesh(m.coords, m.conn[:0].copy())
+    with pytest.raises(femx.FemxError):
+        femx.block_stats(empty, 256)

@@ -113,6 +113,22 @@ def test_fixed_dofs(gpu, fixed):
         assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-11, step
 
 
+@pytest.mark.parametrize("fixed", ["Z", "XY"])
+def test_fixed_dofs_mixed(gpu, fixed):
+    """FEMX_MIXED with fixed DOFs (the kernel variant that keeps the state
+    before the zeroing for the pressure) vs the FP64 oracle after 20 steps;
+    the zeroed components stay exactly zero."""
+    m, bc, params, ic = _case("hexa", 6, 0.15, seed=12)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), fixed=fixed, precision=1)
+    o.step(20)
+    assert g.step(20) == 20
+    st = g.get_state()
+    for c in FIXED[fixed]:
+        assert not st["sol"][:, c].any()
+    assert rel_l2(st["sol"], o.get("sol_n")) <= 1e-5
+    assert rel_l2(st["pres"], o.get("pres_n")) <= 1e-4
+
+
 @pytest.mark.parametrize("n", [5, 7])
 def test_log_columns(gpu, n):
     m, bc, params, ic = _case("hexa", n, 0.2, seed=7)

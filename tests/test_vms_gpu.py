@@ -223,6 +223,27 @@ def test_element_diagnostics(gpu, tot_dual):
     assert rel_l2(sol, o.get("sol_n")) <= TOL64
 
 
+def test_element_diagnostics_mixed(gpu):
+    """The same diagnostics with FP32 element math (the drop-in's --mixed
+    solve): res_v / res_p and the printed norms within 1e-6 of the oracle.
+    subgrid_u is bounded against the scale of the subgrid velocities: once
+    the orthogonal projection has acted, its Gauss-point mean is zero up to
+    rounding (about 1e-17 in FP64, FP32 rounding here)."""
+    m, bc, params, ic = _case("channel", 5, 0.15, seed=5)
+    sdus = np.random.default_rng(3).uniform(-1e-3, 1e-3, (12, m.n_elems))
+    o, g, _ = _pair(gpu, m, bc, params, ic, precision=1, sdus=sdus)
+    g.set_diagnostics(True)
+    for step in range(4):
+        o.step()
+        g.step()
+        d = g.get_diagnostics()
+        assert rel_l2(d["res_v"], o.get("el_u_res")) <= 1e-6, step
+        assert rel_l2(d["res_p"], o.get("el_p_res")) <= 1e-6, step
+        scale = np.abs(o.get("sdus")).max()
+        assert np.abs(d["subgrid_u"] - o.get("el_u_sdus")).max() <= 1e-6 * scale, step
+        np.testing.assert_allclose(d["iter_norms"][-1], o.get("scalars")[2:5], rtol=1e-6, atol=0)
+
+
 def test_norms_ring_without_fields(gpu):
     """Diagnostics mode 2 (norms only): the per-iteration norms of 70 steps
     stay on the device and femx_vms_get_norms returns the last 64, oldest

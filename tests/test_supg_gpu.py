@@ -141,6 +141,20 @@ def test_log_columns(gpu, n):
     np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-12, atol=0)
 
 
+def test_log_columns_mixed(gpu):
+    """The log row with FP32 element math (a mixed drop-in solve) against
+    the FP64 oracle: step and time exact, the other columns within 1e-5."""
+    m, bc, params, ic = _case("hexa", 7, 0.2, seed=7)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 1, 1, 1, 1, 1), precision=1)
+    g.set_diagnostics(True)
+    for step in range(3):
+        o.step()
+        g.step()
+        lo, lg = o.get("log"), g.get_log()
+        assert lg[0] == lo[0] and lg[1] == lo[1]
+        np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-5, atol=0, err_msg=f"step {step}")
+
+
 def test_log_lcfl_running_max(gpu):
     """Column 14 (l-CFL) is a running maximum since step 1: the reference sets
     l_CFL_max once before the loop (femSUPGExplicitFluidSolver.cpp:996) and

@@ -155,6 +155,20 @@ def test_log_columns_mixed(gpu):
         np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-5, atol=0, err_msg=f"step {step}")
 
 
+def test_log_columns_tet4(gpu):
+    """The log row on tet4 (femTetra4 kinematics, no B-hat) against the
+    oracle, FP64, 1e-12."""
+    m, bc, params, ic = _case("tet", 4, 0.1, seed=9)
+    o, g, _ = _pair(gpu, m, bc, params, ic, (0, 0, 1, 1, 1, 1))
+    g.set_diagnostics(True)
+    for step in range(4):
+        o.step()
+        g.step()
+        lo, lg = o.get("log"), g.get_log()
+        assert lg[0] == lo[0] and lg[1] == lo[1]
+        np.testing.assert_allclose(lg[2:], lo[2:], rtol=1e-12, atol=0, err_msg=f"step {step}")
+
+
 def test_log_lcfl_running_max(gpu):
     """Column 14 (l-CFL) is a running maximum since step 1: the reference sets
     l_CFL_max once before the loop (femSUPGExplicitFluidSolver.cpp:996) and

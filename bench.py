@@ -443,9 +443,10 @@ def workload_serve(args):
 
 def workload_c1(args):
     """C1: VMS tet4 20^3 x 6 lid cavity, Re 100 (48,000 tets).  It moves
-    about 6 MB per step, so its step is bound by launch latency, not HBM:
-    the single handle's launches, and the same mesh as a batch of one
-    replayed from a HIP graph.  CPU: the oracle over all 100 steps."""
+    about 6 MB per step and its 188 blocks fill a small part of the chip,
+    so its step is bound by the latency of one block, not by HBM: the
+    single handle's launches, and the same mesh as a batch of one replayed
+    from a HIP graph.  CPU: the oracle over all 100 steps."""
     m = meshgen.tet_cube(20)
     bc = meshgen.lid_cavity_bc(m)
     p = dict(rho=1.0, mu=0.01, inv_eps=1.0, c1=4.0, c2=2.0, dt=0.1 * meshgen.min_tet_h(m))
@@ -489,7 +490,7 @@ def workload_c1(args):
            "data": "synthetic (seeded Kuhn-split cube, lid cavity)",
            "config": {"workload": f"C1: VMS tet4 20^3x6 lid cavity Re100 ({E} tets, {m.n_nodes} nodes)",
                       "elements": E, "nodes": m.n_nodes},
-           "roofline": {"bound": "launch latency", "note": "the step's compulsory bytes "
+           "roofline": {"bound": "latency of one block", "note": "the step's compulsory bytes "
                         f"({step_bytes / 1e6:.1f} MB) take {step_bytes / (HBM_PEAK_GBS * 1e9) * 1e6:.2f} us at "
                         "the HBM peak; not judged against HBM"},
            "fp64": res["fp64"], "mixed_kernels_us": main_["kernels_us"],
